@@ -1,0 +1,217 @@
+/*
+ * tnhip.h — C ABI of libtnhip.so, the MI355X (gfx950) MPS contraction library.
+ *
+ * Conventions
+ *   - Every entry point returns int: 0 = TNH_OK, < 0 = error code. The message of
+ *     the last error on a context is returned by tnh_last_error(ctx). Nothing aborts
+ *     and no C++ exception crosses this boundary.
+ *   - Tensors live on the device inside tnh_buf handles owned by a context. Their
+ *     dims are column-major (first index fastest), exactly as Julia passes arrays.
+ *   - Complex tensors are stored planar on the device: all real parts, then all
+ *     imaginary parts. Host transfers accept interleaved complex (Julia ComplexF64).
+ *   - Host pointers are borrowed for the duration of a call only.
+ *   - One context = one HIP stream. Calls on one context are serialised by the
+ *     caller; separate contexts may be used from separate threads.
+ *   - MPS site tensors are (chi_left, d, chi_right); MPO site tensors are
+ *     (D_left, d_out(bra), d_in(ket), D_right); environment blocks are
+ *     (chi_bra, D, chi_ket).
+ *
+ * Reference citations are to lcauser/TensorNetworks.jl (file:line).
+ */
+#ifndef TNHIP_H
+#define TNHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tnh_ctx tnh_ctx;
+typedef struct tnh_buf tnh_buf;
+typedef struct tnh_dmrg tnh_dmrg;
+typedef struct tnh_tebd tnh_tebd;
+typedef struct tnh_qjmc tnh_qjmc;
+
+/* Element types. Device storage of C128/C64 is planar. */
+enum {
+  TNH_F64 = 0,
+  TNH_F32 = 1,
+  TNH_C128 = 2,
+  TNH_C64 = 3
+};
+
+/* Arithmetic used by the matrix-core contractions. Accumulation is always at
+ * least FP32; TNH_PREC_F64 runs FP64 MFMA end to end. */
+enum {
+  TNH_PREC_F64 = 0,
+  TNH_PREC_F32 = 1,
+  TNH_PREC_BF16X3 = 2
+};
+
+enum {
+  TNH_OK = 0,
+  TNH_ERR_ARG = -1,
+  TNH_ERR_HIP = -2,
+  TNH_ERR_SHAPE = -3,
+  TNH_ERR_NOMEM = -4,
+  TNH_ERR_NUMERIC = -5,
+  TNH_ERR_NOCONV = -6,
+  TNH_ERR_UNSUPPORTED = -7
+};
+
+/* ------------------------------------------------------------------ context */
+
+/* No reference counterpart (the reference is one CPU process): creates a HIP
+ * stream and workspace on `device`. */
+int tnh_ctx_create(int device, tnh_ctx **out);
+int tnh_ctx_destroy(tnh_ctx *ctx);
+/* Message of the last failing call on ctx ("" if none). Valid until the next call. */
+const char *tnh_last_error(tnh_ctx *ctx);
+int tnh_ctx_synchronize(tnh_ctx *ctx);
+/* Library version string, e.g. "tnhip 0.1 gfx950". */
+const char *tnh_version(void);
+
+/* ------------------------------------------------------------------ buffers */
+
+/* Allocate an uninitialised device tensor. ndim <= 8. */
+int tnh_buf_create(tnh_ctx *ctx, int dtype, int ndim, const int64_t *dims, tnh_buf **out);
+/* Allocate a device tensor of `dtype` and fill it from host memory of
+ * `host_dtype` (converting precision; interleaved -> planar for complex; a real
+ * device dtype from a complex host array keeps the real part and fails with
+ * TNH_ERR_NUMERIC if any imaginary part is non-zero). */
+int tnh_buf_upload(tnh_ctx *ctx, const void *host, int host_dtype, int dtype, int ndim,
+                   const int64_t *dims, tnh_buf **out);
+/* Copy a device tensor to host memory of `host_dtype` (converting). */
+int tnh_buf_download(tnh_ctx *ctx, const tnh_buf *buf, void *host, int host_dtype);
+/* Wrap caller-owned device memory (not freed by the library). */
+int tnh_buf_wrap(tnh_ctx *ctx, void *device_ptr, int dtype, int ndim, const int64_t *dims,
+                 tnh_buf **out);
+int tnh_buf_free(tnh_ctx *ctx, tnh_buf *buf);
+/* dims must hold 8 entries. */
+int tnh_buf_shape(const tnh_buf *buf, int *dtype, int *ndim, int64_t *dims);
+void *tnh_buf_device_ptr(const tnh_buf *buf);
+/* Device-to-device copy with dtype conversion into a new buffer. */
+int tnh_buf_convert(tnh_ctx *ctx, const tnh_buf *src, int dtype, tnh_buf **out);
+
+/* ------------------------------------------------------- hot-path operators */
+
+/* Pairwise contraction primitive: column-major matrix product on device buffers,
+ *   C = alpha * op(A) * op(B) + beta * C,   op in {0: N, 1: T, 2: conjugate transpose},
+ * with A, B, C viewed as matrices of the given leading dimensions (elements).
+ * Replaces the permute-plus-GEMM lowering of contract(x, y, idxs1, idxs2, conjx, conjy),
+ * src/tensors.jl:13-18, for contractions whose index groups are already contiguous.
+ * Real or planar-complex buffers (C complex when A or B is); alpha, beta real. */
+int tnh_gemm(tnh_ctx *ctx, int opA, int opB, int64_t M, int64_t N, int64_t K, double alpha,
+             const tnh_buf *A, int64_t lda, const tnh_buf *B, int64_t ldb, double beta,
+             tnh_buf *C, int64_t ldc, int precision);
+
+/* H_eff matvec of two-site DMRG:
+ *   out[a,o1,o2,b'] = coeff * sum L[a,w,a'] W1[w,o1,i1,w1] W2[w1,o2,i2,w2] R[b',w2,b] x[a',i1,i2,b]
+ * Replaces product(::ProjMPS, A, direction, nsites=2), src/structures/mps/projmps.jl:103-144
+ * (rank-2, non-squared branch :107-134, coefficient :144). `out` must have x's shape
+ * and dtype. Evaluated as the GEMM chain L*x -> (W1 W2) -> *R^T in the given precision. */
+int tnh_heff_apply(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *W1, const tnh_buf *W2,
+                   const tnh_buf *R, const tnh_buf *x, tnh_buf *out, double coeff_re,
+                   double coeff_im, int precision);
+
+/* Left environment update L'[b,w',b'] = sum L[a,w,a'] conj(A[a,s,b]) W[w,s,t,w'] A[a',t,b'].
+ * Replaces buildleft!(::ProjMPS, idx), src/structures/mps/projmps.jl:50-66.
+ * *Lout is (re)allocated when NULL or of the wrong shape. Pass Lprev = NULL for the
+ * edge block ones(1,1,1) (src/structures/mps/abstractprojmps.jl:33-36). */
+int tnh_env_left(tnh_ctx *ctx, const tnh_buf *Lprev, const tnh_buf *A, const tnh_buf *W,
+                 tnh_buf **Lout, int precision);
+/* Right environment update R'[a,w,a'] = sum A[a',t,b'] R[b,w',b'] W[w,s,t,w'] conj(A[a,s,b]).
+ * Replaces buildright!(::ProjMPS, idx), src/structures/mps/projmps.jl:74-95. */
+int tnh_env_right(tnh_ctx *ctx, const tnh_buf *Rprev, const tnh_buf *A, const tnh_buf *W,
+                  tnh_buf **Rout, int precision);
+/* Fully contracted <A|L W R|A> at one site. Replaces calculate(::ProjMPS),
+ * src/structures/mps/projmps.jl:192-216. NULL L or R means the edge block. */
+int tnh_env_calculate(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *A, const tnh_buf *W,
+                      const tnh_buf *R, double coeff, double *re, double *im);
+
+/* Lowest eigenpair of H_eff by restarted Lanczos (Krylov-Schur, full
+ * re-orthogonalisation, FP64 vectors). Replaces the KrylovKit call
+ * eigsolve(Heff, A0, 1, :SR; maxiter, krylovdim, ishermitian=true, tol) at
+ * src/algorithms/mps/dmrg.jl:50-53. `theta` holds A0 on entry and the normalised
+ * Ritz vector on exit (dtype F64 or C128). The matvecs use `precision`. */
+int tnh_eigsolve_ground(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *W1, const tnh_buf *W2,
+                        const tnh_buf *R, tnh_buf *theta, int krylovdim, int maxiter,
+                        double tol, double coeff, int precision, double *energy,
+                        int *nmatvec);
+
+/* Truncated SVD of a 2-D device matrix M (rows x cols) with the truncation rule of
+ * svd(x, idx; cutoff, maxdim, mindim), src/tensors.jl:168-227 (:202-215):
+ * maxdim = 0 means unlimited; k = max(min(maxdim, k_cut), min(mindim, n)) >= 1 where
+ * k_cut is the smallest k whose discarded weight sum_{i>k} s_i^2 / sum s_i^2 <= cutoff.
+ * Primary path: FP64 Gram matrix + Jacobi eigensolver; fallback (on non-convergence
+ * or non-finite values, mirroring gesdd -> gesvd at src/tensors.jl:190-195):
+ * one-sided Jacobi on M. Outputs U (rows x k), S (k, F64), Vt (k x cols), each
+ * (re)allocated. `discarded` receives sum_{i>k} s_i^2 / sum s_i^2, `fallback` 1 if
+ * the fallback ran. Any of U/S/Vt may be NULL when not wanted. force_fallback != 0
+ * runs the fallback directly. */
+int tnh_svd_trunc(tnh_ctx *ctx, const tnh_buf *M, double cutoff, int maxdim, int mindim,
+                  int force_fallback, tnh_buf **U, tnh_buf **S, tnh_buf **Vt, int *k,
+                  double *discarded, int *fallback);
+
+/* Split a two-site tensor theta (chi_l, d, d, chi_r) into two site tensors.
+ * Replaces replacesites!(psi, A, site, direction, normalize; cutoff, maxdim, mindim),
+ * src/structures/mps/gmps.jl:199-267, for nsites = 2, rank 1:
+ *   direction = 0 (sweeping right): A1 (chi_l,d,k) left-orthonormal, A2 = S*V (k,d,chi_r);
+ *   direction = 1 (sweeping left):  A1 = U*S (chi_l,d,k), A2 (k,d,chi_r) right-orthonormal.
+ * normalize != 0 rescales the centre tensor to unit norm (gmps.jl:263-265).
+ * If schmidt != NULL, up to schmidt_cap kept singular values are written there. */
+int tnh_replacesites2(tnh_ctx *ctx, const tnh_buf *theta, int direction, int normalize,
+                      double cutoff, int maxdim, int mindim, tnh_buf **A1, tnh_buf **A2,
+                      int *k, double *discarded, double *schmidt, int schmidt_cap);
+
+/* theta (chi_l, d, d, chi_r) = A1 (chi_l,d,chi) * A2 (chi,d,chi_r): contract(A1, A2, 3, 1),
+ * src/algorithms/mps/dmrg.jl:44-47 and src/structures/mps/gatelist.jl:142-145. */
+int tnh_two_site_theta(tnh_ctx *ctx, const tnh_buf *A1, const tnh_buf *A2, tnh_buf **theta);
+
+/* Two-site gate application followed by the truncated split (no normalisation).
+ * Replaces applygate(psi, site, gate, direction; cutoff, maxdim, mindim),
+ * src/structures/mps/gatelist.jl:137-171 (rng = 2, rank 1):
+ *   theta'[a,o1,o2,b] = sum G[o1,i1,o2,i2] theta[a,i1,i2,b]   (G is (d,d,d,d)).
+ * A1 and A2 are replaced in place by the split factors. */
+int tnh_applygate2(tnh_ctx *ctx, tnh_buf **A1, tnh_buf **A2, const tnh_buf *G, int direction,
+                   double cutoff, int maxdim, int mindim, int *k, double *discarded);
+
+/* ---------------------------------------------------- device-resident DMRG */
+
+typedef struct tnh_dmrg_params {
+  int krylovdim;  /* dmrg.jl:4 default 3 */
+  int maxiter;    /* dmrg.jl:5 default 2 */
+  double lanczos_tol; /* dmrg.jl:53 (1e-14) */
+  double cutoff;  /* dmrg.jl:17 default 1e-12 */
+  int maxdim;     /* dmrg.jl:18 default 1000 */
+  int mindim;     /* dmrg.jl:19 default 1 */
+  int precision;  /* TNH_PREC_* for the matvecs and environment updates */
+} tnh_dmrg_params;
+
+/* Device-resident state of dmrg(psi, H::GMPS...), src/algorithms/mps/dmrg.jl:128-154:
+ * copies the N site tensors of psi (any dtype; stored F64 or C128) and the nterms
+ * MPOs (site tensors mpo[t*N + i]) with coefficients coeffs[t]; moves the centre of
+ * psi to site 1 and builds all right blocks (projmps.jl:16-42). */
+int tnh_dmrg_create(tnh_ctx *ctx, int N, const tnh_buf *const *psi, int nterms,
+                    const tnh_buf *const *mpo, const double *coeffs, tnh_dmrg **out);
+int tnh_dmrg_destroy(tnh_dmrg *st);
+/* One directional pass (the reference's "sweep", dmrg.jl:35-63): for each bond,
+ * move the environment centre, form theta, eigsolve, replacesites!. direction 0 =
+ * left-to-right. Returns the last Ritz value, the max bond dim and the max
+ * discarded weight of the pass. */
+int tnh_dmrg_sweep(tnh_dmrg *st, int direction, const tnh_dmrg_params *p, double *energy,
+                   int *maxbonddim, double *max_discarded);
+/* Current energy <psi|H|psi> at the centre: calculate(Hs), dmrg.jl:22. */
+int tnh_dmrg_energy(tnh_dmrg *st, int precision, double *energy);
+/* Borrow site tensor `site` (1-based). Valid until the next sweep. */
+int tnh_dmrg_site(tnh_dmrg *st, int site, const tnh_buf **out);
+int tnh_dmrg_center(tnh_dmrg *st, int *center);
+/* Bond dims chi_0..chi_N into dims[N+1]. */
+int tnh_dmrg_bonddims(tnh_dmrg *st, int64_t *dims);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TNHIP_H */

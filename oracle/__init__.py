@@ -1,0 +1,19 @@
+"""CPU FP64 oracle for the MPS contraction hot path — TEST INFRASTRUCTURE ONLY.
+
+This package restates, in numpy/LAPACK FP64, the algorithms of lcauser/TensorNetworks.jl
+that the HIP library replaces (each function cites the reference file:line it
+follows). It is imported only by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, as the checker — never by the product library.
+
+The reference is Julia and cannot run here (no Julia toolchain; its dependencies
+TensorOperations.jl, KrylovKit.jl are absent), and its test suite holds no golden
+vectors. Each oracle function is therefore pinned by checks the mathematics fixes:
+brute-force dense contractions for tiny bond dimension, equality of the reference
+contraction order with the optimal order, Hermiticity, LAPACK singular values,
+dense exact diagonalisation and closed forms (critical transverse-field Ising:
+E0(N) = 1 - csc(pi/(4N+2))). See tests/test_oracle.py.
+
+Functions with no such pin say "parity unpinned" in their docstring:
+  * krylov.eigsolve_ground's per-iteration Ritz sequence (KrylovKit's exact restart
+    sequence is version-dependent); only its converged eigenpair is pinned by ED.
+"""

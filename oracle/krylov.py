@@ -1,0 +1,75 @@
+"""Restarted Lanczos for the lowest eigenpair, standing in for KrylovKit's
+eigsolve(Heff, A0, 1, :SR; maxiter, krylovdim, ishermitian=true, tol) call at
+src/algorithms/mps/dmrg.jl:51-53.
+
+KrylovKit.jl is a third-party dependency absent from the reference tree (no pinned
+version: Project.toml lists UUIDs only, Manifest.toml is git-ignored). Its published
+algorithm for Hermitian problems is a Lanczos factorisation with full
+re-orthogonalisation, restarted Krylov-Schur style keeping the wanted Ritz vector
+(keep = div(3*krylovdim + 2*converged, 5) = 1 vector for krylovdim <= 3 and one
+wanted pair). This restatement keeps exactly one Ritz vector per restart, so one
+cycle costs krylovdim matvecs first and krylovdim - 1 after each restart.
+
+Parity unpinned for the per-iteration Ritz sequence (version-dependent); the
+converged eigenpair is pinned by dense diagonalisation (tests/test_oracle.py).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def eigsolve_ground(matvec, x0, krylovdim=3, maxiter=2, tol=1e-14):
+    """Return (energy, normalised Ritz vector, number of matvecs)."""
+    shape = x0.shape
+    x = x0.reshape(-1, order="F").astype(np.result_type(x0, np.float64), copy=True)
+    n = x.size
+    kdim = max(1, min(krylovdim, n))
+    V = np.zeros((kdim + 1, n), dtype=x.dtype)
+    H = np.zeros((kdim + 1, kdim), dtype=x.dtype)
+    V[0] = x / np.linalg.norm(x)
+    start = 0
+    nmv = 0
+
+    def mv(v):
+        return matvec(v.reshape(shape, order="F")).reshape(-1, order="F")
+
+    theta, y, m = 0.0, None, 0
+    for it in range(maxiter):
+        m = kdim
+        breakdown = False
+        for j in range(start, kdim):
+            w = mv(V[j])
+            nmv += 1
+            for _ in range(2):  # classical Gram-Schmidt, twice
+                c = V[: j + 1].conj() @ w
+                H[: j + 1, j] += c
+                w = w - c @ V[: j + 1]
+            beta = np.linalg.norm(w)
+            H[j + 1, j] = beta
+            scale = max(1.0, np.abs(H[: j + 2, : j + 1]).max())
+            if beta <= 1e-14 * scale:
+                m = j + 1
+                breakdown = True
+                break
+            V[j + 1] = w / beta
+        Hm = H[:m, :m]
+        Hm = 0.5 * (Hm + Hm.conj().T)
+        evals, evecs = np.linalg.eigh(Hm)
+        theta, y = float(evals[0]), evecs[:, 0]
+        b = H[m, :m] @ y  # residual coefficient on V[m]
+        res = abs(b)
+        if breakdown or res <= tol or it == maxiter - 1:
+            break
+        # restart: keep the Ritz vector and the residual direction
+        ritz = y @ V[:m]
+        V2 = np.zeros_like(V)
+        V2[0] = ritz / np.linalg.norm(ritz)
+        V2[1] = V[m]
+        V = V2
+        H = np.zeros_like(H)
+        H[0, 0] = theta
+        H[1, 0] = b
+        start = 1
+    ritz = y @ V[:m]
+    ritz = ritz / np.linalg.norm(ritz)
+    return theta, ritz.reshape(shape, order="F"), nmv

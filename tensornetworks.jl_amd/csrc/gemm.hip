@@ -1,0 +1,108 @@
+// GEMM dispatch: picks the arithmetic mode and tile shape, and builds planar-complex
+// products from real MFMA GEMMs.
+#include "gemm_kernel.hpp"
+#include "tnh_internal.hpp"
+
+namespace tnh {
+namespace gk {
+void launch_f64(hipStream_t, const Params &, int, bool, bool, int, int, int, bool);
+void launch_f32(hipStream_t, const Params &, int, bool, bool, int, int, int, bool);
+void launch_bf16x3(hipStream_t, const Params &, int, bool, bool, int, int, int, bool);
+}  // namespace gk
+
+void gemm(tnh_ctx *ctx, const Gemm &g, int prec) {
+  if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return;
+  TNH_REQUIRE(g.K >= 0, TNH_ERR_ARG, "gemm: negative K");
+  TNH_REQUIRE(g.ldc >= g.M, TNH_ERR_ARG, "gemm: ldc < M");
+  TNH_REQUIRE(g.K == 0 || (g.transA ? g.lda >= g.K : g.lda >= g.M), TNH_ERR_ARG, "gemm: bad lda");
+  TNH_REQUIRE(g.K == 0 || (g.transB ? g.ldb >= g.N : g.ldb >= g.K), TNH_ERR_ARG, "gemm: bad ldb");
+  TNH_REQUIRE(g.batch <= 65535, TNH_ERR_ARG, "gemm: batch too large");
+  // FP64 arithmetic needs FP64 operands; lower-precision modes take either.
+  if (prec == TNH_PREC_F64)
+    TNH_REQUIRE(g.tA == TNH_F64 && g.tB == TNH_F64 && g.tC == TNH_F64, TNH_ERR_ARG,
+                "gemm: FP64 arithmetic with FP32 operands");
+  // Large tiles once the product fills the chip with 128x128 blocks.
+  const bool big = (int64_t((g.M + 127) / 128) * ((g.N + 127) / 128) * g.batch >= ctx->cu_count) &&
+                   g.M >= 96 && g.N >= 96;
+  const int bm = big ? 128 : 64, bn = big ? 128 : 64;
+  gk::Params p;
+  p.M = g.M;
+  p.N = g.N;
+  p.K = g.K;
+  p.A = g.A;
+  p.lda = g.lda;
+  p.sA = g.sA;
+  p.B = g.B;
+  p.ldb = g.ldb;
+  p.sB = g.sB;
+  p.C = g.C;
+  p.ldc = g.ldc;
+  p.sC = g.sC;
+  p.alpha = g.alpha;
+  p.beta = g.beta;
+  p.tiles_m = (g.M + bm - 1) / bm;
+  p.tiles_n = (g.N + bn - 1) / bn;
+  switch (prec) {
+    case TNH_PREC_F64:
+      gk::launch_f64(ctx->stream, p, g.batch, g.transA, g.transB, g.tA, g.tB, g.tC, big);
+      break;
+    case TNH_PREC_F32:
+      gk::launch_f32(ctx->stream, p, g.batch, g.transA, g.transB, g.tA, g.tB, g.tC, big);
+      break;
+    case TNH_PREC_BF16X3:
+      gk::launch_bf16x3(ctx->stream, p, g.batch, g.transA, g.transB, g.tA, g.tB, g.tC, big);
+      break;
+    default:
+      throw Error(TNH_ERR_ARG, "unknown precision " + std::to_string(prec));
+  }
+  TNH_HIP(hipGetLastError());
+}
+
+// Planar complex product from real products:
+//   Re C = a (Ar Br - sa sb Ai Bi) + b Re C,  Im C = a (sb Ar Bi + sa Ai Br) + b Im C
+// with sa = -1 when A is conjugated (likewise sb). Purely real operands drop terms.
+void cgemm(tnh_ctx *ctx, Gemm g, COperand A, COperand B, void *Cre, void *Cim, int tC, int prec) {
+  const double alpha = g.alpha, beta = g.beta;
+  const double sa = A.conj ? -1.0 : 1.0, sb = B.conj ? -1.0 : 1.0;
+  g.tA = A.dtype;
+  g.tB = B.dtype;
+  g.tC = tC;
+  auto run = [&](const void *a, const void *b, void *c, double al, double be) {
+    Gemm h = g;
+    h.A = a;
+    h.B = b;
+    h.C = c;
+    h.alpha = al;
+    h.beta = be;
+    gemm(ctx, h, prec);
+  };
+  // real part
+  run(A.re, B.re, Cre, alpha, beta);
+  if (A.im && B.im) run(A.im, B.im, Cre, -sa * sb * alpha, 1.0);
+  if (!Cim) {
+    TNH_REQUIRE(!A.im && !B.im, TNH_ERR_ARG, "cgemm: complex operands need a complex output");
+    return;
+  }
+  // imaginary part
+  bool started = false;
+  if (B.im) {
+    run(A.re, B.im, Cim, sb * alpha, beta);
+    started = true;
+  }
+  if (A.im) {
+    run(A.im, B.re, Cim, sa * alpha, started ? 1.0 : beta);
+    started = true;
+  }
+  if (!started) {
+    // both operands real: the imaginary part of C becomes beta * Im C
+    TNH_REQUIRE(g.batch == 1 && g.ldc == g.M, TNH_ERR_ARG,
+                "cgemm: real operands with a strided complex output");
+    int64_t n = int64_t(g.M) * g.N;
+    if (beta == 0.0)
+      fill(ctx, Cim, real_of(tC), n, 0.0);
+    else
+      scale(ctx, Cim, real_of(tC), n, beta);
+  }
+}
+
+}  // namespace tnh

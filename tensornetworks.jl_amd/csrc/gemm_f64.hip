@@ -1,0 +1,35 @@
+// Instantiations of the F64 GEMM kernels and their launcher.
+#include "gemm_kernel.hpp"
+#include "tnh_internal.hpp"
+
+namespace tnh {
+namespace gk {
+
+template <typename TA, typename TB, typename TC, bool TRA, bool TRB, int BM, int BN>
+static void launch(hipStream_t s, const Params &p, int batch) {
+  dim3 grid(p.tiles_m * p.tiles_n, batch);
+  hipLaunchKernelGGL((gemm_kernel<MODE_F64, TA, TB, TC, TRA, TRB, BM, BN>), grid, dim3(256), 0, s, p);
+}
+
+template <typename TA, typename TB, typename TC, int BM, int BN>
+static void launch_tr(hipStream_t s, const Params &p, int batch, bool tra, bool trb) {
+  if (!tra && !trb) launch<TA, TB, TC, false, false, BM, BN>(s, p, batch);
+  else if (!tra && trb) launch<TA, TB, TC, false, true, BM, BN>(s, p, batch);
+  else if (tra && !trb) launch<TA, TB, TC, true, false, BM, BN>(s, p, batch);
+  else throw Error(TNH_ERR_UNSUPPORTED, "gemm: op(A)=T with op(B)=T is not instantiated");
+}
+
+template <int BM, int BN>
+static void launch_types(hipStream_t s, const Params &p, int batch, bool tra, bool trb, int tA, int tB, int tC) {
+  if (tA != TNH_F64 || tB != TNH_F64 || tC != TNH_F64)
+    throw Error(TNH_ERR_UNSUPPORTED, "FP64 GEMM needs FP64 operands");
+  launch_tr<double, double, double, BM, BN>(s, p, batch, tra, trb);
+}
+
+void launch_f64(hipStream_t s, const Params &p, int batch, bool tra, bool trb, int tA, int tB, int tC, bool big) {
+  if (big) launch_types<128, 128>(s, p, batch, tra, trb, tA, tB, tC);
+  else launch_types<64, 64>(s, p, batch, tra, trb, tA, tB, tC);
+}
+
+}  // namespace gk
+}  // namespace tnh

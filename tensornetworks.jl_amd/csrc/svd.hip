@@ -1,0 +1,913 @@
+// Truncated SVD of theta on the GPU (replaces svd(x, idx; ...) of src/tensors.jl:168-227 and
+// the two-site split of replacesites!, src/structures/mps/gmps.jl:199-267).
+//
+// Primary path: FP64 Gram matrix G = M M^H (or M^H M, the smaller side) on the matrix
+// cores, then a two-sided block-Jacobi eigensolver on G. Fallback (non-convergence,
+// non-finite values, or on request) -- mirroring gesdd -> gesvd at src/tensors.jl:190-195:
+// one-sided block-Jacobi (Hestenes) directly on M, which keeps relative accuracy of
+// small singular values.
+//
+// Block Jacobi: columns are cut into blocks of JB = 16; a round-robin (circle)
+// schedule pairs all blocks per round, and every pair (32 columns) is handled by one
+// workgroup that diagonalises its 32 x 32 sub-problem in LDS with a parallel cyclic
+// Jacobi (16 disjoint rotations per step, exact zeroing of the rotated entry). The
+// rotations are then applied to the full columns (and rows, two-sided) of G / M and
+// to the accumulated unitary V. Pairs already orthogonal to tolerance are skipped.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "tnh_internal.hpp"
+
+namespace tnh {
+
+namespace {
+
+constexpr int JB = 16;
+constexpr int JP = 2 * JB;
+constexpr int kGramRows = 64;
+constexpr int kMaxInner = 15;
+
+// circle-method pairing of n (even) players in round `step`, pair `i`
+__host__ __device__ inline void circle_pair(int n, int step, int i, int *p, int *q) {
+  if (i == 0) {
+    *p = 0;
+    *q = (step % (n - 1)) + 1;
+  } else {
+    *p = ((step + i) % (n - 1)) + 1;
+    *q = ((step - i + (n - 1)) % (n - 1)) + 1;
+  }
+}
+
+__device__ inline int pair_col(const int *pr, int k, int cols) {
+  const int blk = k < JB ? pr[0] : pr[1];
+  const int c = blk * JB + (k % JB);
+  return (blk >= 0 && c < cols) ? c : -1;
+}
+
+__device__ inline double block_max(double v) {
+  __shared__ double red[4];
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  __syncthreads();
+  return m;
+}
+
+// One-sided mode: partial Grams of the column pairs, split over row chunks.
+// partial[((pair * nsplit + split) * 2 + plane) * JP * JP + (i + JP * j)] = sum_r conj(x_ri) x_rj
+__global__ __launch_bounds__(256) void jac_gram_kernel(const double *__restrict__ X, int rows,
+                                                       int64_t imX, int cols, bool cplx,
+                                                       const int *__restrict__ pairs, int nsplit,
+                                                       double *__restrict__ partial) {
+  const int pair = blockIdx.x, split = blockIdx.y;
+  __shared__ double tr[kGramRows][JP + 1], ti[kGramRows][JP + 1];
+  __shared__ int colmap[JP];
+  const int t = threadIdx.x;
+  if (t < JP) colmap[t] = pair_col(pairs + 2 * pair, t, cols);
+  __syncthreads();
+  int chunk = (rows + nsplit - 1) / nsplit;
+  chunk = (chunk + kGramRows - 1) / kGramRows * kGramRows;
+  const int r0 = split * chunk, r1 = min(rows, r0 + chunk);
+  double ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
+  for (int rb = r0; rb < r1; rb += kGramRows) {
+    for (int e = t; e < kGramRows * JP; e += 256) {
+      const int rr = e % kGramRows, k = e / kGramRows;
+      const int r = rb + rr, c = colmap[k];
+      double vr = 0, vi = 0;
+      if (r < r1 && c >= 0) {
+        vr = X[r + int64_t(c) * rows];
+        if (cplx) vi = X[imX + r + int64_t(c) * rows];
+      }
+      tr[rr][k] = vr;
+      ti[rr][k] = vi;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = t + 256 * q, i = e % JP, j = e / JP;
+      double sr = 0, si = 0;
+      for (int rr = 0; rr < kGramRows; ++rr) {
+        const double xr = tr[rr][i], xi = ti[rr][i], yr = tr[rr][j], yi = ti[rr][j];
+        sr += xr * yr + xi * yi;
+        si += xr * yi - xi * yr;
+      }
+      ar[q] += sr;
+      ai[q] += si;
+    }
+    __syncthreads();
+  }
+  double *out = partial + int64_t(pair * nsplit + split) * 2 * JP * JP;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = t + 256 * q;
+    out[e] = ar[q];
+    out[JP * JP + e] = ai[q];
+  }
+}
+
+// Diagonalise the 32x32 Hermitian sub-problem of each pair. Two-sided mode reads
+// G[P,P] from the global matrix; one-sided mode sums the partial Grams. Writes the
+// accumulated unitary Q (and the diagonalised block D in two-sided mode), an
+// `active` flag (0 = pair already converged, nothing to apply) and the pair's
+// off-diagonal measure into conv (atomic max over the sweep).
+__global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__ G, int64_t imG,
+                                                      int n, const double *__restrict__ partial,
+                                                      int nsplit, bool two_sided, bool cplx,
+                                                      const int *__restrict__ pairs, int cols,
+                                                      double tol, double *__restrict__ Qout,
+                                                      double *__restrict__ Dout,
+                                                      int *__restrict__ active,
+                                                      unsigned long long *__restrict__ conv) {
+  const int pair = blockIdx.x, t = threadIdx.x;
+  __shared__ double gr[JP][JP + 1], gi[JP][JP + 1], qr[JP][JP + 1], qi[JP][JP + 1];
+  __shared__ int colmap[JP];
+  __shared__ int rp[JP / 2], rq[JP / 2], rrot[JP / 2];
+  __shared__ double rcs[JP / 2], rsn[JP / 2], rer[JP / 2], rei[JP / 2], ra[JP / 2],
+      rc[JP / 2], rtb[JP / 2];
+  __shared__ int any;
+  if (t < JP) colmap[t] = pair_col(pairs + 2 * pair, t, cols);
+  __syncthreads();
+  for (int e = t; e < JP * JP; e += 256) {
+    const int i = e % JP, j = e / JP;
+    double vr = 0, vi = 0;
+    if (two_sided) {
+      const int ci = colmap[i], cj = colmap[j];
+      if (ci >= 0 && cj >= 0) {
+        vr = G[ci + int64_t(cj) * n];
+        if (cplx) vi = G[imG + ci + int64_t(cj) * n];
+      }
+    } else {
+      for (int s = 0; s < nsplit; ++s) {
+        const double *pp = partial + int64_t(pair * nsplit + s) * 2 * JP * JP;
+        vr += pp[e];
+        vi += pp[JP * JP + e];
+      }
+    }
+    if (i == j) vi = 0;
+    gr[i][j] = vr;
+    gi[i][j] = vi;
+    qr[i][j] = (i == j) ? 1.0 : 0.0;
+    qi[i][j] = 0.0;
+  }
+  __syncthreads();
+  // off-diagonal measure: max |g_ij| / sqrt(|g_ii g_jj|) over i != j
+  double m = 0;
+  for (int e = t; e < JP * JP; e += 256) {
+    const int i = e % JP, j = e / JP;
+    if (i == j) continue;
+    const double b = hypot(gr[i][j], gi[i][j]);
+    if (b == 0) continue;
+    const double s = sqrt(fabs(gr[i][i] * gr[j][j]));
+    m = fmax(m, s > 0 ? b / s : 1.0);
+  }
+  m = block_max(m);
+  if (t == 0) atomicMax(conv, (unsigned long long)__double_as_longlong(m));
+  if (m <= tol) {
+    if (t == 0) active[pair] = 0;
+    return;
+  }
+  for (int sw = 0; sw < kMaxInner; ++sw) {
+    if (t == 0) any = 0;
+    __syncthreads();
+    for (int step = 0; step < JP - 1; ++step) {
+      if (t < JP / 2) {
+        int p, q;
+        circle_pair(JP, step, t, &p, &q);
+        const double a = gr[p][p], c = gr[q][q];
+        const double br = gr[p][q], bi = gi[p][q];
+        const double bb = hypot(br, bi);
+        const bool rot = bb > 0 && bb > tol * sqrt(fabs(a * c));
+        double cs = 1, sn = 0, tt = 0, er = 1, ei = 0;
+        if (rot) {
+          const double tau = (c - a) / (2 * bb);
+          tt = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1 + tau * tau));
+          cs = 1 / sqrt(1 + tt * tt);
+          sn = tt * cs;
+          er = br / bb;  // e^{-i phi} = conj(b) / |b|
+          ei = -bi / bb;
+          any = 1;
+        }
+        rp[t] = p;
+        rq[t] = q;
+        rrot[t] = rot;
+        rcs[t] = cs;
+        rsn[t] = sn;
+        rer[t] = er;
+        rei[t] = ei;
+        ra[t] = a;
+        rc[t] = c;
+        rtb[t] = tt * bb;
+      }
+      __syncthreads();
+      // columns: [g_p g_q] <- [g_p g_q] J,  J = [[cs, sn], [-sn e, cs e]]
+      for (int task = t; task < (JP / 2) * JP; task += 256) {
+        const int k = task / JP, r = task % JP;
+        if (!rrot[k]) continue;
+        const int p = rp[k], q = rq[k];
+        const double cs = rcs[k], sn = rsn[k], er = rer[k], ei = rei[k];
+        {
+          const double pr_ = gr[r][p], pi_ = gi[r][p], qr_ = gr[r][q], qi_ = gi[r][q];
+          const double eqr = er * qr_ - ei * qi_, eqi = er * qi_ + ei * qr_;
+          gr[r][p] = cs * pr_ - sn * eqr;
+          gi[r][p] = cs * pi_ - sn * eqi;
+          gr[r][q] = sn * pr_ + cs * eqr;
+          gi[r][q] = sn * pi_ + cs * eqi;
+        }
+        {
+          const double pr_ = qr[r][p], pi_ = qi[r][p], qr_ = qr[r][q], qi_ = qi[r][q];
+          const double eqr = er * qr_ - ei * qi_, eqi = er * qi_ + ei * qr_;
+          qr[r][p] = cs * pr_ - sn * eqr;
+          qi[r][p] = cs * pi_ - sn * eqi;
+          qr[r][q] = sn * pr_ + cs * eqr;
+          qi[r][q] = sn * pi_ + cs * eqi;
+        }
+      }
+      __syncthreads();
+      // rows: [g_p; g_q] <- J^H [g_p; g_q]
+      for (int task = t; task < (JP / 2) * JP; task += 256) {
+        const int k = task / JP, c = task % JP;
+        if (!rrot[k]) continue;
+        const int p = rp[k], q = rq[k];
+        const double cs = rcs[k], sn = rsn[k], er = rer[k], ei = -rei[k];  // conj(e)
+        const double pr_ = gr[p][c], pi_ = gi[p][c], qr_ = gr[q][c], qi_ = gi[q][c];
+        const double eqr = er * qr_ - ei * qi_, eqi = er * qi_ + ei * qr_;
+        gr[p][c] = cs * pr_ - sn * eqr;
+        gi[p][c] = cs * pi_ - sn * eqi;
+        gr[q][c] = sn * pr_ + cs * eqr;
+        gi[q][c] = sn * pi_ + cs * eqi;
+      }
+      __syncthreads();
+      if (t < JP / 2 && rrot[t]) {
+        const int p = rp[t], q = rq[t];
+        gr[p][q] = gi[p][q] = gr[q][p] = gi[q][p] = 0.0;
+        gr[p][p] = ra[t] - rtb[t];
+        gr[q][q] = rc[t] + rtb[t];
+        gi[p][p] = gi[q][q] = 0.0;
+      }
+      __syncthreads();
+    }
+    if (!any) break;
+  }
+  double *Q = Qout + int64_t(pair) * 2 * JP * JP;
+  for (int e = t; e < JP * JP; e += 256) {
+    const int i = e % JP, j = e / JP;
+    Q[e] = qr[i][j];
+    Q[JP * JP + e] = qi[i][j];
+  }
+  if (two_sided) {
+    double *D = Dout + int64_t(pair) * 2 * JP * JP;
+    for (int e = t; e < JP * JP; e += 256) {
+      const int i = e % JP, j = e / JP;
+      D[e] = gr[i][j];
+      D[JP * JP + e] = gi[i][j];
+    }
+  }
+  if (t == 0) active[pair] = 1;
+}
+
+// Y[:, P] <- Y[:, P] Q_P for Y in {X (rows_x rows), V (rows_v rows)}.
+__global__ __launch_bounds__(256) void jac_colupd_kernel(double *__restrict__ X, int rows_x,
+                                                         int64_t imX, double *__restrict__ V,
+                                                         int rows_v, int64_t imV, bool cplx,
+                                                         const int *__restrict__ pairs, int cols,
+                                                         const double *__restrict__ Qin,
+                                                         const int *__restrict__ active) {
+  const int pair = blockIdx.y;
+  if (!active[pair]) return;
+  __shared__ double qr[JP * JP], qi[JP * JP];
+  __shared__ int colmap[JP];
+  const int t = threadIdx.x;
+  const double *Q = Qin + int64_t(pair) * 2 * JP * JP;
+  for (int e = t; e < JP * JP; e += 256) {
+    qr[e] = Q[e];
+    qi[e] = Q[JP * JP + e];
+  }
+  if (t < JP) colmap[t] = pair_col(pairs + 2 * pair, t, cols);
+  __syncthreads();
+  const int r = blockIdx.x * 256 + t;
+  for (int which = 0; which < 2; ++which) {
+    double *Y = which ? V : X;
+    const int rows = which ? rows_v : rows_x;
+    const int64_t im = which ? imV : imX;
+    if (!Y || r >= rows) continue;
+    double yr[JP], yi[JP];
+#pragma unroll
+    for (int k = 0; k < JP; ++k) {
+      const int c = colmap[k];
+      yr[k] = c >= 0 ? Y[r + int64_t(c) * rows] : 0.0;
+      yi[k] = (c >= 0 && cplx) ? Y[im + r + int64_t(c) * rows] : 0.0;
+    }
+    for (int j = 0; j < JP; ++j) {
+      const int c = colmap[j];
+      if (c < 0) continue;
+      double sr = 0, si = 0;
+#pragma unroll
+      for (int k = 0; k < JP; ++k) {
+        const double a = qr[k + JP * j], b = qi[k + JP * j];
+        sr += yr[k] * a - yi[k] * b;
+        si += yr[k] * b + yi[k] * a;
+      }
+      Y[r + int64_t(c) * rows] = sr;
+      if (cplx) Y[im + r + int64_t(c) * rows] = si;
+    }
+  }
+}
+
+// Two-sided mode: G[P, :] <- Q_P^H G[P, :]; entries in P x P take the diagonalised block D.
+__global__ __launch_bounds__(256) void jac_rowupd_kernel(double *__restrict__ G, int n, int64_t imG,
+                                                         bool cplx, const int *__restrict__ pairs,
+                                                         const double *__restrict__ Qin,
+                                                         const double *__restrict__ Din,
+                                                         const int *__restrict__ active) {
+  const int pair = blockIdx.y;
+  if (!active[pair]) return;
+  __shared__ double qr[JP * JP], qi[JP * JP];
+  __shared__ int colmap[JP];
+  const int t = threadIdx.x;
+  const double *Q = Qin + int64_t(pair) * 2 * JP * JP;
+  const double *D = Din + int64_t(pair) * 2 * JP * JP;
+  for (int e = t; e < JP * JP; e += 256) {
+    qr[e] = Q[e];
+    qi[e] = Q[JP * JP + e];
+  }
+  if (t < JP) colmap[t] = pair_col(pairs + 2 * pair, t, n);
+  __syncthreads();
+  const int c = blockIdx.x * 256 + t;
+  if (c >= n) return;
+  int local = -1;
+  for (int k = 0; k < JP; ++k)
+    if (colmap[k] == c) local = k;
+  if (local >= 0) {
+    for (int j = 0; j < JP; ++j) {
+      const int rj = colmap[j];
+      if (rj < 0) continue;
+      G[rj + int64_t(c) * n] = D[j + JP * local];
+      if (cplx) G[imG + rj + int64_t(c) * n] = D[JP * JP + j + JP * local];
+    }
+    return;
+  }
+  double gr[JP], gi[JP];
+#pragma unroll
+  for (int k = 0; k < JP; ++k) {
+    const int rk = colmap[k];
+    gr[k] = rk >= 0 ? G[rk + int64_t(c) * n] : 0.0;
+    gi[k] = (rk >= 0 && cplx) ? G[imG + rk + int64_t(c) * n] : 0.0;
+  }
+  for (int j = 0; j < JP; ++j) {
+    const int rj = colmap[j];
+    if (rj < 0) continue;
+    double sr = 0, si = 0;
+#pragma unroll
+    for (int k = 0; k < JP; ++k) {
+      const double a = qr[k + JP * j], b = -qi[k + JP * j];  // conj(Q[k][j])
+      sr += a * gr[k] - b * gi[k];
+      si += a * gi[k] + b * gr[k];
+    }
+    G[rj + int64_t(c) * n] = sr;
+    if (cplx) G[imG + rj + int64_t(c) * n] = si;
+  }
+}
+
+__global__ void eye_kernel(double *V, int n, int64_t imV, bool cplx) {
+  const int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  const int64_t nn = int64_t(n) * n;
+  if (e >= nn) return;
+  V[e] = (e % n == e / n) ? 1.0 : 0.0;
+  if (cplx) V[imV + e] = 0.0;
+}
+
+// diag[j] = Re G[j, j]  or  colnorm2[j] = sum_r |X[r, j]|^2 (one block per column)
+__global__ void diag_kernel(const double *G, int n, double *out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) out[j] = G[j + int64_t(j) * n];
+}
+
+__global__ void colnorm2_kernel(const double *X, int rows, int64_t imX, bool cplx, double *out) {
+  const int j = blockIdx.x;
+  double s = 0;
+  for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+    const double a = X[r + int64_t(j) * rows];
+    s += a * a;
+    if (cplx) {
+      const double b = X[imX + r + int64_t(j) * rows];
+      s += b * b;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[j] = red[0] + red[1] + red[2] + red[3];
+}
+
+// out[:, j] = scale_j * src[:, perm[j]]  (rows x k, planar complex when cplx)
+__global__ void gather_cols_kernel(const double *src, int rows, int64_t imS, const int *perm,
+                                   const double *scale, int k, double *out, int64_t imO,
+                                   bool cplx) {
+  const int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (e >= int64_t(rows) * k) return;
+  const int r = int(e % rows), j = int(e / rows);
+  const int64_t s = r + int64_t(perm[j]) * rows;
+  const double f = scale ? scale[j] : 1.0;
+  out[e] = f * src[s];
+  if (cplx) out[imO + e] = f * src[imS + s];
+}
+
+// out[j, r] = scale_j * conj(src[r, perm[j]])  (k x rows), via a 32 x 32 LDS tile
+__global__ void gather_cols_adjoint_kernel(const double *src, int rows, int64_t imS,
+                                           const int *perm, const double *scale, int k,
+                                           double *out, int64_t imO, bool cplx) {
+  __shared__ double tr[32][33], ti[32][33];
+  const int r0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 256 threads: ty in [0, 8)
+  for (int jj = ty; jj < 32; jj += 8) {
+    const int j = j0 + jj, r = r0 + tx;
+    double a = 0, b = 0;
+    if (j < k && r < rows) {
+      const int64_t s = r + int64_t(perm[j]) * rows;
+      const double f = scale ? scale[j] : 1.0;
+      a = f * src[s];
+      b = cplx ? -f * src[imS + s] : 0.0;
+    }
+    tr[jj][tx] = a;
+    ti[jj][tx] = b;
+  }
+  __syncthreads();
+  for (int rr = ty; rr < 32; rr += 8) {
+    const int r = r0 + rr, j = j0 + tx;
+    if (j < k && r < rows) {
+      out[j + int64_t(r) * k] = tr[tx][rr];
+      if (cplx) out[imO + j + int64_t(r) * k] = ti[tx][rr];
+    }
+  }
+}
+
+struct Schedule {
+  int nb = 0, nbp = 0, rounds = 0, npairs = 0;
+  int *dev = nullptr;  // [round][pair][2]
+};
+
+}  // namespace
+
+// ------------------------------------------------------------------ engine
+
+struct JacobiResult {
+  int sweeps = 0;
+  bool converged = false;
+  double measure = 0;
+};
+
+static Schedule schedule_for(tnh_ctx *ctx, int cols) {
+  Schedule s;
+  s.nb = (cols + JB - 1) / JB;
+  s.nbp = s.nb + (s.nb & 1);
+  if (s.nbp < 2) s.nbp = 2;
+  s.rounds = s.nbp - 1;
+  s.npairs = s.nbp / 2;
+  std::vector<int> h(size_t(s.rounds) * s.npairs * 2);
+  for (int r = 0; r < s.rounds; ++r)
+    for (int i = 0; i < s.npairs; ++i) {
+      int p, q;
+      circle_pair(s.nbp, r, i, &p, &q);
+      h[(size_t(r) * s.npairs + i) * 2] = p < s.nb ? p : -1;
+      h[(size_t(r) * s.npairs + i) * 2 + 1] = q < s.nb ? q : -1;
+    }
+  s.dev = static_cast<int *>(ctx->workspace(600, sizeof(int) * h.size()));
+  TNH_HIP(hipMemcpyAsync(s.dev, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+  // the host vector dies at return: make the copy complete first
+  TNH_HIP(hipStreamSynchronize(ctx->stream));
+  return s;
+}
+
+// two_sided: diagonalise Hermitian G (n x n) in place, V <- V * rotations.
+// one-sided: orthogonalise the columns of X (rows x n) in place, V (n x n) accumulates.
+static JacobiResult jacobi(tnh_ctx *ctx, bool two_sided, double *X, int rows, int n, bool cplx,
+                           double *V, double tol, int max_sweeps) {
+  JacobiResult res;
+  const Schedule s = schedule_for(ctx, n);
+  const int64_t imX = int64_t(rows) * n, imV = int64_t(n) * n;
+  int nsplit = 1;
+  if (!two_sided) {
+    nsplit = std::max(1, std::min((2 * ctx->cu_count + s.npairs - 1) / s.npairs,
+                                  (rows + kGramRows - 1) / kGramRows));
+  }
+  double *partial = nullptr;
+  if (!two_sided)
+    partial = static_cast<double *>(
+        ctx->workspace(601, sizeof(double) * size_t(s.npairs) * nsplit * 2 * JP * JP));
+  double *Q = static_cast<double *>(ctx->workspace(602, sizeof(double) * size_t(s.npairs) * 2 * JP * JP));
+  double *D = two_sided ? static_cast<double *>(ctx->workspace(603, sizeof(double) * size_t(s.npairs) * 2 * JP * JP))
+                        : nullptr;
+  int *active = static_cast<int *>(ctx->workspace(604, sizeof(int) * s.npairs));
+  unsigned long long *conv = static_cast<unsigned long long *>(
+      ctx->workspace(605, sizeof(unsigned long long) * (max_sweeps + 1)));
+  TNH_HIP(hipMemsetAsync(conv, 0, sizeof(unsigned long long) * (max_sweeps + 1), ctx->stream));
+  const int rmax = std::max(rows, n);
+  for (int sw = 0; sw < max_sweeps; ++sw) {
+    for (int r = 0; r < s.rounds; ++r) {
+      const int *pr = s.dev + size_t(r) * s.npairs * 2;
+      if (!two_sided)
+        hipLaunchKernelGGL(jac_gram_kernel, dim3(s.npairs, nsplit), dim3(256), 0, ctx->stream, X,
+                           rows, imX, n, cplx, pr, nsplit, partial);
+      hipLaunchKernelGGL(jac_eig_kernel, dim3(s.npairs), dim3(256), 0, ctx->stream, X, imX, n,
+                         partial, nsplit, two_sided, cplx, pr, n, tol, Q, D, active, conv + sw);
+      hipLaunchKernelGGL(jac_colupd_kernel, dim3((rmax + 255) / 256, s.npairs), dim3(256), 0,
+                         ctx->stream, X, rows, imX, V, n, imV, cplx, pr, n, Q, active);
+      if (two_sided)
+        hipLaunchKernelGGL(jac_rowupd_kernel, dim3((n + 255) / 256, s.npairs), dim3(256), 0,
+                           ctx->stream, X, n, imX, cplx, pr, Q, D, active);
+    }
+    TNH_HIP(hipGetLastError());
+    unsigned long long bits = 0;
+    to_host(ctx, &bits, conv + sw, sizeof(bits));
+    double m;
+    std::memcpy(&m, &bits, sizeof(m));
+    res.sweeps = sw + 1;
+    res.measure = m;
+    if (!std::isfinite(m)) return res;
+    if (m <= tol) {
+      res.converged = true;
+      return res;
+    }
+  }
+  return res;
+}
+
+// ------------------------------------------------------------- decomposition
+
+struct SvdOut {
+  int m = 0, n = 0, k = 0;
+  std::vector<double> s;  // all singular values, descending
+  double discarded = 0;
+  bool fallback = false;
+};
+
+// Host copy of the truncation rule, src/tensors.jl:202-215.
+static int truncation_rank(const std::vector<double> &S, double cutoff, int maxdim, int mindim,
+                           double *disc) {
+  const int sz = int(S.size());
+  mindim = std::min(mindim, sz);
+  maxdim = (maxdim == 0 || maxdim > sz) ? sz : maxdim;
+  maxdim = maxdim == 0 ? 1 : maxdim;
+  std::vector<double> S2(sz);
+  double tot = 0;
+  for (int i = 0; i < sz; ++i) {
+    S2[i] = S[i] * S[i];
+    tot += S2[i];
+  }
+  if (cutoff != 0) {
+    // reverse cumulative sum divided by the total, as the reference computes it
+    std::vector<double> cum(sz);
+    double acc = 0;
+    for (int i = sz - 1; i >= 0; --i) {
+      acc += S2[i];
+      cum[i] = acc;
+    }
+    int idxs = 0;
+    for (int i = 0; i < sz; ++i)
+      if (cum[i] / tot > cutoff) idxs = i + 1;
+    if (idxs == 0) idxs = 1;
+    maxdim = std::min(maxdim, idxs);
+  }
+  const int k = std::max(maxdim, mindim);
+  double d = 0;
+  for (int i = k; i < sz; ++i) d += S2[i];
+  *disc = tot > 0 ? d / tot : 0.0;
+  return k;
+}
+
+// Decompose the m x n matrix Mp (device, column-major, planar if cplx).
+// side 0 ("left"):  returns the left factor basis Uk (m x k, orthonormal columns, in
+//                   order of descending s) in *basis;
+// side 1 ("right"): returns the right factor basis Vk (n x k) with M ~ (M Vk) Vk^H.
+// The basis buffer is workspace slot 620 (valid until the next SVD).
+static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx, int side,
+                         double cutoff, int maxdim, int mindim, bool force_fallback, SvdOut *o) {
+  o->m = m;
+  o->n = n;
+  const int p = side == 0 ? m : n;  // basis dimension
+  const int64_t pp = int64_t(p) * p;
+  const int planes = cplx ? 2 : 1;
+  double *G = static_cast<double *>(ctx->workspace(610, sizeof(double) * pp * planes));
+  double *V = static_cast<double *>(ctx->workspace(611, sizeof(double) * pp * planes));
+  std::vector<double> val(p);
+  bool ok = false;
+  if (!force_fallback) {
+    // G = M M^H (side 0) or M^H M (side 1)
+    Gemm g;
+    g.M = p;
+    g.N = p;
+    g.K = side == 0 ? n : m;
+    g.lda = m;
+    g.ldb = m;
+    g.ldc = p;
+    COperand A, B;
+    A.re = Mp;
+    A.im = cplx ? Mp + int64_t(m) * n : nullptr;
+    B = A;
+    if (side == 0) {
+      g.transB = true;  // M * M^H
+      B.conj = true;
+    } else {
+      g.transA = true;  // M^H * M
+      A.conj = true;
+    }
+    cgemm(ctx, g, A, B, G, cplx ? G + pp : nullptr, TNH_F64, TNH_PREC_F64);
+    hipLaunchKernelGGL(eye_kernel, dim3(unsigned((pp + 255) / 256)), dim3(256), 0, ctx->stream, V, p,
+                       pp, cplx);
+    JacobiResult jr = jacobi(ctx, true, G, p, p, cplx, V, 1e-14, 40);
+    if (jr.converged) {
+      double *dv = static_cast<double *>(ctx->workspace(612, sizeof(double) * p));
+      hipLaunchKernelGGL(diag_kernel, dim3((p + 255) / 256), dim3(256), 0, ctx->stream, G, p, dv);
+      TNH_HIP(hipGetLastError());
+      to_host(ctx, val.data(), dv, sizeof(double) * p);
+      ok = true;
+      for (auto &v : val) {
+        if (!std::isfinite(v)) ok = false;
+        v = std::sqrt(std::max(v, 0.0));
+      }
+    }
+  }
+  double *W = nullptr;  // the matrix whose columns are (unnormalised) singular directions
+  if (!ok) {
+    // one-sided Jacobi on M (side 1) or on M^H (side 0): its accumulated V is the basis
+    o->fallback = true;
+    const int rows = side == 0 ? n : m;
+    double *X = static_cast<double *>(ctx->workspace(613, sizeof(double) * int64_t(rows) * p * planes));
+    if (side == 1) {
+      TNH_HIP(hipMemcpyAsync(X, Mp, sizeof(double) * int64_t(m) * n * planes,
+                             hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+      // X = M^H (n x m): adjoint copy with the identity permutation
+      std::vector<int> id(m);
+      std::iota(id.begin(), id.end(), 0);
+      int *dperm = static_cast<int *>(ctx->workspace(614, sizeof(int) * m));
+      TNH_HIP(hipMemcpyAsync(dperm, id.data(), sizeof(int) * m, hipMemcpyHostToDevice, ctx->stream));
+      dim3 grid((n + 31) / 32, (m + 31) / 32);
+      hipLaunchKernelGGL(gather_cols_adjoint_kernel, grid, dim3(256), 0, ctx->stream, Mp, n,
+                         int64_t(m) * n, dperm, nullptr, m, X, int64_t(n) * m, cplx);
+      TNH_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    hipLaunchKernelGGL(eye_kernel, dim3(unsigned((pp + 255) / 256)), dim3(256), 0, ctx->stream, V, p,
+                       pp, cplx);
+    JacobiResult jr = jacobi(ctx, false, X, rows, p, cplx, V, 1e-13, 60);
+    TNH_REQUIRE(jr.converged, TNH_ERR_NOCONV,
+                "svd: one-sided Jacobi did not converge (measure " + std::to_string(jr.measure) + ")");
+    double *dv = static_cast<double *>(ctx->workspace(612, sizeof(double) * p));
+    hipLaunchKernelGGL(colnorm2_kernel, dim3(p), dim3(256), 0, ctx->stream, X, rows,
+                       int64_t(rows) * p, cplx, dv);
+    TNH_HIP(hipGetLastError());
+    to_host(ctx, val.data(), dv, sizeof(double) * p);
+    for (auto &v : val) {
+      TNH_REQUIRE(std::isfinite(v), TNH_ERR_NUMERIC, "svd: non-finite singular value");
+      v = std::sqrt(std::max(v, 0.0));
+    }
+    W = X;
+    (void)W;
+  }
+  // order by descending singular value (stable)
+  std::vector<int> perm(p);
+  std::iota(perm.begin(), perm.end(), 0);
+  std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return val[a] > val[b]; });
+  // the reference's S has min(m, n) entries
+  const int r = std::min(m, n);
+  o->s.resize(r);
+  for (int i = 0; i < r; ++i) o->s[i] = val[perm[i]];
+  o->k = truncation_rank(o->s, cutoff, maxdim, mindim, &o->discarded);
+  const int k = o->k;
+  int *dperm = static_cast<int *>(ctx->workspace(615, sizeof(int) * k));
+  TNH_HIP(hipMemcpyAsync(dperm, perm.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx->stream));
+  double *basis = static_cast<double *>(ctx->workspace(620, sizeof(double) * int64_t(p) * k * planes));
+  hipLaunchKernelGGL(gather_cols_kernel, dim3(unsigned((int64_t(p) * k + 255) / 256)), dim3(256), 0,
+                     ctx->stream, V, p, pp, dperm, nullptr, k, basis, int64_t(p) * k, cplx);
+  TNH_HIP(hipGetLastError());
+  TNH_HIP(hipStreamSynchronize(ctx->stream));  // perm lives on the host stack
+  return basis;
+}
+
+// Real-or-complex FP64 view of a 2-D device tensor; converts other dtypes into workspace.
+static const double *as_f64(tnh_ctx *ctx, const tnh_buf *b, int slot) {
+  if (is_double(b->dtype)) return static_cast<const double *>(b->ptr);
+  void *w = ctx->workspace(slot, sizeof(double) * b->numel * (is_complex(b->dtype) ? 2 : 1));
+  convert(ctx, b->ptr, b->dtype, w, is_complex(b->dtype) ? TNH_C128 : TNH_F64, b->numel);
+  return static_cast<const double *>(w);
+}
+
+void cscale(tnh_ctx *ctx, void *x, int t, int64_t n, double cr, double ci);
+void cdot_host(tnh_ctx *ctx, const tnh_buf *x, const tnh_buf *y, double *re, double *im);
+
+static double frob(tnh_ctx *ctx, const tnh_buf *b) {
+  double re = 0, im = 0;
+  // sum conj(x) x = sum |x|^2 : use the real planes, which hold re then im
+  tnh_buf v = *b;
+  v.dtype = TNH_F64;
+  v.numel = b->numel * (is_complex(b->dtype) ? 2 : 1);
+  cdot_host(ctx, &v, &v, &re, &im);
+  return std::sqrt(std::max(re, 0.0));
+}
+
+void replacesites2(tnh_ctx *ctx, const tnh_buf *theta, int direction, int normalize,
+                   double cutoff, int maxdim, int mindim, tnh_buf **A1, tnh_buf **A2, int *k_out,
+                   double *disc_out, double *schmidt, int schmidt_cap, bool force_fallback) {
+  TNH_REQUIRE(theta && theta->ndim == 4, TNH_ERR_SHAPE, "theta must be (chi_l, d, d, chi_r)");
+  TNH_REQUIRE(A1 && A2, TNH_ERR_ARG, "A1/A2 out pointers are NULL");
+  const int64_t cl = theta->dims[0], d1 = theta->dims[1], d2 = theta->dims[2], cr = theta->dims[3];
+  const int m = int(cl * d1), n = int(d2 * cr);
+  TNH_REQUIRE(m > 0 && n > 0, TNH_ERR_SHAPE, "empty theta");
+  const bool cplx = is_complex(theta->dtype);
+  const double *M = as_f64(ctx, theta, 630);
+  SvdOut o;
+  const int side = direction ? 1 : 0;
+  double *basis = decompose(ctx, M, m, n, cplx, side, cutoff, maxdim, mindim, force_fallback, &o);
+  const int k = o.k;
+  const int dt = cplx ? TNH_C128 : TNH_F64;
+  int64_t d1v[3] = {cl, d1, k}, d2v[3] = {k, d2, cr};
+  buf_ensure(ctx, A1, dt, 3, d1v);
+  buf_ensure(ctx, A2, dt, 3, d2v);
+  COperand Mo;
+  Mo.re = M;
+  Mo.im = cplx ? M + int64_t(m) * n : nullptr;
+  if (side == 0) {
+    // A1 = U_k (left-orthonormal), A2 = U_k^H M (= S V^H)
+    TNH_HIP(hipMemcpyAsync((*A1)->ptr, basis, (*A1)->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    Gemm g;
+    g.M = k;
+    g.N = n;
+    g.K = m;
+    g.transA = true;
+    g.lda = m;
+    g.ldb = m;
+    g.ldc = k;
+    COperand U;
+    U.re = basis;
+    U.im = cplx ? basis + int64_t(m) * k : nullptr;
+    U.conj = true;
+    cgemm(ctx, g, U, Mo, (*A2)->re(), cplx ? (*A2)->im() : nullptr, TNH_F64, TNH_PREC_F64);
+  } else {
+    // A2 = V_k^H (right-orthonormal), A1 = M V_k (= U S)
+    std::vector<int> id(k);
+    std::iota(id.begin(), id.end(), 0);
+    int *dperm = static_cast<int *>(ctx->workspace(631, sizeof(int) * k));
+    TNH_HIP(hipMemcpyAsync(dperm, id.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx->stream));
+    dim3 grid((n + 31) / 32, (k + 31) / 32);
+    hipLaunchKernelGGL(gather_cols_adjoint_kernel, grid, dim3(256), 0, ctx->stream, basis, n,
+                       int64_t(n) * k, dperm, nullptr, k, static_cast<double *>((*A2)->ptr),
+                       int64_t(k) * n, cplx);
+    TNH_HIP(hipGetLastError());
+    Gemm g;
+    g.M = m;
+    g.N = k;
+    g.K = n;
+    g.lda = m;
+    g.ldb = n;
+    g.ldc = m;
+    COperand Vk;
+    Vk.re = basis;
+    Vk.im = cplx ? basis + int64_t(n) * k : nullptr;
+    cgemm(ctx, g, Mo, Vk, (*A1)->re(), cplx ? (*A1)->im() : nullptr, TNH_F64, TNH_PREC_F64);
+    TNH_HIP(hipStreamSynchronize(ctx->stream));  // id lives on the host stack
+  }
+  if (normalize) {
+    // normalize!(psi): divide the centre tensor by its norm, gmps.jl:46-51
+    tnh_buf *c = side == 0 ? *A2 : *A1;
+    const double nr = frob(ctx, c);
+    TNH_REQUIRE(std::isfinite(nr) && nr > 0, TNH_ERR_NUMERIC, "replacesites: zero centre tensor");
+    scale(ctx, c->ptr, c->dtype, c->numel, 1.0 / nr);
+  }
+  if (k_out) *k_out = k;
+  if (disc_out) *disc_out = o.discarded;
+  if (schmidt)
+    for (int i = 0; i < std::min(k, schmidt_cap); ++i) schmidt[i] = o.s[i];
+}
+
+void svd_trunc(tnh_ctx *ctx, const tnh_buf *Mb, double cutoff, int maxdim, int mindim,
+               bool force_fallback, tnh_buf **U, tnh_buf **S, tnh_buf **Vt, int *k_out,
+               double *disc_out, int *fb_out) {
+  TNH_REQUIRE(Mb && Mb->ndim == 2, TNH_ERR_SHAPE, "svd_trunc: M must be a 2-D tensor");
+  const int m = int(Mb->dims[0]), n = int(Mb->dims[1]);
+  TNH_REQUIRE(m > 0 && n > 0, TNH_ERR_SHAPE, "svd_trunc: empty matrix");
+  const bool cplx = is_complex(Mb->dtype);
+  const double *M = as_f64(ctx, Mb, 630);
+  SvdOut o;
+  const int side = m <= n ? 0 : 1;
+  double *basis = decompose(ctx, M, m, n, cplx, side, cutoff, maxdim, mindim, force_fallback, &o);
+  const int k = o.k;
+  const int dt = cplx ? TNH_C128 : TNH_F64;
+  // singular values and their reciprocals (zero rows for s = 0)
+  std::vector<double> sv(o.s.begin(), o.s.begin() + k), inv(k);
+  for (int i = 0; i < k; ++i) inv[i] = sv[i] > 0 ? 1.0 / sv[i] : 0.0;
+  double *dinv = static_cast<double *>(ctx->workspace(632, sizeof(double) * k));
+  TNH_HIP(hipMemcpyAsync(dinv, inv.data(), sizeof(double) * k, hipMemcpyHostToDevice, ctx->stream));
+  std::vector<int> id(k);
+  std::iota(id.begin(), id.end(), 0);
+  int *dperm = static_cast<int *>(ctx->workspace(631, sizeof(int) * k));
+  TNH_HIP(hipMemcpyAsync(dperm, id.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx->stream));
+  COperand Mo;
+  Mo.re = M;
+  Mo.im = cplx ? M + int64_t(m) * n : nullptr;
+  double *other = static_cast<double *>(ctx->workspace(633, sizeof(double) * int64_t(std::max(m, n)) * k * (cplx ? 2 : 1)));
+  if (side == 0) {
+    // U = basis (m x k); Vt = diag(1/s) U^H M
+    if (U) {
+      int64_t dU[2] = {m, k};
+      buf_ensure(ctx, U, dt, 2, dU);
+      TNH_HIP(hipMemcpyAsync((*U)->ptr, basis, (*U)->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    if (Vt) {
+      int64_t dV[2] = {k, n};
+      buf_ensure(ctx, Vt, dt, 2, dV);
+      Gemm g;
+      g.M = k;
+      g.N = n;
+      g.K = m;
+      g.transA = true;
+      g.lda = m;
+      g.ldb = m;
+      g.ldc = k;
+      COperand Ub;
+      Ub.re = basis;
+      Ub.im = cplx ? basis + int64_t(m) * k : nullptr;
+      Ub.conj = true;
+      cgemm(ctx, g, Ub, Mo, (*Vt)->re(), cplx ? (*Vt)->im() : nullptr, TNH_F64, TNH_PREC_F64);
+      // row scaling by 1/s: gather columns of the transpose is awkward; scale via a
+      // diagonal GEMM-free pass: Vt[i, :] *= inv[i]
+      std::vector<double> h(size_t(k) * n * (cplx ? 2 : 1));
+      to_host(ctx, h.data(), (*Vt)->ptr, (*Vt)->bytes);
+      for (int pl = 0; pl < (cplx ? 2 : 1); ++pl)
+        for (int c = 0; c < n; ++c)
+          for (int i = 0; i < k; ++i) h[size_t(pl) * k * n + i + size_t(c) * k] *= inv[i];
+      TNH_HIP(hipMemcpyAsync((*Vt)->ptr, h.data(), (*Vt)->bytes, hipMemcpyHostToDevice, ctx->stream));
+      TNH_HIP(hipStreamSynchronize(ctx->stream));
+    }
+  } else {
+    // V = basis (n x k): Vt = basis^H; U = M V diag(1/s)
+    if (Vt) {
+      int64_t dV[2] = {k, n};
+      buf_ensure(ctx, Vt, dt, 2, dV);
+      dim3 grid((n + 31) / 32, (k + 31) / 32);
+      hipLaunchKernelGGL(gather_cols_adjoint_kernel, grid, dim3(256), 0, ctx->stream, basis, n,
+                         int64_t(n) * k, dperm, nullptr, k, static_cast<double *>((*Vt)->ptr),
+                         int64_t(k) * n, cplx);
+    }
+    if (U) {
+      int64_t dU[2] = {m, k};
+      buf_ensure(ctx, U, dt, 2, dU);
+      Gemm g;
+      g.M = m;
+      g.N = k;
+      g.K = n;
+      g.lda = m;
+      g.ldb = n;
+      g.ldc = m;
+      COperand Vb;
+      Vb.re = basis;
+      Vb.im = cplx ? basis + int64_t(n) * k : nullptr;
+      cgemm(ctx, g, Mo, Vb, other, cplx ? other + int64_t(m) * k : nullptr, TNH_F64, TNH_PREC_F64);
+      hipLaunchKernelGGL(gather_cols_kernel, dim3(unsigned((int64_t(m) * k + 255) / 256)), dim3(256), 0,
+                         ctx->stream, other, m, int64_t(m) * k, dperm, dinv, k,
+                         static_cast<double *>((*U)->ptr), int64_t(m) * k, cplx);
+    }
+  }
+  TNH_HIP(hipGetLastError());
+  if (S) {
+    int64_t dS[1] = {k};
+    buf_ensure(ctx, S, TNH_F64, 1, dS);
+    TNH_HIP(hipMemcpyAsync((*S)->ptr, sv.data(), sizeof(double) * k, hipMemcpyHostToDevice, ctx->stream));
+  }
+  TNH_HIP(hipStreamSynchronize(ctx->stream));  // host vectors die at return
+  if (k_out) *k_out = k;
+  if (disc_out) *disc_out = o.discarded;
+  if (fb_out) *fb_out = o.fallback ? 1 : 0;
+}
+
+}  // namespace tnh
+
+using namespace tnh;
+
+extern "C" {
+
+int tnh_svd_trunc(tnh_ctx *ctx, const tnh_buf *M, double cutoff, int maxdim, int mindim,
+                  int force_fallback, tnh_buf **U, tnh_buf **S, tnh_buf **Vt, int *k,
+                  double *discarded, int *fallback) {
+  if (!ctx) return TNH_ERR_ARG;
+  return tnh_guard(ctx, [&] {
+    TNH_REQUIRE(maxdim >= 0 && mindim >= 0, TNH_ERR_ARG, "maxdim and mindim must be >= 0");
+    svd_trunc(ctx, M, cutoff, maxdim, mindim, force_fallback != 0, U, S, Vt, k, discarded, fallback);
+  });
+}
+
+int tnh_replacesites2(tnh_ctx *ctx, const tnh_buf *theta, int direction, int normalize,
+                      double cutoff, int maxdim, int mindim, tnh_buf **A1, tnh_buf **A2, int *k,
+                      double *discarded, double *schmidt, int schmidt_cap) {
+  if (!ctx) return TNH_ERR_ARG;
+  return tnh_guard(ctx, [&] {
+    TNH_REQUIRE(maxdim >= 0 && mindim >= 0, TNH_ERR_ARG, "maxdim and mindim must be >= 0");
+    replacesites2(ctx, theta, direction, normalize, cutoff, maxdim, mindim, A1, A2, k, discarded,
+                  schmidt, schmidt_cap, false);
+  });
+}
+
+}  // extern "C"
