@@ -72,6 +72,17 @@ int tnh_ctx_synchronize(tnh_ctx *ctx);
 /* Library version string, e.g. "tnhip 0.1 gfx950". */
 const char *tnh_version(void);
 
+/* Kernel timing (no counterpart in the reference, which has no profiling): while
+ * enabled, HIP events on the context's stream bracket every launch of the kernel
+ * families "gemm_f64", "gemm_f32", "gemm_bf16x3" (matrix-core GEMMs), "mix" (MPO /
+ * gate mixing stage) and "jacobi" (SVD sweeps). tnh_profile_read synchronises and
+ * returns the launch count, the summed kernel time and the summed algorithmic work
+ * (flops = 2 * multiply-adds; bytes = operands read once + results written once). */
+int tnh_profile_enable(tnh_ctx *ctx, int on);
+int tnh_profile_reset(tnh_ctx *ctx);
+int tnh_profile_read(tnh_ctx *ctx, const char *family, int64_t *launches, double *total_ms,
+                     double *flops, double *bytes);
+
 /* ------------------------------------------------------------------ buffers */
 
 /* Allocate an uninitialised device tensor. ndim <= 8. */
@@ -164,6 +175,13 @@ int tnh_svd_trunc(tnh_ctx *ctx, const tnh_buf *M, double cutoff, int maxdim, int
 int tnh_replacesites2(tnh_ctx *ctx, const tnh_buf *theta, int direction, int normalize,
                       double cutoff, int maxdim, int mindim, tnh_buf **A1, tnh_buf **A2,
                       int *k, double *discarded, double *schmidt, int schmidt_cap);
+
+/* Move the orthogonality centre of a rank-1 MPS of N device site tensors (replaced in
+ * place, bond dimensions may change) from `center` (0 = not set) to `new_center`
+ * (1-based), by truncated SVDs. Replaces movecenter!(psi, idx; cutoff, maxdim, mindim),
+ * src/structures/mps/gmps.jl:90-112, with moveleft!/moveright! at :60-82. */
+int tnh_mps_movecenter(tnh_ctx *ctx, int N, tnh_buf **sites, int center, int new_center,
+                       double cutoff, int maxdim, int mindim);
 
 /* theta (chi_l, d, d, chi_r) = A1 (chi_l,d,chi) * A2 (chi,d,chi_r): contract(A1, A2, 3, 1),
  * src/algorithms/mps/dmrg.jl:44-47 and src/structures/mps/gatelist.jl:142-145. */

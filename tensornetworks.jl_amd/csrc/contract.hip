@@ -208,7 +208,8 @@ void small_contract(tnh_ctx *ctx, const Small &s) {
   const bool i64 = s.tin == TNH_F64, o64 = s.tout == TNH_F64;
   const size_t shm = size_t(s.I) * s.J * (cw ? 2 : 1) * ((i64 || o64) ? 8 : 4);
   dim3 grid(unsigned((p.total + 255) / 256));
-#define TNH_SMALL(TI, TO)                                                                       \
+  hipEvent_t ev = ctx->prof_begin();
+#define TNH_SMALL(TI, TO)                                                                     \
   do {                                                                                          \
     if (cin && cw)                                                                              \
       hipLaunchKernelGGL((small_kernel<TI, TO, true, true>), grid, dim3(256), shm, ctx->stream, p); \
@@ -229,6 +230,13 @@ void small_contract(tnh_ctx *ctx, const Small &s) {
     TNH_SMALL(float, float);
 #undef TNH_SMALL
   TNH_HIP(hipGetLastError());
+  if (ev) {
+    const double planes_in = cin ? 2 : 1, planes_out = (cin || cw) ? 2 : 1;
+    const double cmul = (cin && cw) ? 4 : ((cin || cw) ? 2 : 1);
+    ctx->prof_end("mix", ev, 2.0 * cmul * double(p.total) * s.I * s.J,
+                  double(p.total) * (s.I * planes_in * real_size(s.tin) +
+                                     s.J * planes_out * real_size(s.tout) * (s.beta != 0.0 ? 2 : 1)));
+  }
 }
 
 namespace {

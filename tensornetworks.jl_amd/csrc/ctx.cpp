@@ -72,8 +72,62 @@ void *tnh_ctx::workspace(int slot, size_t bytes) {
   return w.first;
 }
 
+hipEvent_t tnh_ctx::take_event() {
+  if (!event_pool.empty()) {
+    hipEvent_t e = event_pool.back();
+    event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  TNH_HIP(hipEventCreate(&e));
+  return e;
+}
+
+hipEvent_t tnh_ctx::prof_begin() {
+  if (!profiling) return nullptr;
+  hipEvent_t e = take_event();
+  TNH_HIP(hipEventRecord(e, stream));
+  return e;
+}
+
+void tnh_ctx::prof_end(const char *family, hipEvent_t start, double flops, double bytes) {
+  if (!profiling || !start) return;
+  hipEvent_t e = take_event();
+  TNH_HIP(hipEventRecord(e, stream));
+  prof_pending[family].push_back({start, e, flops, bytes});
+  // keep the number of outstanding events bounded
+  size_t n = 0;
+  for (auto &kv : prof_pending) n += kv.second.size();
+  if (n > 4096) prof_resolve();
+}
+
+void tnh_ctx::prof_resolve() {
+  if (prof_pending.empty()) return;
+  TNH_HIP(hipStreamSynchronize(stream));
+  for (auto &kv : prof_pending) {
+    ProfSum &s = prof_done[kv.first];
+    for (auto &r : kv.second) {
+      float ms = 0;
+      TNH_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      s.launches += 1;
+      s.ms += ms;
+      s.flops += r.flops;
+      s.bytes += r.bytes;
+      event_pool.push_back(r.a);
+      event_pool.push_back(r.b);
+    }
+  }
+  prof_pending.clear();
+}
+
 tnh_ctx::~tnh_ctx() {
   if (stream) (void)hipStreamSynchronize(stream);
+  for (auto &kv : prof_pending)
+    for (auto &r : kv.second) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+  for (auto e : event_pool) (void)hipEventDestroy(e);
   for (auto &kv : ws)
     if (kv.second.first) (void)hipFree(kv.second.first);
   for (auto &kv : cache) (void)hipFree(kv.second);
@@ -187,6 +241,37 @@ const char *tnh_last_error(tnh_ctx *ctx) { return ctx ? ctx->err.c_str() : "null
 int tnh_ctx_synchronize(tnh_ctx *ctx) {
   if (!ctx) return TNH_ERR_ARG;
   return tnh_guard(ctx, [&] { TNH_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+int tnh_profile_enable(tnh_ctx *ctx, int on) {
+  if (!ctx) return TNH_ERR_ARG;
+  return tnh_guard(ctx, [&] {
+    if (!on) ctx->prof_resolve();
+    ctx->profiling = on != 0;
+  });
+}
+
+int tnh_profile_reset(tnh_ctx *ctx) {
+  if (!ctx) return TNH_ERR_ARG;
+  return tnh_guard(ctx, [&] {
+    ctx->prof_resolve();
+    ctx->prof_done.clear();
+  });
+}
+
+int tnh_profile_read(tnh_ctx *ctx, const char *family, int64_t *launches, double *total_ms,
+                     double *flops, double *bytes) {
+  if (!ctx || !family) return TNH_ERR_ARG;
+  return tnh_guard(ctx, [&] {
+    ctx->prof_resolve();
+    auto it = ctx->prof_done.find(family);
+    tnh_ctx::ProfSum s;
+    if (it != ctx->prof_done.end()) s = it->second;
+    if (launches) *launches = s.launches;
+    if (total_ms) *total_ms = s.ms;
+    if (flops) *flops = s.flops;
+    if (bytes) *bytes = s.bytes;
+  });
 }
 
 int tnh_buf_create(tnh_ctx *ctx, int dtype, int ndim, const int64_t *dims, tnh_buf **out) {

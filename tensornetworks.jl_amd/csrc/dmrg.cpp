@@ -1,0 +1,282 @@
+// Device-resident two-site DMRG state and sweep: dmrg(psi, H::GMPS...),
+// src/algorithms/mps/dmrg.jl:128-154 and the bond loop of :34-63, with the
+// environment cache of ProjMPS / ProjMPSSum (src/structures/mps/projmps.jl:1-42,
+// abstractprojmps.jl:60-81, projmpssum.jl:63-73). The convergence bookkeeping of
+// dmrg.jl:66-95 stays with the host caller, which calls tnh_dmrg_sweep per pass.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "tnh_internal.hpp"
+
+using namespace tnh;
+
+struct tnh_dmrg {
+  tnh_ctx *ctx = nullptr;
+  int N = 0;
+  int nterms = 0;
+  std::vector<tnh_buf *> psi;                 // N site tensors (F64 or C128)
+  std::vector<std::vector<tnh_buf *>> W;      // [term][site]
+  std::vector<double> coeffs;
+  std::vector<std::vector<tnh_buf *>> blocks; // [term][site] environment blocks
+  std::vector<tnh_buf *> shadowL, shadowR;    // FP32 copies of the bond's blocks, per term
+  tnh_buf *theta = nullptr;
+  tnh_buf *tmp = nullptr;
+  int env_center = 0;   // ProjMPS centre (0 = blocks not built)
+  int psi_center = 0;
+
+  ~tnh_dmrg() {
+    auto del = [&](tnh_buf *&b) {
+      buf_delete(ctx, b);
+      b = nullptr;
+    };
+    for (auto &b : psi) del(b);
+    for (auto &v : W)
+      for (auto &b : v) del(b);
+    for (auto &v : blocks)
+      for (auto &b : v) del(b);
+    for (auto &b : shadowL) del(b);
+    for (auto &b : shadowR) del(b);
+    del(theta);
+    del(tmp);
+  }
+
+  // block(projV, idx), abstractprojmps.jl:33-46 (nullptr = edge block)
+  const tnh_buf *block(int t, int idx) const {
+    if (idx < 1 || idx > N) return nullptr;
+    return blocks[t][idx - 1];
+  }
+
+  void buildleft(int idx, int prec) {
+    for (int t = 0; t < nterms; ++t)
+      env_left(ctx, block(t, idx - 1), psi[idx - 1], W[t][idx - 1], &blocks[t][idx - 1], prec);
+  }
+  void buildright(int idx, int prec) {
+    for (int t = 0; t < nterms; ++t)
+      env_right(ctx, block(t, idx + 1), psi[idx - 1], W[t][idx - 1], &blocks[t][idx - 1], prec);
+  }
+
+  // movecenter!(projV, idx), abstractprojmps.jl:60-81
+  void env_movecenter(int idx, int prec) {
+    if (env_center == 0) {
+      for (int i = 1; i < idx; ++i) buildleft(i, prec);
+      for (int i = 1; i <= N - idx; ++i) buildright(N + 1 - i, prec);
+    } else if (idx > env_center) {
+      for (int i = 1; i <= idx - env_center; ++i) buildleft(env_center - 1 + i, prec);
+    } else if (idx < env_center) {
+      for (int i = 1; i <= env_center - idx; ++i) buildright(env_center + 1 - i, prec);
+    }
+    env_center = idx;
+  }
+};
+
+namespace {
+
+void copy_in(tnh_ctx *ctx, const tnh_buf *src, bool cplx, tnh_buf **dst) {
+  TNH_REQUIRE(src, TNH_ERR_ARG, "NULL tensor");
+  const int dt = cplx ? TNH_C128 : TNH_F64;
+  buf_ensure(ctx, dst, dt, src->ndim, src->dims);
+  if (src->numel == 0) return;
+  if (src->dtype == dt) {
+    TNH_HIP(hipMemcpyAsync((*dst)->ptr, src->ptr, src->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  } else {
+    convert(ctx, src->ptr, src->dtype, (*dst)->ptr, dt, src->numel);
+  }
+}
+
+// y = sum_t coeff_t H_t x for the bond (site1, site1 + 1)
+void bond_matvec(tnh_dmrg *st, int site1, int prec, const tnh_buf *x, tnh_buf *y) {
+  tnh_ctx *ctx = st->ctx;
+  for (int t = 0; t < st->nterms; ++t) {
+    const tnh_buf *L = st->block(t, site1 - 1);
+    const tnh_buf *R = st->block(t, site1 + 2);
+    if (prec != TNH_PREC_F64) {
+      if (L) L = st->shadowL[t];
+      if (R) R = st->shadowR[t];
+    }
+    if (t == 0) {
+      heff_apply(ctx, L, st->W[t][site1 - 1], st->W[t][site1], R, x, y, st->coeffs[t], 0.0, prec);
+    } else {
+      buf_ensure(ctx, &st->tmp, y->dtype, y->ndim, y->dims);
+      heff_apply(ctx, L, st->W[t][site1 - 1], st->W[t][site1], R, x, st->tmp, st->coeffs[t], 0.0,
+                 prec);
+      axpby(ctx, y->numel * (is_complex(y->dtype) ? 2 : 1), 1.0,
+            static_cast<const double *>(st->tmp->ptr), 1.0, static_cast<double *>(y->ptr));
+    }
+  }
+}
+
+void make_shadows(tnh_dmrg *st, int site1) {
+  tnh_ctx *ctx = st->ctx;
+  for (int t = 0; t < st->nterms; ++t) {
+    const tnh_buf *L = st->block(t, site1 - 1);
+    const tnh_buf *R = st->block(t, site1 + 2);
+    if (L) {
+      const int dt = is_complex(L->dtype) ? TNH_C64 : TNH_F32;
+      buf_ensure(ctx, &st->shadowL[t], dt, L->ndim, L->dims);
+      convert(ctx, L->ptr, L->dtype, st->shadowL[t]->ptr, dt, L->numel);
+    }
+    if (R) {
+      const int dt = is_complex(R->dtype) ? TNH_C64 : TNH_F32;
+      buf_ensure(ctx, &st->shadowR[t], dt, R->ndim, R->dims);
+      convert(ctx, R->ptr, R->dtype, st->shadowR[t]->ptr, dt, R->numel);
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int tnh_dmrg_create(tnh_ctx *ctx, int N, const tnh_buf *const *psi, int nterms,
+                    const tnh_buf *const *mpo, const double *coeffs, tnh_dmrg **out) {
+  if (!ctx || !out) return TNH_ERR_ARG;
+  *out = nullptr;
+  tnh_dmrg *st = new (std::nothrow) tnh_dmrg();
+  if (!st) return TNH_ERR_NOMEM;
+  st->ctx = ctx;
+  int rc = tnh_guard(ctx, [&] {
+    TNH_REQUIRE(N >= 2, TNH_ERR_ARG, "two-site DMRG needs N >= 2");
+    TNH_REQUIRE(nterms >= 1, TNH_ERR_ARG, "You must provide atleast one MPS/MPO for the Hamiltonian.");
+    TNH_REQUIRE(psi && mpo, TNH_ERR_ARG, "NULL psi or mpo array");
+    st->N = N;
+    st->nterms = nterms;
+    bool cplx = false;
+    for (int i = 0; i < N; ++i) {
+      TNH_REQUIRE(psi[i] && psi[i]->ndim == 3, TNH_ERR_SHAPE, "Psi must be a GMPS of rank 1 (vector).");
+      cplx |= is_complex(psi[i]->dtype);
+    }
+    const int64_t d = psi[0]->dims[1];
+    for (int t = 0; t < nterms; ++t)
+      for (int i = 0; i < N; ++i) {
+        const tnh_buf *w = mpo[size_t(t) * N + i];
+        TNH_REQUIRE(w && w->ndim == 4, TNH_ERR_SHAPE,
+                    "Hamiltonian must be composed of MPOs (rank 2) or MPSs (rank 1).");
+        TNH_REQUIRE(w->dims[1] == d && w->dims[2] == d && psi[i]->dims[1] == d, TNH_ERR_SHAPE,
+                    "GMPS must share the same properties.");
+        cplx |= is_complex(w->dtype);
+      }
+    // bond consistency of psi and the MPOs
+    for (int i = 0; i + 1 < N; ++i)
+      TNH_REQUIRE(psi[i]->dims[2] == psi[i + 1]->dims[0], TNH_ERR_SHAPE, "psi bond dims do not match");
+    TNH_REQUIRE(psi[0]->dims[0] == 1 && psi[N - 1]->dims[2] == 1, TNH_ERR_SHAPE,
+                "psi edge bonds must be 1");
+    for (int t = 0; t < nterms; ++t) {
+      for (int i = 0; i + 1 < N; ++i)
+        TNH_REQUIRE(mpo[size_t(t) * N + i]->dims[3] == mpo[size_t(t) * N + i + 1]->dims[0],
+                    TNH_ERR_SHAPE, "MPO bond dims do not match");
+      TNH_REQUIRE(mpo[size_t(t) * N]->dims[0] == 1 && mpo[size_t(t) * N + N - 1]->dims[3] == 1,
+                  TNH_ERR_SHAPE, "MPO edge bonds must be 1");
+    }
+    st->psi.assign(N, nullptr);
+    for (int i = 0; i < N; ++i) copy_in(ctx, psi[i], cplx, &st->psi[i]);
+    st->W.assign(nterms, std::vector<tnh_buf *>(N, nullptr));
+    st->blocks.assign(nterms, std::vector<tnh_buf *>(N, nullptr));
+    st->shadowL.assign(nterms, nullptr);
+    st->shadowR.assign(nterms, nullptr);
+    st->coeffs.assign(nterms, 1.0);
+    for (int t = 0; t < nterms; ++t) {
+      if (coeffs) st->coeffs[t] = coeffs[t];
+      for (int i = 0; i < N; ++i) copy_in(ctx, mpo[size_t(t) * N + i], cplx, &st->W[t][i]);
+    }
+    // movecenter!(psi, 1), dmrg.jl:133 (the caller's centre is unknown: canonicalise fully)
+    mps_movecenter(ctx, N, st->psi.data(), 0, 1, 0.0, 0, 1);
+    st->psi_center = 1;
+    // ProjMPS(psi, H, psi; rank=2): centre 1 builds all right blocks, projmps.jl:38-40
+    st->env_movecenter(1, TNH_PREC_F64);
+    TNH_HIP(hipStreamSynchronize(ctx->stream));
+  });
+  if (rc != TNH_OK) {
+    delete st;
+    return rc;
+  }
+  *out = st;
+  return TNH_OK;
+}
+
+int tnh_dmrg_destroy(tnh_dmrg *st) {
+  delete st;
+  return TNH_OK;
+}
+
+int tnh_dmrg_sweep(tnh_dmrg *st, int direction, const tnh_dmrg_params *p, double *energy,
+                   int *maxbonddim, double *max_discarded) {
+  if (!st || !p) return TNH_ERR_ARG;
+  tnh_ctx *ctx = st->ctx;
+  return tnh_guard(ctx, [&] {
+    const int N = st->N;
+    const int prec = p->precision;
+    TNH_REQUIRE(prec >= TNH_PREC_F64 && prec <= TNH_PREC_BF16X3, TNH_ERR_ARG, "bad precision");
+    TNH_REQUIRE(p->maxdim >= 0 && p->mindim >= 0, TNH_ERR_ARG, "maxdim, mindim must be >= 0");
+    double cost = 0, maxdisc = 0;
+    for (int j = 1; j <= N - 1; ++j) {  // dmrg.jl:35
+      const int site = direction ? N + 1 - j : j;
+      const int site1 = direction ? site - 1 : site;
+      st->env_movecenter(site, prec);  // :41
+      // the psi centre must sit on the bond: it does after the previous split
+      two_site_theta(ctx, st->psi[site1 - 1], st->psi[site1], &st->theta);  // :44-47
+      if (prec != TNH_PREC_F64) make_shadows(st, site1);
+      int nmv = 0;
+      cost = eigsolve_ground_op(
+          ctx, [&](const tnh_buf *x, tnh_buf *y) { bond_matvec(st, site1, prec, x, y); },
+          st->theta, p->krylovdim, p->maxiter, p->lanczos_tol, &nmv);  // :50-55
+      int k = 0;
+      double disc = 0;
+      replacesites2(ctx, st->theta, direction, 1, p->cutoff, p->maxdim, p->mindim,
+                    &st->psi[site1 - 1], &st->psi[site1], &k, &disc, nullptr, 0, false);  // :57
+      st->psi_center = direction ? site1 : site1 + 1;
+      maxdisc = std::max(maxdisc, disc);
+    }
+    st->env_movecenter(direction ? 1 : N, prec);  // :62
+    int D = 0;
+    for (int i = 0; i + 1 < N; ++i) D = std::max<int>(D, int(st->psi[i]->dims[2]));
+    TNH_HIP(hipStreamSynchronize(ctx->stream));
+    if (energy) *energy = cost;
+    if (maxbonddim) *maxbonddim = D;
+    if (max_discarded) *max_discarded = maxdisc;
+  });
+}
+
+int tnh_dmrg_energy(tnh_dmrg *st, int precision, double *energy) {
+  if (!st) return TNH_ERR_ARG;
+  (void)precision;
+  return tnh_guard(st->ctx, [&] {
+    // calculate(Hs) at the projection centre, projmps.jl:192-216 (summed over terms)
+    double e = 0;
+    const int c = st->env_center;
+    TNH_REQUIRE(c >= 1 && c <= st->N, TNH_ERR_ARG, "environment centre not set");
+    TNH_REQUIRE(st->psi_center == c, TNH_ERR_ARG,
+                "energy: the MPS centre is not at the environment centre");
+    for (int t = 0; t < st->nterms; ++t) {
+      double im = 0;
+      e += st->coeffs[t] * env_calculate(st->ctx, st->block(t, c - 1), st->psi[c - 1],
+                                         st->W[t][c - 1], st->block(t, c + 1), &im);
+    }
+    if (energy) *energy = e;
+  });
+}
+
+int tnh_dmrg_site(tnh_dmrg *st, int site, const tnh_buf **out) {
+  if (!st || !out) return TNH_ERR_ARG;
+  if (site < 1 || site > st->N) {
+    st->ctx->err = "site out of range";
+    return TNH_ERR_ARG;
+  }
+  *out = st->psi[site - 1];
+  return TNH_OK;
+}
+
+int tnh_dmrg_center(tnh_dmrg *st, int *center) {
+  if (!st || !center) return TNH_ERR_ARG;
+  *center = st->psi_center;
+  return TNH_OK;
+}
+
+int tnh_dmrg_bonddims(tnh_dmrg *st, int64_t *dims) {
+  if (!st || !dims) return TNH_ERR_ARG;
+  dims[0] = st->psi[0]->dims[0];
+  for (int i = 0; i < st->N; ++i) dims[i + 1] = st->psi[i]->dims[2];
+  return TNH_OK;
+}
+
+}  // extern "C"

@@ -42,6 +42,8 @@ void gemm(tnh_ctx *ctx, const Gemm &g, int prec) {
   p.beta = g.beta;
   p.tiles_m = (g.M + bm - 1) / bm;
   p.tiles_n = (g.N + bn - 1) / bn;
+  static const char *kFamily[3] = {"gemm_f64", "gemm_f32", "gemm_bf16x3"};
+  hipEvent_t ev = ctx->prof_begin();
   switch (prec) {
     case TNH_PREC_F64:
       gk::launch_f64(ctx->stream, p, g.batch, g.transA, g.transB, g.tA, g.tB, g.tC, big);
@@ -56,6 +58,13 @@ void gemm(tnh_ctx *ctx, const Gemm &g, int prec) {
       throw Error(TNH_ERR_ARG, "unknown precision " + std::to_string(prec));
   }
   TNH_HIP(hipGetLastError());
+  if (ev) {
+    const double mnk = double(g.M) * g.N * g.K * g.batch;
+    const double bytes = (double(g.M) * g.K * real_size(g.tA) + double(g.K) * g.N * real_size(g.tB) +
+                          double(g.M) * g.N * real_size(g.tC) * (g.beta != 0.0 ? 2 : 1)) *
+                         g.batch;
+    ctx->prof_end(kFamily[prec], ev, 2.0 * mnk, bytes);
+  }
 }
 
 // Planar complex product from real products:

@@ -135,6 +135,9 @@ int grid1(tnh_ctx *ctx, int64_t n) {
 
 }  // namespace
 
+double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, int kdim,
+                          int maxiter, double tol, int *nmv_out);
+
 // Symmetric eigenproblem of a small dense matrix (column-major n x n) by cyclic
 // Jacobi rotations; eigenvalues ascending in w, eigenvectors in the columns of Z.
 void jacobi_eig_sym_host(int n, std::vector<double> A, std::vector<double> &w,
@@ -215,6 +218,14 @@ static void herm_lowest(int m, const std::vector<std::complex<double>> &H, doubl
 double eigsolve_ground(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *W1, const tnh_buf *W2,
                        const tnh_buf *R, tnh_buf *theta, int kdim, int maxiter, double tol,
                        double coeff, int prec, int *nmv_out) {
+  return eigsolve_ground_op(
+      ctx,
+      [&](const tnh_buf *x, tnh_buf *y) { heff_apply(ctx, L, W1, W2, R, x, y, coeff, 0.0, prec); },
+      theta, kdim, maxiter, tol, nmv_out);
+}
+
+double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, int kdim,
+                          int maxiter, double tol, int *nmv_out) {
   TNH_REQUIRE(theta && theta->ndim == 4, TNH_ERR_SHAPE, "theta must be (chi_l, d, d, chi_r)");
   TNH_REQUIRE(theta->dtype == TNH_F64 || theta->dtype == TNH_C128, TNH_ERR_ARG,
               "theta must be F64 or C128 (Krylov vectors are FP64)");
@@ -278,7 +289,7 @@ double eigsolve_ground(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *W1, const 
     bool breakdown = false;
     for (int j = start; j < kdim; ++j) {
       tnh_buf vj = view(V + j * vstride), wb = view(Wv);
-      heff_apply(ctx, L, W1, W2, R, &vj, &wb, coeff, 0.0, prec);
+      matvec(&vj, &wb);
       ++nmv;
       for (int pass = 0; pass < 2; ++pass) {
         dots(j + 1, Wv);

@@ -117,7 +117,8 @@ __global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__
                                                       int n, const double *__restrict__ partial,
                                                       int nsplit, bool two_sided, bool cplx,
                                                       const int *__restrict__ pairs, int cols,
-                                                      double tol, double *__restrict__ Qout,
+                                                      double tol, double abstol,
+                                                      double *__restrict__ Qout,
                                                       double *__restrict__ Dout,
                                                       int *__restrict__ active,
                                                       unsigned long long *__restrict__ conv) {
@@ -159,7 +160,7 @@ __global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__
     const int i = e % JP, j = e / JP;
     if (i == j) continue;
     const double b = hypot(gr[i][j], gi[i][j]);
-    if (b == 0) continue;
+    if (b <= abstol) continue;
     const double s = sqrt(fabs(gr[i][i] * gr[j][j]));
     m = fmax(m, s > 0 ? b / s : 1.0);
   }
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__
         const double a = gr[p][p], c = gr[q][q];
         const double br = gr[p][q], bi = gi[p][q];
         const double bb = hypot(br, bi);
-        const bool rot = bb > 0 && bb > tol * sqrt(fabs(a * c));
+        const bool rot = bb > abstol && bb > tol * sqrt(fabs(a * c));
         double cs = 1, sn = 0, tt = 0, er = 1, ei = 0;
         if (rot) {
           const double tau = (c - a) / (2 * bb);
@@ -371,6 +372,24 @@ __global__ __launch_bounds__(256) void jac_rowupd_kernel(double *__restrict__ G,
   }
 }
 
+// G <- (G + G^H) / 2: the MFMA product M M^H is not bitwise Hermitian (G[i,j] and G[j,i]
+// come from different tiles), and the Jacobi rotations assume it is.
+__global__ void hermitize_kernel(double *G, int n, int64_t imG, bool cplx) {
+  const int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (e >= int64_t(n) * n) return;
+  const int i = int(e % n), j = int(e / n);
+  if (i > j) return;
+  const int64_t a = i + int64_t(j) * n, b = j + int64_t(i) * n;
+  const double re = 0.5 * (G[a] + G[b]);
+  G[a] = re;
+  G[b] = re;
+  if (cplx) {
+    const double im = 0.5 * (G[imG + a] - G[imG + b]);
+    G[imG + a] = (i == j) ? 0.0 : im;
+    G[imG + b] = (i == j) ? 0.0 : -im;
+  }
+}
+
 __global__ void eye_kernel(double *V, int n, int64_t imV, bool cplx) {
   const int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   const int64_t nn = int64_t(n) * n;
@@ -485,7 +504,7 @@ static Schedule schedule_for(tnh_ctx *ctx, int cols) {
 // two_sided: diagonalise Hermitian G (n x n) in place, V <- V * rotations.
 // one-sided: orthogonalise the columns of X (rows x n) in place, V (n x n) accumulates.
 static JacobiResult jacobi(tnh_ctx *ctx, bool two_sided, double *X, int rows, int n, bool cplx,
-                           double *V, double tol, int max_sweeps) {
+                           double *V, double tol, double abstol, int max_sweeps) {
   JacobiResult res;
   const Schedule s = schedule_for(ctx, n);
   const int64_t imX = int64_t(rows) * n, imV = int64_t(n) * n;
@@ -513,7 +532,8 @@ static JacobiResult jacobi(tnh_ctx *ctx, bool two_sided, double *X, int rows, in
         hipLaunchKernelGGL(jac_gram_kernel, dim3(s.npairs, nsplit), dim3(256), 0, ctx->stream, X,
                            rows, imX, n, cplx, pr, nsplit, partial);
       hipLaunchKernelGGL(jac_eig_kernel, dim3(s.npairs), dim3(256), 0, ctx->stream, X, imX, n,
-                         partial, nsplit, two_sided, cplx, pr, n, tol, Q, D, active, conv + sw);
+                         partial, nsplit, two_sided, cplx, pr, n, tol, abstol, Q, D, active,
+                         conv + sw);
       hipLaunchKernelGGL(jac_colupd_kernel, dim3((rmax + 255) / 256, s.npairs), dim3(256), 0,
                          ctx->stream, X, rows, imX, V, n, imV, cplx, pr, n, Q, active);
       if (two_sided)
@@ -616,9 +636,23 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
       A.conj = true;
     }
     cgemm(ctx, g, A, B, G, cplx ? G + pp : nullptr, TNH_F64, TNH_PREC_F64);
+    hipLaunchKernelGGL(hermitize_kernel, dim3(unsigned((pp + 255) / 256)), dim3(256), 0, ctx->stream,
+                       G, p, pp, cplx);
+    // Entries below 4 eps sqrt(p) max_i G_ii are rounding noise: each rotation of a
+    // large pair leaves ~eps * lambda_max in the rows it touches, so a purely relative
+    // test on noise-level eigenvalues never settles.
+    double dmax = 0;
+    {
+      double *dv = static_cast<double *>(ctx->workspace(612, sizeof(double) * p));
+      hipLaunchKernelGGL(diag_kernel, dim3((p + 255) / 256), dim3(256), 0, ctx->stream, G, p, dv);
+      TNH_HIP(hipGetLastError());
+      to_host(ctx, val.data(), dv, sizeof(double) * p);
+      for (double v : val) dmax = std::max(dmax, std::fabs(v));
+    }
     hipLaunchKernelGGL(eye_kernel, dim3(unsigned((pp + 255) / 256)), dim3(256), 0, ctx->stream, V, p,
                        pp, cplx);
-    JacobiResult jr = jacobi(ctx, true, G, p, p, cplx, V, 1e-14, 40);
+    const double abstol = 4.0 * 2.220446049250313e-16 * std::sqrt(double(p)) * dmax;
+    JacobiResult jr = jacobi(ctx, true, G, p, p, cplx, V, 1e-14, abstol, 40);
     if (jr.converged) {
       double *dv = static_cast<double *>(ctx->workspace(612, sizeof(double) * p));
       hipLaunchKernelGGL(diag_kernel, dim3((p + 255) / 256), dim3(256), 0, ctx->stream, G, p, dv);
@@ -641,19 +675,35 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
       TNH_HIP(hipMemcpyAsync(X, Mp, sizeof(double) * int64_t(m) * n * planes,
                              hipMemcpyDeviceToDevice, ctx->stream));
     } else {
-      // X = M^H (n x m): adjoint copy with the identity permutation
-      std::vector<int> id(m);
+      // X = M^H (n x m): X[c, r] = conj(M[r, c]), i.e. the adjoint gather of all n
+      // columns of the m-row matrix M
+      std::vector<int> id(n);
       std::iota(id.begin(), id.end(), 0);
-      int *dperm = static_cast<int *>(ctx->workspace(614, sizeof(int) * m));
-      TNH_HIP(hipMemcpyAsync(dperm, id.data(), sizeof(int) * m, hipMemcpyHostToDevice, ctx->stream));
-      dim3 grid((n + 31) / 32, (m + 31) / 32);
-      hipLaunchKernelGGL(gather_cols_adjoint_kernel, grid, dim3(256), 0, ctx->stream, Mp, n,
-                         int64_t(m) * n, dperm, nullptr, m, X, int64_t(n) * m, cplx);
+      int *dperm = static_cast<int *>(ctx->workspace(614, sizeof(int) * n));
+      TNH_HIP(hipMemcpyAsync(dperm, id.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+      dim3 grid((m + 31) / 32, (n + 31) / 32);
+      hipLaunchKernelGGL(gather_cols_adjoint_kernel, grid, dim3(256), 0, ctx->stream, Mp, m,
+                         int64_t(m) * n, dperm, nullptr, n, X, int64_t(n) * m, cplx);
       TNH_HIP(hipStreamSynchronize(ctx->stream));
     }
     hipLaunchKernelGGL(eye_kernel, dim3(unsigned((pp + 255) / 256)), dim3(256), 0, ctx->stream, V, p,
                        pp, cplx);
-    JacobiResult jr = jacobi(ctx, false, X, rows, p, cplx, V, 1e-13, 60);
+    // noise floor in Gram units: (4 eps sqrt(rows))^2 ||M||_F^2 (columns below
+    // ~4 eps sqrt(rows) ||M||_F are re-seeded by every rotation with a large column)
+    double fro2 = 0;
+    {
+      tnh_buf v;
+      v.dtype = TNH_F64;
+      v.ndim = 1;
+      v.numel = int64_t(m) * n * planes;
+      v.dims[0] = v.numel;
+      v.ptr = const_cast<double *>(Mp);
+      v.owned = false;
+      double im = 0;
+      cdot_host(ctx, &v, &v, &fro2, &im);
+    }
+    const double f1 = 4.0 * 2.220446049250313e-16 * std::sqrt(double(rows));
+    JacobiResult jr = jacobi(ctx, false, X, rows, p, cplx, V, 1e-13, f1 * f1 * fro2, 60);
     TNH_REQUIRE(jr.converged, TNH_ERR_NOCONV,
                 "svd: one-sided Jacobi did not converge (measure " + std::to_string(jr.measure) + ")");
     double *dv = static_cast<double *>(ctx->workspace(612, sizeof(double) * p));
@@ -709,6 +759,15 @@ static double frob(tnh_ctx *ctx, const tnh_buf *b) {
   return std::sqrt(std::max(re, 0.0));
 }
 
+// Factor the m x n matrix of `src` (any dtype; its first `split_at` dims are rows) as
+// left * right with truncation: side 0 -> left has orthonormal columns (U_k) and
+// right = U_k^H M (= S V^H); side 1 -> right has orthonormal rows (V_k^H) and
+// left = M V_k (= U S). left / right are (re)allocated with the given dims.
+// Returns the kept rank; all singular values go to o->s.
+int split_matrix(tnh_ctx *ctx, const tnh_buf *src, int m, int n, int side, double cutoff,
+                 int maxdim, int mindim, bool force_fallback, int nl, const int64_t *ldims,
+                 int nr, const int64_t *rdims, tnh_buf **A1, tnh_buf **A2, SvdOut *o);
+
 void replacesites2(tnh_ctx *ctx, const tnh_buf *theta, int direction, int normalize,
                    double cutoff, int maxdim, int mindim, tnh_buf **A1, tnh_buf **A2, int *k_out,
                    double *disc_out, double *schmidt, int schmidt_cap, bool force_fallback) {
@@ -717,16 +776,40 @@ void replacesites2(tnh_ctx *ctx, const tnh_buf *theta, int direction, int normal
   const int64_t cl = theta->dims[0], d1 = theta->dims[1], d2 = theta->dims[2], cr = theta->dims[3];
   const int m = int(cl * d1), n = int(d2 * cr);
   TNH_REQUIRE(m > 0 && n > 0, TNH_ERR_SHAPE, "empty theta");
-  const bool cplx = is_complex(theta->dtype);
-  const double *M = as_f64(ctx, theta, 630);
   SvdOut o;
   const int side = direction ? 1 : 0;
-  double *basis = decompose(ctx, M, m, n, cplx, side, cutoff, maxdim, mindim, force_fallback, &o);
-  const int k = o.k;
+  int64_t d1v[3] = {cl, d1, 0}, d2v[3] = {0, d2, cr};
+  const int k = split_matrix(ctx, theta, m, n, side, cutoff, maxdim, mindim, force_fallback, 3,
+                             d1v, 3, d2v, A1, A2, &o);
+  if (normalize) {
+    // normalize!(psi): divide the centre tensor by its norm, gmps.jl:46-51
+    tnh_buf *c = side == 0 ? *A2 : *A1;
+    const double nr = frob(ctx, c);
+    TNH_REQUIRE(std::isfinite(nr) && nr > 0, TNH_ERR_NUMERIC, "replacesites: zero centre tensor");
+    scale(ctx, c->ptr, c->dtype, c->numel, 1.0 / nr);
+  }
+  if (k_out) *k_out = k;
+  if (disc_out) *disc_out = o.discarded;
+  if (schmidt)
+    for (int i = 0; i < std::min(k, schmidt_cap); ++i) schmidt[i] = o.s[i];
+}
+
+int split_matrix(tnh_ctx *ctx, const tnh_buf *src, int m, int n, int side, double cutoff,
+                 int maxdim, int mindim, bool force_fallback, int nl, const int64_t *ldims_in,
+                 int nr, const int64_t *rdims_in, tnh_buf **A1, tnh_buf **A2, SvdOut *o) {
+  TNH_REQUIRE(int64_t(m) * n == src->numel, TNH_ERR_SHAPE, "split: matrix size != tensor size");
+  const bool cplx = is_complex(src->dtype);
+  const double *M = as_f64(ctx, src, 630);
+  double *basis = decompose(ctx, M, m, n, cplx, side, cutoff, maxdim, mindim, force_fallback, o);
+  const int k = o->k;
   const int dt = cplx ? TNH_C128 : TNH_F64;
-  int64_t d1v[3] = {cl, d1, k}, d2v[3] = {k, d2, cr};
-  buf_ensure(ctx, A1, dt, 3, d1v);
-  buf_ensure(ctx, A2, dt, 3, d2v);
+  int64_t d1v[8], d2v[8];
+  for (int i = 0; i < nl; ++i) d1v[i] = ldims_in[i];
+  for (int i = 0; i < nr; ++i) d2v[i] = rdims_in[i];
+  d1v[nl - 1] = k;  // left factor: rows..., k
+  d2v[0] = k;       // right factor: k, cols...
+  buf_ensure(ctx, A1, dt, nl, d1v);
+  buf_ensure(ctx, A2, dt, nr, d2v);
   COperand Mo;
   Mo.re = M;
   Mo.im = cplx ? M + int64_t(m) * n : nullptr;
@@ -770,17 +853,7 @@ void replacesites2(tnh_ctx *ctx, const tnh_buf *theta, int direction, int normal
     cgemm(ctx, g, Mo, Vk, (*A1)->re(), cplx ? (*A1)->im() : nullptr, TNH_F64, TNH_PREC_F64);
     TNH_HIP(hipStreamSynchronize(ctx->stream));  // id lives on the host stack
   }
-  if (normalize) {
-    // normalize!(psi): divide the centre tensor by its norm, gmps.jl:46-51
-    tnh_buf *c = side == 0 ? *A2 : *A1;
-    const double nr = frob(ctx, c);
-    TNH_REQUIRE(std::isfinite(nr) && nr > 0, TNH_ERR_NUMERIC, "replacesites: zero centre tensor");
-    scale(ctx, c->ptr, c->dtype, c->numel, 1.0 / nr);
-  }
-  if (k_out) *k_out = k;
-  if (disc_out) *disc_out = o.discarded;
-  if (schmidt)
-    for (int i = 0; i < std::min(k, schmidt_cap); ++i) schmidt[i] = o.s[i];
+  return k;
 }
 
 void svd_trunc(tnh_ctx *ctx, const tnh_buf *Mb, double cutoff, int maxdim, int mindim,
@@ -883,11 +956,115 @@ void svd_trunc(tnh_ctx *ctx, const tnh_buf *Mb, double cutoff, int maxdim, int m
   if (fb_out) *fb_out = o.fallback ? 1 : 0;
 }
 
+// theta = A * B over the shared bond for a rank-1 pair of site tensors, output (k, d, chi)
+// as a plain matrix product (rows of A fused, columns of B fused).
+static void matmul_sites(tnh_ctx *ctx, const tnh_buf *A, int64_t arows, const tnh_buf *B,
+                         int64_t bcols, int nd, const int64_t *dims, tnh_buf **out) {
+  const int64_t kk = A->numel / arows;
+  TNH_REQUIRE(kk * bcols == B->numel, TNH_ERR_SHAPE, "matmul_sites: bond mismatch");
+  const bool c = is_complex(A->dtype) || is_complex(B->dtype);
+  tnh_buf *res = nullptr;
+  buf_ensure(ctx, &res, c ? TNH_C128 : TNH_F64, nd, dims);
+  Gemm g;
+  g.M = int(arows);
+  g.N = int(bcols);
+  g.K = int(kk);
+  g.lda = arows;
+  g.ldb = kk;
+  g.ldc = arows;
+  COperand a, b;
+  a.re = A->re();
+  a.im = is_complex(A->dtype) ? A->im() : nullptr;
+  a.dtype = real_of(A->dtype);
+  b.re = B->re();
+  b.im = is_complex(B->dtype) ? B->im() : nullptr;
+  b.dtype = real_of(B->dtype);
+  try {
+    cgemm(ctx, g, a, b, res->re(), c ? res->im() : nullptr, TNH_F64, TNH_PREC_F64);
+  } catch (...) {
+    buf_delete(ctx, res);
+    throw;
+  }
+  buf_delete(ctx, *out);
+  *out = res;
+}
+
+// moveright!(psi, idx), src/structures/mps/gmps.jl:75-82: A -> U, B -> (S V) B.
+void mps_moveright(tnh_ctx *ctx, tnh_buf **A, tnh_buf **B, double cutoff, int maxdim, int mindim) {
+  const int64_t ca = (*A)->dims[0], d = (*A)->dims[1], cb = (*A)->dims[2];
+  TNH_REQUIRE((*B)->ndim == 3 && (*B)->dims[0] == cb, TNH_ERR_SHAPE, "moveright: bond mismatch");
+  SvdOut o;
+  tnh_buf *U = nullptr, *SV = nullptr;
+  int64_t ld[3] = {ca, d, 0}, rd[2] = {0, cb};
+  try {
+    split_matrix(ctx, *A, int(ca * d), int(cb), 0, cutoff, maxdim, mindim, false, 3, ld, 2, rd,
+                 &U, &SV, &o);
+    int64_t nb[3] = {int64_t(o.k), (*B)->dims[1], (*B)->dims[2]};
+    matmul_sites(ctx, SV, o.k, *B, (*B)->dims[1] * (*B)->dims[2], 3, nb, B);
+  } catch (...) {
+    buf_delete(ctx, U);
+    buf_delete(ctx, SV);
+    throw;
+  }
+  buf_delete(ctx, SV);
+  buf_delete(ctx, *A);
+  *A = U;
+}
+
+// moveleft!(psi, idx), src/structures/mps/gmps.jl:60-67: B -> V^H, A -> A (U S).
+void mps_moveleft(tnh_ctx *ctx, tnh_buf **A, tnh_buf **B, double cutoff, int maxdim, int mindim) {
+  const int64_t ca = (*B)->dims[0], d = (*B)->dims[1], cb = (*B)->dims[2];
+  TNH_REQUIRE((*A)->ndim == 3 && (*A)->dims[2] == ca, TNH_ERR_SHAPE, "moveleft: bond mismatch");
+  SvdOut o;
+  tnh_buf *US = nullptr, *Vh = nullptr;
+  int64_t ld[2] = {ca, 0}, rd[3] = {0, d, cb};
+  try {
+    split_matrix(ctx, *B, int(ca), int(d * cb), 1, cutoff, maxdim, mindim, false, 2, ld, 3, rd,
+                 &US, &Vh, &o);
+    int64_t na[3] = {(*A)->dims[0], (*A)->dims[1], int64_t(o.k)};
+    matmul_sites(ctx, *A, (*A)->dims[0] * (*A)->dims[1], US, o.k, 3, na, A);
+  } catch (...) {
+    buf_delete(ctx, US);
+    buf_delete(ctx, Vh);
+    throw;
+  }
+  buf_delete(ctx, US);
+  buf_delete(ctx, *B);
+  *B = Vh;
+}
+
+// movecenter!(psi, idx; kwargs), src/structures/mps/gmps.jl:90-112 (1-based sites).
+void mps_movecenter(tnh_ctx *ctx, int N, tnh_buf **s, int center, int idx, double cutoff,
+                    int maxdim, int mindim) {
+  TNH_REQUIRE(idx >= 1 && idx <= N, TNH_ERR_ARG, "The index is out of range.");
+  for (int i = 0; i < N; ++i)
+    TNH_REQUIRE(s[i] && s[i]->ndim == 3, TNH_ERR_SHAPE, "site tensors must be (chi_l, d, chi_r)");
+  auto right = [&](int i) { mps_moveright(ctx, &s[i - 1], &s[i], cutoff, maxdim, mindim); };
+  auto left = [&](int i) { mps_moveleft(ctx, &s[i - 2], &s[i - 1], cutoff, maxdim, mindim); };
+  if (center == 0) {
+    for (int i = 1; i < idx; ++i) right(i);
+    for (int i = 1; i <= N - idx; ++i) left(N + 1 - i);
+  } else if (idx > center) {
+    for (int i = center; i < idx; ++i) right(i);
+  } else if (idx < center) {
+    for (int i = 1; i <= center - idx; ++i) left(center + 1 - i);
+  }
+}
+
 }  // namespace tnh
 
 using namespace tnh;
 
 extern "C" {
+
+int tnh_mps_movecenter(tnh_ctx *ctx, int N, tnh_buf **sites, int center, int new_center,
+                       double cutoff, int maxdim, int mindim) {
+  if (!ctx || !sites) return TNH_ERR_ARG;
+  return tnh_guard(ctx, [&] {
+    TNH_REQUIRE(N >= 1 && center >= 0 && center <= N, TNH_ERR_ARG, "bad N or centre");
+    mps_movecenter(ctx, N, sites, center, new_center, cutoff, maxdim, mindim);
+  });
+}
 
 int tnh_svd_trunc(tnh_ctx *ctx, const tnh_buf *M, double cutoff, int maxdim, int mindim,
                   int force_fallback, tnh_buf **U, tnh_buf **S, tnh_buf **Vt, int *k,

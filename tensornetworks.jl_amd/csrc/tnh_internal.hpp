@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstring>
 #include <exception>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -86,6 +87,25 @@ struct tnh_ctx {
   // The 1x1x1 ones block standing for environments beyond the chain ends.
   tnh_buf edge;
 
+  // Optional kernel timing: HIP event pairs per kernel family, resolved on read.
+  struct ProfRec {
+    hipEvent_t a, b;
+    double flops, bytes;
+  };
+  bool profiling = false;
+  std::map<std::string, std::vector<ProfRec>> prof_pending;
+  struct ProfSum {
+    int64_t launches = 0;
+    double ms = 0, flops = 0, bytes = 0;
+  };
+  std::map<std::string, ProfSum> prof_done;
+  std::vector<hipEvent_t> event_pool;
+  hipEvent_t take_event();
+  // bracket one launch: call prof_begin before and prof_end after the launch
+  hipEvent_t prof_begin();
+  void prof_end(const char *family, hipEvent_t start, double flops, double bytes);
+  void prof_resolve();
+
   // Pinned host staging for small device->host scalar transfers.
   double *host_scalars = nullptr;
   static constexpr int kHostScalars = 4096;
@@ -156,6 +176,31 @@ void axpby(tnh_ctx *ctx, int64_t n, double a, const double *x, double b, double 
 // FP64 reductions with deterministic two-stage sums; result stays on the device
 // (dst) so that chains of Krylov steps need no host round trip.
 void dot_f64(tnh_ctx *ctx, int64_t n, const double *x, const double *y, double *dst);
+
+// -------------------------------------------------------- hot-path operators
+void heff_apply(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *W1, const tnh_buf *W2,
+                const tnh_buf *R, const tnh_buf *x, tnh_buf *out, double cr, double ci, int prec);
+void env_left(tnh_ctx *ctx, const tnh_buf *Lp, const tnh_buf *A, const tnh_buf *W,
+              tnh_buf **Lout, int prec);
+void env_right(tnh_ctx *ctx, const tnh_buf *Rp, const tnh_buf *A, const tnh_buf *W,
+               tnh_buf **Rout, int prec);
+double env_calculate(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *A, const tnh_buf *W,
+                     const tnh_buf *R, double *im_out);
+void two_site_theta(tnh_ctx *ctx, const tnh_buf *A1, const tnh_buf *A2, tnh_buf **theta);
+void cscale(tnh_ctx *ctx, void *x, int t, int64_t n, double cr, double ci);
+void cdot_host(tnh_ctx *ctx, const tnh_buf *x, const tnh_buf *y, double *re, double *im);
+
+// y = H_eff x for the Krylov solver
+using Matvec = std::function<void(const tnh_buf *x, tnh_buf *y)>;
+double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, int kdim,
+                          int maxiter, double tol, int *nmv_out);
+void replacesites2(tnh_ctx *ctx, const tnh_buf *theta, int direction, int normalize,
+                   double cutoff, int maxdim, int mindim, tnh_buf **A1, tnh_buf **A2, int *k_out,
+                   double *disc_out, double *schmidt, int schmidt_cap, bool force_fallback);
+void mps_movecenter(tnh_ctx *ctx, int N, tnh_buf **s, int center, int idx, double cutoff,
+                    int maxdim, int mindim);
+void mps_moveleft(tnh_ctx *ctx, tnh_buf **A, tnh_buf **B, double cutoff, int maxdim, int mindim);
+void mps_moveright(tnh_ctx *ctx, tnh_buf **A, tnh_buf **B, double cutoff, int maxdim, int mindim);
 
 }  // namespace tnh
 

@@ -58,6 +58,9 @@ SIGNATURES = {
     "tnh_ctx_destroy": (C.c_int, [ctx_p]),
     "tnh_last_error": (C.c_char_p, [ctx_p]),
     "tnh_ctx_synchronize": (C.c_int, [ctx_p]),
+    "tnh_profile_enable": (C.c_int, [ctx_p, C.c_int]),
+    "tnh_profile_reset": (C.c_int, [ctx_p]),
+    "tnh_profile_read": (C.c_int, [ctx_p, C.c_char_p, i64p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "tnh_buf_create": (C.c_int, [ctx_p, C.c_int, C.c_int, i64p, C.POINTER(buf_p)]),
     "tnh_buf_upload": (C.c_int, [ctx_p, C.c_void_p, C.c_int, C.c_int, C.c_int, i64p, C.POINTER(buf_p)]),
     "tnh_buf_download": (C.c_int, [ctx_p, buf_p, C.c_void_p, C.c_int]),
@@ -75,6 +78,7 @@ SIGNATURES = {
     "tnh_svd_trunc": (C.c_int, [ctx_p, buf_p, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(buf_p), C.POINTER(buf_p), C.POINTER(buf_p), C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "tnh_replacesites2": (C.c_int, [ctx_p, buf_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(buf_p), C.POINTER(buf_p), C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]),
     "tnh_two_site_theta": (C.c_int, [ctx_p, buf_p, buf_p, C.POINTER(buf_p)]),
+    "tnh_mps_movecenter": (C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]),
     "tnh_applygate2": (C.c_int, [ctx_p, C.POINTER(buf_p), C.POINTER(buf_p), buf_p, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "tnh_dmrg_create": (C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.c_int, C.POINTER(buf_p), C.POINTER(C.c_double), C.POINTER(C.c_void_p)]),
     "tnh_dmrg_destroy": (C.c_int, [C.c_void_p]),

@@ -71,6 +71,19 @@ class Context:
     def synchronize(self):
         self.call("tnh_ctx_synchronize")
 
+    # kernel timing with HIP events on this context's stream
+    def profile(self, on: bool = True):
+        self.call("tnh_profile_enable", int(bool(on)))
+
+    def profile_reset(self):
+        self.call("tnh_profile_reset")
+
+    def profile_read(self, family: str) -> dict:
+        n = C.c_int64()
+        ms, fl, by = C.c_double(), C.c_double(), C.c_double()
+        self.call("tnh_profile_read", family.encode(), C.byref(n), C.byref(ms), C.byref(fl), C.byref(by))
+        return {"launches": n.value, "ms": ms.value, "flops": fl.value, "bytes": by.value}
+
     def close(self):
         if self.h:
             lib.tnh_ctx_destroy(self.h)
