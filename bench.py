@@ -60,6 +60,22 @@ def make_inputs(chi, D, seed, torch):
     return L, R, th
 
 
+def measured_traffic(chi):
+    """HBM bytes per launch of the FP32 GEMM (mean of the two stages), from the
+    copy-calibrated FETCH_SIZE + WRITE_SIZE counter runs summarised in
+    profiles/heff_matvec_chi2048_hbm_traffic.md (valid for the chi = 2048 workload)."""
+    if chi != 2048:
+        return None
+    p = ROOT / "profiles" / "heff_matvec_chi2048_hbm_traffic.md"
+    try:
+        for line in p.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(budget_s=12.0):
     """FP64 CPU port (oracle, optimal contraction order, numpy/OpenBLAS on all host
     cores) on a bounded sample of the same workload: matvecs at chi = 512 and 1024."""
@@ -118,7 +134,22 @@ def main(argv=None):
     ap.add_argument("--no-svd", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 1 GiB FP64 device copy (reformat kernel), the "
+                         "known-size reference for scaling FETCH_SIZE / WRITE_SIZE counters")
     a = ap.parse_args(argv)
+    if a.copy_calibration:
+        import tnhip
+        from tnhip import Tensor
+
+        n = 1 << 27  # 2^27 doubles = 1 GiB read + 1 GiB written
+        src = Tensor.empty((n,), np.float64)
+        dst = src.astype(np.float64)
+        tnhip.Context.default().synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 8 * n, "copy_calibration_bytes_written": 8 * n,
+                          "kernel": "reformat_kernel<double,double>"}))
+        del dst
+        return 0
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the matvec would shard theta along its right bond with a reduce-scatter; "
@@ -162,7 +193,7 @@ def main(argv=None):
     gemm_flops = g["flops"] / max(g["launches"], 1)
     achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK["f32"], "unit": "TFLOP/s",
-                "frac": achieved / PEAK["f32"], "traffic": None,
+                "frac": achieved / PEAK["f32"], "traffic": measured_traffic(chi),
                 "kernel": "gemm_kernel<MODE_F32> (stage 1 L*theta and stage 3 T2*R^T)",
                 "flops_per_launch": gemm_flops, "avg_launch_ms": gemm_ms}
     mix = f32["mix"]
