@@ -104,6 +104,16 @@ int tnh_buf_shape(const tnh_buf *buf, int *dtype, int *ndim, int64_t *dims);
 void *tnh_buf_device_ptr(const tnh_buf *buf);
 /* Device-to-device copy with dtype conversion into a new buffer. */
 int tnh_buf_convert(tnh_ctx *ctx, const tnh_buf *src, int dtype, tnh_buf **out);
+/* Frobenius norm of a device tensor: norm(psi) at the centre, src/structures/mps/gmps.jl:29-37. */
+int tnh_buf_norm(tnh_ctx *ctx, const tnh_buf *buf, double *norm);
+/* In-place scaling by (re + i im) (promotes a real tensor to complex when im != 0):
+ * psi[center] *= x, gmps.jl:46-51 and abstractmps.jl:100-108. */
+int tnh_buf_scale(tnh_ctx *ctx, tnh_buf *buf, double re, double im);
+/* A[a,o,b] <- sum_i G[o,i] A[a,i,b] for a (chi_l, d, chi_r) site tensor and a (d, d)
+ * operator; *A is replaced (its dtype may become complex). Replaces applyop!,
+ * src/structures/mps/mps.jl:141-151, and one-site gates of applygate,
+ * src/structures/mps/gatelist.jl:137-157 (rng = 1). */
+int tnh_apply_site_op(tnh_ctx *ctx, tnh_buf **A, const tnh_buf *G);
 
 /* ------------------------------------------------------- hot-path operators */
 
@@ -194,6 +204,16 @@ int tnh_two_site_theta(tnh_ctx *ctx, const tnh_buf *A1, const tnh_buf *A2, tnh_b
  * A1 and A2 are replaced in place by the split factors. */
 int tnh_applygate2(tnh_ctx *ctx, tnh_buf **A1, tnh_buf **A2, const tnh_buf *G, int direction,
                    double cutoff, int maxdim, int mindim, int *k, double *discarded);
+
+/* Local expectation values <psi| O_t |psi> (unnormalised) of nterms operator-list terms
+ * over the N device site tensors of psi. Replaces inner(st, psi, oplist, psi),
+ * src/structures/mps/mps.jl:87-134 (coefficients are applied by the caller). Term t acts
+ * on sites first_site[t] .. first_site[t] + range[t] - 1 (1-based); `ops` holds, term
+ * after term, range[t] d x d operator matrices O[s_out, s_in], column-major, interleaved
+ * complex (identity where a term has no operator). out receives 2 * nterms doubles
+ * (re, im). */
+int tnh_inner_oplist(tnh_ctx *ctx, int N, const tnh_buf *const *psi, int nterms,
+                     const int *first_site, const int *range, const double *ops, double *out);
 
 /* ---------------------------------------------------- device-resident DMRG */
 

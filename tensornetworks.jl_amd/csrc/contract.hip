@@ -715,6 +715,115 @@ int tnh_two_site_theta(tnh_ctx *ctx, const tnh_buf *A1, const tnh_buf *A2, tnh_b
   return tnh_guard(ctx, [&] { two_site_theta(ctx, A1, A2, theta); });
 }
 
+int tnh_buf_norm(tnh_ctx *ctx, const tnh_buf *b, double *norm) {
+  if (!ctx || !b || !norm) return TNH_ERR_ARG;
+  return tnh_guard(ctx, [&] {
+    const tnh_buf *x = b;
+    tnh_buf *tmp = nullptr;
+    if (!is_double(b->dtype)) {
+      tmp = buf_new(ctx, is_complex(b->dtype) ? TNH_C128 : TNH_F64, b->ndim, b->dims);
+      convert(ctx, b->ptr, b->dtype, tmp->ptr, tmp->dtype, b->numel);
+      x = tmp;
+    }
+    tnh_buf v = *x;
+    v.dtype = TNH_F64;
+    v.numel = x->numel * (is_complex(x->dtype) ? 2 : 1);
+    double re = 0, im = 0;
+    try {
+      cdot_host(ctx, &v, &v, &re, &im);
+    } catch (...) {
+      buf_delete(ctx, tmp);
+      throw;
+    }
+    buf_delete(ctx, tmp);
+    *norm = std::sqrt(std::max(re, 0.0));
+  });
+}
+
+int tnh_buf_scale(tnh_ctx *ctx, tnh_buf *b, double re, double im) {
+  if (!ctx || !b) return TNH_ERR_ARG;
+  return tnh_guard(ctx, [&] {
+    if (im != 0.0 && !is_complex(b->dtype)) {
+      // promote to complex in place (new storage), then scale
+      tnh_buf *c = buf_new(ctx, complex_of(b->dtype), b->ndim, b->dims);
+      convert(ctx, b->ptr, b->dtype, c->ptr, c->dtype, b->numel);
+      if (b->owned) ctx->release(b->ptr, b->cap);
+      b->dtype = c->dtype;
+      b->ptr = c->ptr;
+      b->bytes = c->bytes;
+      b->cap = c->cap;
+      b->owned = true;
+      c->owned = false;
+      buf_delete(ctx, c);
+    }
+    cscale(ctx, b->ptr, b->dtype, b->numel, re, im);
+  });
+}
+
+int tnh_apply_site_op(tnh_ctx *ctx, tnh_buf **A, const tnh_buf *G) {
+  if (!ctx || !A || !*A || !G) return TNH_ERR_ARG;
+  return tnh_guard(ctx, [&] {
+    tnh_buf *a = *A;
+    TNH_REQUIRE(a->ndim == 3, TNH_ERR_SHAPE, "site tensor must be (chi_l, d, chi_r)");
+    const int64_t cl = a->dims[0], d = a->dims[1], cr = a->dims[2];
+    TNH_REQUIRE(G->ndim == 2 && G->dims[0] == d && G->dims[1] == d, TNH_ERR_SHAPE,
+                "operator must be (d, d)");
+    TNH_REQUIRE(d <= 64, TNH_ERR_UNSUPPORTED, "d > 64");
+    const bool gc = is_complex(G->dtype), ac = is_complex(a->dtype), oc = gc || ac;
+    // Wc[i, o] = G[o, i]
+    std::vector<double> hg(size_t(d) * d * (gc ? 2 : 1));
+    {
+      tnh_buf *g64 = nullptr;
+      const tnh_buf *gsrc = G;
+      if (!is_double(G->dtype)) {
+        g64 = buf_new(ctx, gc ? TNH_C128 : TNH_F64, 2, G->dims);
+        convert(ctx, G->ptr, G->dtype, g64->ptr, g64->dtype, G->numel);
+        gsrc = g64;
+      }
+      to_host(ctx, hg.data(), gsrc->ptr, hg.size() * sizeof(double));
+      buf_delete(ctx, g64);
+    }
+    std::vector<double> wc(size_t(d) * d * (gc ? 2 : 1));
+    for (int64_t o = 0; o < d; ++o)
+      for (int64_t i = 0; i < d; ++i) {
+        wc[i + d * o] = hg[o + d * i];
+        if (gc) wc[d * d + i + d * o] = hg[d * d + o + d * i];
+      }
+    double *Wc = static_cast<double *>(ctx->workspace(850, wc.size() * sizeof(double)));
+    TNH_HIP(hipMemcpyAsync(Wc, wc.data(), wc.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    tnh_buf *out = nullptr;
+    const int odt = is_double(a->dtype) ? (oc ? TNH_C128 : TNH_F64) : (oc ? TNH_C64 : TNH_F32);
+    buf_ensure(ctx, &out, odt, 3, a->dims);
+    Small s;
+    s.na = int(cl);
+    s.nb = int(cr);
+    s.I = int(d);
+    s.J = int(d);
+    s.in = a->ptr;
+    s.in_im = ac ? a->numel : 0;
+    s.tin = real_of(a->dtype);
+    s.out = out->ptr;
+    s.out_im = oc ? out->numel : 0;
+    s.tout = real_of(odt);
+    s.Wc = Wc;
+    s.Wc_cplx = gc;
+    s.sbI = cl * d;
+    s.sbJ = cl * d;
+    s.offI.resize(d);
+    s.offJ.resize(d);
+    for (int i = 0; i < d; ++i) s.offI[i] = s.offJ[i] = cl * i;
+    try {
+      small_contract(ctx, s);
+      TNH_HIP(hipStreamSynchronize(ctx->stream));  // wc lives on the host stack
+    } catch (...) {
+      buf_delete(ctx, out);
+      throw;
+    }
+    buf_delete(ctx, a);
+    *A = out;
+  });
+}
+
 int tnh_gemm(tnh_ctx *ctx, int opA, int opB, int64_t M, int64_t N, int64_t K, double alpha,
              const tnh_buf *A, int64_t lda, const tnh_buf *B, int64_t ldb, double beta,
              tnh_buf *Cb, int64_t ldc, int precision) {

@@ -24,7 +24,8 @@ void gemm(tnh_ctx *ctx, const Gemm &g, int prec) {
   // Large tiles once the product fills the chip with 128x128 blocks.
   const bool big = (int64_t((g.M + 127) / 128) * ((g.N + 127) / 128) * g.batch >= ctx->cu_count) &&
                    g.M >= 96 && g.N >= 96;
-  const int bm = big ? 128 : 64, bn = big ? 128 : 64;
+  // FP64 keeps its 64-wide accumulators at two waves per SIMD with 128 x 64 blocks
+  const int bm = big ? 128 : 64, bn = big ? (prec == TNH_PREC_F64 ? 64 : 128) : 64;
   gk::Params p;
   p.M = g.M;
   p.N = g.N;
@@ -42,6 +43,22 @@ void gemm(tnh_ctx *ctx, const Gemm &g, int prec) {
   p.beta = g.beta;
   p.tiles_m = (g.M + bm - 1) / bm;
   p.tiles_n = (g.N + bn - 1) / bn;
+  // 16-byte vector paths: every vector must lie wholly inside or outside the matrix and
+  // be 16-byte aligned, i.e. the contiguous extent, the leading dimension, the batch
+  // stride and the base address are multiples of the vector width.
+  auto aligned16 = [](const void *ptr) { return (reinterpret_cast<uintptr_t>(ptr) & 15u) == 0; };
+  auto vec_ok = [&](const void *ptr, int dtype, int64_t contiguous_extent, int64_t ld, int64_t stride) {
+    const int64_t vw = 16 / int64_t(real_size(dtype));
+    return aligned16(ptr) && contiguous_extent % vw == 0 && ld % vw == 0 && stride % vw == 0;
+  };
+  // A: k-contiguous when transposed (extent K), else row-contiguous (extent M)
+  p.vecA = vec_ok(g.A, g.tA, g.transA ? g.K : g.M, g.lda, g.sA);
+  // B: k-contiguous when not transposed (extent K), else row-contiguous (extent N)
+  p.vecB = vec_ok(g.B, g.tB, g.transB ? g.N : g.K, g.ldb, g.sB);
+  {
+    const int64_t cw = g.tC == TNH_F64 ? 2 : 4;
+    p.vecC = aligned16(g.C) && g.ldc % cw == 0 && g.sC % cw == 0;
+  }
   static const char *kFamily[3] = {"gemm_f64", "gemm_f32", "gemm_bf16x3"};
   hipEvent_t ev = ctx->prof_begin();
   switch (prec) {

@@ -8,7 +8,9 @@ namespace gk {
 template <typename TA, typename TB, typename TC, bool TRA, bool TRB, int BM, int BN>
 static void launch(hipStream_t s, const Params &p, int batch) {
   dim3 grid(p.tiles_m * p.tiles_n, batch);
-  hipLaunchKernelGGL((gemm_kernel<MODE_BF16X3, TA, TB, TC, TRA, TRB, BM, BN>), grid, dim3(256), 0, s, p);
+  // the hi/lo split writes LDS element by element; the scalar load order keeps those
+  // writes on consecutive banks, the vector order does not
+  hipLaunchKernelGGL((gemm_kernel<MODE_BF16X3, TA, TB, TC, TRA, TRB, BM, BN, false>), grid, dim3(256), 0, s, p);
 }
 
 template <typename TA, typename TB, typename TC, int BM, int BN>

@@ -8,7 +8,10 @@ namespace gk {
 template <typename TA, typename TB, typename TC, bool TRA, bool TRB, int BM, int BN>
 static void launch(hipStream_t s, const Params &p, int batch) {
   dim3 grid(p.tiles_m * p.tiles_n, batch);
-  hipLaunchKernelGGL((gemm_kernel<MODE_F32, TA, TB, TC, TRA, TRB, BM, BN>), grid, dim3(256), 0, s, p);
+  if (p.vecA && p.vecB)
+    hipLaunchKernelGGL((gemm_kernel<MODE_F32, TA, TB, TC, TRA, TRB, BM, BN, true>), grid, dim3(256), 0, s, p);
+  else
+    hipLaunchKernelGGL((gemm_kernel<MODE_F32, TA, TB, TC, TRA, TRB, BM, BN, false>), grid, dim3(256), 0, s, p);
 }
 
 template <typename TA, typename TB, typename TC, int BM, int BN>

@@ -8,7 +8,10 @@ namespace gk {
 template <typename TA, typename TB, typename TC, bool TRA, bool TRB, int BM, int BN>
 static void launch(hipStream_t s, const Params &p, int batch) {
   dim3 grid(p.tiles_m * p.tiles_n, batch);
-  hipLaunchKernelGGL((gemm_kernel<MODE_F64, TA, TB, TC, TRA, TRB, BM, BN>), grid, dim3(256), 0, s, p);
+  if (p.vecA && p.vecB)
+    hipLaunchKernelGGL((gemm_kernel<MODE_F64, TA, TB, TC, TRA, TRB, BM, BN, true>), grid, dim3(256), 0, s, p);
+  else
+    hipLaunchKernelGGL((gemm_kernel<MODE_F64, TA, TB, TC, TRA, TRB, BM, BN, false>), grid, dim3(256), 0, s, p);
 }
 
 template <typename TA, typename TB, typename TC, int BM, int BN>
@@ -27,7 +30,7 @@ static void launch_types(hipStream_t s, const Params &p, int batch, bool tra, bo
 }
 
 void launch_f64(hipStream_t s, const Params &p, int batch, bool tra, bool trb, int tA, int tB, int tC, bool big) {
-  if (big) launch_types<128, 128>(s, p, batch, tra, trb, tA, tB, tC);
+  if (big) launch_types<128, 64>(s, p, batch, tra, trb, tA, tB, tC);
   else launch_types<64, 64>(s, p, batch, tra, trb, tA, tB, tC);
 }
 

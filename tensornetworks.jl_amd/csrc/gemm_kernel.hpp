@@ -3,28 +3,36 @@
 //   C[M,N] = alpha * op(A)[M,K] * op(B)[K,N] + beta * C        (per batch entry)
 //
 // Three arithmetic modes share one LDS-tiled skeleton:
-//   MODE_F64    : v_mfma_f64_16x16x4f64, FP64 in LDS, FP64 accumulate.
-//   MODE_F32    : v_mfma_f32_16x16x4f32, FP32 in LDS, FP32 accumulate.
+//   MODE_F64    : v_mfma_f64_16x16x4f64, FP64 in LDS, FP64 accumulate (BK = 16).
+//   MODE_F32    : v_mfma_f32_16x16x4f32, FP32 in LDS, FP32 accumulate (BK = 32).
 //   MODE_BF16X3 : each FP32 operand is split into hi + lo bf16 when staged to LDS;
 //                 v_mfma_f32_16x16x32_bf16 accumulates hi*hi + hi*lo + lo*hi in FP32.
 //
 // Tiling: 256 threads = 4 waves (2 x 2); a block computes BM x BN, a wave
-// (BM/2) x (BN/2) as a grid of 16x16 MFMA tiles. Operand tiles are staged in LDS
-// with k contiguous per row (row = m for A, n for B), padded so that the 16 lanes
-// of a 16x16 MFMA operand read 16 distinct bank groups. The k order inside an MFMA
-// is permuted (lane group g owns k = 4g..4g+3, or 8g..8g+7 for bf16) so every
-// operand fragment is one contiguous LDS vector read; the sum over k is unchanged.
-// Global tiles are prefetched into registers one k-step ahead and LDS is double
-// buffered, so each k-step needs one barrier.
+// (BM/2) x (BN/2) as a grid of 16x16 MFMA tiles. The k order inside an MFMA is permuted
+// (lane group g = lane/16 owns k = 16c + 4g + kk, kk = 0..3, or k = 8g..8g+7 for bf16);
+// A and B use the same permutation, so the sum over k is unchanged.
+//
+// LDS layouts follow the operand's global layout, so nothing is transposed in flight:
+//   k-contiguous operand  -> [row][k], row stride BK + KPAD: a fragment is one 16-byte
+//                            LDS read (ds_read_b128);
+//   row-contiguous operand -> [k][row], row stride ROWS + 16 elements: the 16 lanes of a
+//                            lane group read 16 consecutive rows, and the +16 padding puts
+//                            the 4 lane groups on disjoint bank ranges (FP32).
+// (bf16 always uses [row][k], splitting into hi/lo planes as it stages.)
+// Global loads are 16-byte vectors when the host has checked alignment and divisibility
+// (Params::vecA / vecB), else bounds-checked scalars. Tiles are prefetched into registers
+// one k-step ahead and LDS is double buffered: one barrier per k-step.
 //
 // Blocks are remapped so that each XCD (blocks are dealt round-robin to the 8 XCDs)
 // owns a contiguous range of tiles, and within it tiles walk M in groups of 8 rows
-// of tiles so consecutive blocks share B columns in that XCD's L2.
+// of tiles so concurrently running blocks share operand panels in that XCD's L2.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace tnh {
 namespace gk {
@@ -32,10 +40,12 @@ namespace gk {
 enum { MODE_F64 = 0, MODE_F32 = 1, MODE_BF16X3 = 2 };
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v2f __attribute__((ext_vector_type(2)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef double v2d __attribute__((ext_vector_type(2)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef unsigned short v8us __attribute__((ext_vector_type(8)));
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
 struct Params {
   int M, N, K;
@@ -47,27 +57,28 @@ struct Params {
   int64_t ldc, sC;
   double alpha, beta;
   int tiles_m, tiles_n;
+  int vecA, vecB, vecC;
 };
 
 template <int MODE>
 struct ModeTraits;
 template <>
 struct ModeTraits<MODE_F64> {
-  using S = double;  // LDS staging type
+  using S = double;
   static constexpr int BK = 16;
-  static constexpr int PAD = 2;
+  static constexpr int KPAD = 2;
 };
 template <>
 struct ModeTraits<MODE_F32> {
   using S = float;
-  static constexpr int BK = 16;
-  static constexpr int PAD = 4;
+  static constexpr int BK = 32;
+  static constexpr int KPAD = 4;
 };
 template <>
 struct ModeTraits<MODE_BF16X3> {
   using S = unsigned short;  // bf16 bits, two planes (hi, lo)
   static constexpr int BK = 32;
-  static constexpr int PAD = 8;
+  static constexpr int KPAD = 8;
 };
 
 // Round-to-nearest-even float -> bf16 bits (finite inputs).
@@ -80,66 +91,160 @@ __device__ __forceinline__ float bf2f(unsigned short h) {
   return __uint_as_float((unsigned int)h << 16);
 }
 
-// Element (r, k) of an operand tile: KC = k contiguous in global memory.
-template <typename T, bool KC>
-__device__ __forceinline__ T ld_elem(const T *__restrict__ base, int64_t ld, int r, int k) {
-  return KC ? base[int64_t(k) + int64_t(r) * ld] : base[int64_t(r) + int64_t(k) * ld];
-}
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  using type = v4f;
+};
+template <>
+struct Vec16<double> {
+  using type = v2d;
+};
 
-// Load the ROWS x BK tile starting at (row0, k0) into registers (zero outside).
-template <typename T, bool KC, int ROWS, int BK>
-__device__ __forceinline__ void load_tile(const T *__restrict__ base, int64_t ld, int row0,
-                                          int nrows, int k0, int K, T (&v)[ROWS * BK / 256]) {
-  constexpr int E = ROWS * BK / 256;
-  const int t = threadIdx.x;
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    int idx = t + 256 * e;
-    int r = KC ? idx / BK : idx % ROWS;
-    int k = KC ? idx % BK : idx / ROWS;
-    int gr = row0 + r, gk = k0 + k;
-    v[e] = (gr < nrows && gk < K) ? ld_elem<T, KC>(base, ld, gr, gk) : T(0);
-  }
-}
+// One operand tile (ROWS x BK) of type T in global memory; KCG = k-contiguous globally.
+template <int MODE, typename T, bool KCG, int ROWS>
+struct Tile {
+  using S = typename ModeTraits<MODE>::S;
+  static constexpr int BK = ModeTraits<MODE>::BK;
+  static constexpr bool LDS_KC = KCG || MODE == MODE_BF16X3;
+  static constexpr int LDK = BK + ModeTraits<MODE>::KPAD;  // [row][k] stride
+  static constexpr int LDR = ROWS + 16;                    // [k][row] stride
+  static constexpr int PLANES = (MODE == MODE_BF16X3) ? 2 : 1;
+  static constexpr int ELEMS = (LDS_KC ? ROWS * LDK : BK * LDR) * PLANES;
+  static constexpr int VW = 16 / sizeof(T);  // elements per 16-byte vector
+  static constexpr int NE = ROWS * BK / 256;  // elements per thread
+  static constexpr int NV = NE / VW;          // vectors per thread
+  static_assert(NE % VW == 0, "tile too small for 16-byte vectors");
 
-template <int MODE, typename T, bool KC, int ROWS>
-__device__ __forceinline__ void store_tile(typename ModeTraits<MODE>::S *__restrict__ lds,
-                                           const T (&v)[ROWS * ModeTraits<MODE>::BK / 256]) {
-  constexpr int BK = ModeTraits<MODE>::BK;
-  constexpr int LD = BK + ModeTraits<MODE>::PAD;
-  constexpr int E = ROWS * BK / 256;
-  const int t = threadIdx.x;
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    int idx = t + 256 * e;
-    int r = KC ? idx / BK : idx % ROWS;
-    int k = KC ? idx % BK : idx / ROWS;
-    if constexpr (MODE == MODE_BF16X3) {
-      float x = float(v[e]);
-      unsigned short hi = f2bf(x);
-      unsigned short lo = f2bf(x - bf2f(hi));
-      lds[r * LD + k] = hi;
-      lds[ROWS * LD + r * LD + k] = lo;  // lo plane follows the hi plane
+  // registers hold the operand already converted to the compute type
+  using RT = typename std::conditional<MODE == MODE_F64, double, float>::type;
+  RT v[NE];
+
+  // position (r, k) of element `i` of this thread in the tile
+  __device__ __forceinline__ static void pos(bool vec, int i, int &r, int &k) {
+    const int t = threadIdx.x;
+    if (vec) {
+      const int vi = t + 256 * (i / VW), w = i % VW;
+      if (KCG) {
+        k = (vi % (BK / VW)) * VW + w;
+        r = vi / (BK / VW);
+      } else {
+        r = (vi % (ROWS / VW)) * VW + w;
+        k = vi / (ROWS / VW);
+      }
     } else {
-      lds[r * LD + k] = typename ModeTraits<MODE>::S(v[e]);
+      const int idx = t + 256 * i;
+      r = KCG ? idx / BK : idx % ROWS;
+      k = KCG ? idx % BK : idx / ROWS;
     }
   }
+
+  __device__ __forceinline__ void load(const T *__restrict__ base, int64_t ld, int row0, int nrows,
+                                       int k0, int K, bool vec) {
+    if (vec) {
+#pragma unroll
+      for (int e = 0; e < NV; ++e) {
+        int r, k;
+        pos(true, e * VW, r, k);
+        const int gr = row0 + r, gk = k0 + k;
+        const bool in = gr < nrows && gk < K;  // whole vector in or out (host-checked)
+        // branch-free: out-of-range vectors read the (always valid) base and are zeroed
+        // by a select, so the load's wait is deferred to the LDS store that consumes it
+        const int64_t off = KCG ? (int64_t(gk) + int64_t(gr) * ld) : (int64_t(gr) + int64_t(gk) * ld);
+        const auto q = *reinterpret_cast<const typename Vec16<T>::type *>(base + (in ? off : 0));
+#pragma unroll
+        for (int w = 0; w < VW; ++w) v[e * VW + w] = in ? RT(q[w]) : RT(0);
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < NE; ++e) {
+        int r, k;
+        pos(false, e, r, k);
+        const int gr = row0 + r, gk = k0 + k;
+        const bool in = gr < nrows && gk < K;
+        const int64_t off = KCG ? (int64_t(gk) + int64_t(gr) * ld) : (int64_t(gr) + int64_t(gk) * ld);
+        const T x = base[in ? off : 0];
+        v[e] = in ? RT(x) : RT(0);
+      }
+    }
+  }
+
+  __device__ __forceinline__ void store(S *__restrict__ lds, bool vec) const {
+    if constexpr (MODE == MODE_BF16X3) {
+#pragma unroll
+      for (int e = 0; e < NE; ++e) {
+        int r, k;
+        pos(vec, e, r, k);
+        const float x = float(v[e]);
+        const unsigned short hi = f2bf(x);
+        const unsigned short lo = f2bf(x - bf2f(hi));
+        lds[r * LDK + k] = hi;
+        lds[ROWS * LDK + r * LDK + k] = lo;
+      }
+    } else if (vec) {
+      // a vector's elements are contiguous in the LDS layout too
+#pragma unroll
+      for (int e = 0; e < NV; ++e) {
+        int r, k;
+        pos(true, e * VW, r, k);
+        S *d = LDS_KC ? &lds[r * LDK + k] : &lds[k * LDR + r];
+        if constexpr (std::is_same<S, float>::value && VW == 4) {
+          *reinterpret_cast<v4f *>(d) = v4f{v[e * VW], v[e * VW + 1], v[e * VW + 2], v[e * VW + 3]};
+        } else if constexpr (std::is_same<S, float>::value) {  // two FP64 inputs
+          *reinterpret_cast<v2f *>(d) = v2f{v[e * VW], v[e * VW + 1]};
+        } else {  // FP64 staging
+          *reinterpret_cast<v2d *>(d) = v2d{v[e * VW], v[e * VW + 1]};
+        }
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < NE; ++e) {
+        int r, k;
+        pos(false, e, r, k);
+        (LDS_KC ? lds[r * LDK + k] : lds[k * LDR + r]) = S(v[e]);
+      }
+    }
+  }
+};
+
+// FP32/FP64 fragment: 4 k-values (k = 16c + 4g + kk) of one row for lane (g, rr).
+template <int MODE, bool LDS_KC, int LDK, int LDR>
+__device__ __forceinline__ void frag4(const typename ModeTraits<MODE>::S *__restrict__ lds, int row,
+                                      int c, int g, typename ModeTraits<MODE>::S (&f)[4]) {
+  using S = typename ModeTraits<MODE>::S;
+  if constexpr (LDS_KC) {
+    const S *q = &lds[row * LDK + 16 * c + 4 * g];
+    if constexpr (std::is_same<S, float>::value) {
+      const v4f x = *reinterpret_cast<const v4f *>(q);
+      f[0] = x[0];
+      f[1] = x[1];
+      f[2] = x[2];
+      f[3] = x[3];
+    } else {
+      const v2d x0 = *reinterpret_cast<const v2d *>(q);
+      const v2d x1 = *reinterpret_cast<const v2d *>(q + 2);
+      f[0] = x0[0];
+      f[1] = x0[1];
+      f[2] = x1[0];
+      f[3] = x1[1];
+    }
+  } else {
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) f[kk] = lds[(16 * c + 4 * g + kk) * LDR + row];
+  }
 }
 
-template <int MODE, typename TA, typename TB, typename TC, bool TRA, bool TRB, int BM, int BN>
+template <int MODE, typename TA, typename TB, typename TC, bool TRA, bool TRB, int BM, int BN, bool VEC>
 __global__ __launch_bounds__(256) void gemm_kernel(Params p) {
   using S = typename ModeTraits<MODE>::S;
   constexpr int BK = ModeTraits<MODE>::BK;
-  constexpr int LD = BK + ModeTraits<MODE>::PAD;
-  constexpr int PLANES = (MODE == MODE_BF16X3) ? 2 : 1;
-  constexpr int A_ELEMS = BM * LD * PLANES;
-  constexpr int B_ELEMS = BN * LD * PLANES;
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
-  constexpr int EA = BM * BK / 256, EB = BN * BK / 256;
   // A is k-contiguous in memory when transposed; B when not transposed.
-  constexpr bool KCA = TRA;
-  constexpr bool KCB = !TRB;
+  using TileA = Tile<MODE, TA, TRA, BM>;
+  using TileB = Tile<MODE, TB, !TRB, BN>;
+  constexpr int A_ELEMS = TileA::ELEMS, B_ELEMS = TileB::ELEMS;
 
   __shared__ S lds[2 * (A_ELEMS + B_ELEMS)];
 
@@ -165,6 +270,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(Params p) {
   const int lane = t & 63, wave = t >> 6;
   const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
   const int g = lane >> 4, rr = lane & 15;
+  constexpr bool va = VEC, vb = VEC;  // 16-byte loads for both operands (host-checked)
 
   using Acc = typename std::conditional<MODE == MODE_F64, v4d, v4f>::type;
   Acc acc[FM][FN];
@@ -173,74 +279,37 @@ __global__ __launch_bounds__(256) void gemm_kernel(Params p) {
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = Acc{0, 0, 0, 0};
 
-  TA ra[EA];
-  TB rb[EB];
+  TileA ta;
+  TileB tb;
   const int nk = (p.K + BK - 1) / BK;
   if (nk > 0) {
-    load_tile<TA, KCA, BM, BK>(A, p.lda, m0, p.M, 0, p.K, ra);
-    load_tile<TB, KCB, BN, BK>(B, p.ldb, n0, p.N, 0, p.K, rb);
+    ta.load(A, p.lda, m0, p.M, 0, p.K, va);
+    tb.load(B, p.ldb, n0, p.N, 0, p.K, vb);
   }
 
   for (int kt = 0; kt < nk; ++kt) {
     S *As = lds + (kt & 1) * (A_ELEMS + B_ELEMS);
     S *Bs = As + A_ELEMS;
-    store_tile<MODE, TA, KCA, BM>(As, ra);
-    store_tile<MODE, TB, KCB, BN>(Bs, rb);
+    ta.store(As, va);
+    tb.store(Bs, vb);
     __syncthreads();
     if (kt + 1 < nk) {
-      load_tile<TA, KCA, BM, BK>(A, p.lda, m0, p.M, (kt + 1) * BK, p.K, ra);
-      load_tile<TB, KCB, BN, BK>(B, p.ldb, n0, p.N, (kt + 1) * BK, p.K, rb);
+      ta.load(A, p.lda, m0, p.M, (kt + 1) * BK, p.K, va);
+      tb.load(B, p.ldb, n0, p.N, (kt + 1) * BK, p.K, vb);
     }
 
-    if constexpr (MODE == MODE_F32) {
-      v4f af[FM], bf[FN];
-#pragma unroll
-      for (int i = 0; i < FM; ++i)
-        af[i] = *reinterpret_cast<const v4f *>(&As[(wm0 + 16 * i + rr) * LD + 4 * g]);
-#pragma unroll
-      for (int j = 0; j < FN; ++j)
-        bf[j] = *reinterpret_cast<const v4f *>(&Bs[(wn0 + 16 * j + rr) * LD + 4 * g]);
-#pragma unroll
-      for (int kk = 0; kk < 4; ++kk)
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-          for (int j = 0; j < FN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i][kk], bf[j][kk], acc[i][j],
-                                                              0, 0, 0);
-    } else if constexpr (MODE == MODE_F64) {
-      v2d af[FM][2], bf[FN][2];
-#pragma unroll
-      for (int i = 0; i < FM; ++i) {
-        const S *q = &As[(wm0 + 16 * i + rr) * LD + 4 * g];
-        af[i][0] = *reinterpret_cast<const v2d *>(q);
-        af[i][1] = *reinterpret_cast<const v2d *>(q + 2);
-      }
-#pragma unroll
-      for (int j = 0; j < FN; ++j) {
-        const S *q = &Bs[(wn0 + 16 * j + rr) * LD + 4 * g];
-        bf[j][0] = *reinterpret_cast<const v2d *>(q);
-        bf[j][1] = *reinterpret_cast<const v2d *>(q + 2);
-      }
-#pragma unroll
-      for (int kk = 0; kk < 4; ++kk)
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-          for (int j = 0; j < FN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                af[i][kk >> 1][kk & 1], bf[j][kk >> 1][kk & 1], acc[i][j], 0, 0, 0);
-    } else {  // MODE_BF16X3
+    if constexpr (MODE == MODE_BF16X3) {
+      constexpr int LD = TileA::LDK;
       v8us ah[FM], al[FM], bh[FN], bl[FN];
 #pragma unroll
       for (int i = 0; i < FM; ++i) {
-        int o = (wm0 + 16 * i + rr) * LD + 8 * g;
+        const int o = (wm0 + 16 * i + rr) * LD + 8 * g;
         ah[i] = *reinterpret_cast<const v8us *>(&As[o]);
         al[i] = *reinterpret_cast<const v8us *>(&As[BM * LD + o]);
       }
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
-        int o = (wn0 + 16 * j + rr) * LD + 8 * g;
+        const int o = (wn0 + 16 * j + rr) * LD + 8 * g;
         bh[j] = *reinterpret_cast<const v8us *>(&Bs[o]);
         bl[j] = *reinterpret_cast<const v8us *>(&Bs[BN * LD + o]);
       }
@@ -249,15 +318,34 @@ __global__ __launch_bounds__(256) void gemm_kernel(Params p) {
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-              __builtin_bit_cast(v8bf, al[i]), __builtin_bit_cast(v8bf, bh[j]), acc[i][j], 0,
-              0, 0);
+              __builtin_bit_cast(v8bf, al[i]), __builtin_bit_cast(v8bf, bh[j]), acc[i][j], 0, 0, 0);
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-              __builtin_bit_cast(v8bf, ah[i]), __builtin_bit_cast(v8bf, bl[j]), acc[i][j], 0,
-              0, 0);
+              __builtin_bit_cast(v8bf, ah[i]), __builtin_bit_cast(v8bf, bl[j]), acc[i][j], 0, 0, 0);
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-              __builtin_bit_cast(v8bf, ah[i]), __builtin_bit_cast(v8bf, bh[j]), acc[i][j], 0,
-              0, 0);
+              __builtin_bit_cast(v8bf, ah[i]), __builtin_bit_cast(v8bf, bh[j]), acc[i][j], 0, 0, 0);
         }
+    } else {
+#pragma unroll
+      for (int c = 0; c < BK / 16; ++c) {
+        S af[FM][4], bf[FN][4];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+          frag4<MODE, TileA::LDS_KC, TileA::LDK, TileA::LDR>(As, wm0 + 16 * i + rr, c, g, af[i]);
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          frag4<MODE, TileB::LDS_KC, TileB::LDK, TileB::LDR>(Bs, wn0 + 16 * j + rr, c, g, bf[j]);
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+              if constexpr (MODE == MODE_F32)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i][kk], bf[j][kk], acc[i][j], 0, 0, 0);
+              else
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i][kk], bf[j][kk], acc[i][j], 0, 0, 0);
+            }
+      }
     }
   }
 
@@ -265,20 +353,41 @@ __global__ __launch_bounds__(256) void gemm_kernel(Params p) {
   // 16x16 tile with row = 4g + r for the FP32-output MFMAs and row = g + 4r for
   // v_mfma_f64_16x16x4f64 (measured on gfx950).
   const bool use_beta = p.beta != 0.0;
+  const bool vc = p.vecC != 0;
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
       const int col = n0 + wn0 + 16 * j + rr;
       if (col >= p.N) continue;
+      TC *ccol = C + int64_t(col) * p.ldc;
+      if constexpr (MODE != MODE_F64) {
+        const int row0 = m0 + wm0 + 16 * i + 4 * g;
+        if (vc && row0 + 3 < p.M) {
+          // four consecutive rows of one column: one 16- or 32-byte store
+          TC out[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            double vv = p.alpha * double(acc[i][j][r]);
+            if (use_beta) vv += p.beta * double(ccol[row0 + r]);
+            out[r] = TC(vv);
+          }
+          if constexpr (std::is_same<TC, float>::value) {
+            *reinterpret_cast<v4f *>(ccol + row0) = v4f{out[0], out[1], out[2], out[3]};
+          } else {
+            *reinterpret_cast<v2d *>(ccol + row0) = v2d{out[0], out[1]};
+            *reinterpret_cast<v2d *>(ccol + row0 + 2) = v2d{out[2], out[3]};
+          }
+          continue;
+        }
+      }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm0 + 16 * i + (MODE == MODE_F64 ? (g + 4 * r) : (4 * g + r));
         if (row < p.M) {
-          TC *c = C + int64_t(row) + int64_t(col) * p.ldc;
-          double v = p.alpha * double(acc[i][j][r]);
-          if (use_beta) v += p.beta * double(*c);
-          *c = TC(v);
+          double vv = p.alpha * double(acc[i][j][r]);
+          if (use_beta) vv += p.beta * double(ccol[row]);
+          ccol[row] = TC(vv);
         }
       }
     }

@@ -1,0 +1,103 @@
+// Local expectation values <psi| O_term |psi> of an operator list, the computation of
+// inner(st, psi, oplist, phi), src/structures/mps/mps.jl:87-134 (psi = phi): norm
+// environments from both ends (ProjMPS(psi, psi; rank=1) blocks) and, per term, the
+// left block carried across the term's sites with the operators inserted. Every
+// contraction is the environment-update GEMM chain with a 1 x d x d x 1 "MPO" holding the
+// local operator (identity between the operators of a term).
+#include <algorithm>
+#include <complex>
+#include <vector>
+
+#include "tnh_internal.hpp"
+
+using namespace tnh;
+
+namespace {
+
+// a (1, d, d, 1) device tensor holding the d x d operator O[s_out, s_in]
+tnh_buf *op_tensor(tnh_ctx *ctx, int d, const double *interleaved) {
+  bool cplx = false;
+  for (int i = 0; i < d * d; ++i) cplx |= interleaved[2 * i + 1] != 0.0;
+  int64_t dims[4] = {1, d, d, 1};
+  tnh_buf *b = buf_new(ctx, cplx ? TNH_C128 : TNH_F64, 4, dims);
+  std::vector<double> planar(size_t(d) * d * (cplx ? 2 : 1));
+  for (int i = 0; i < d * d; ++i) {
+    planar[i] = interleaved[2 * i];
+    if (cplx) planar[size_t(d) * d + i] = interleaved[2 * i + 1];
+  }
+  TNH_HIP(hipMemcpyAsync(b->ptr, planar.data(), planar.size() * sizeof(double),
+                         hipMemcpyHostToDevice, ctx->stream));
+  TNH_HIP(hipStreamSynchronize(ctx->stream));
+  return b;
+}
+
+}  // namespace
+
+extern "C" int tnh_inner_oplist(tnh_ctx *ctx, int N, const tnh_buf *const *psi, int nterms,
+                                const int *first_site, const int *range, const double *ops,
+                                double *out) {
+  if (!ctx) return TNH_ERR_ARG;
+  std::vector<tnh_buf *> owned;
+  int rc = tnh_guard(ctx, [&] {
+    TNH_REQUIRE(N >= 1 && psi && (nterms == 0 || (first_site && range && ops && out)),
+                TNH_ERR_ARG, "inner: bad arguments");
+    for (int i = 0; i < N; ++i)
+      TNH_REQUIRE(psi[i] && psi[i]->ndim == 3, TNH_ERR_SHAPE, "site tensors must be (chi_l, d, chi_r)");
+    const int d = int(psi[0]->dims[1]);
+    std::vector<double> eye(2 * size_t(d) * d, 0.0);
+    for (int i = 0; i < d; ++i) eye[2 * (i + size_t(i) * d)] = 1.0;
+    tnh_buf *Wid = op_tensor(ctx, d, eye.data());
+    owned.push_back(Wid);
+    // norm environments: E[i] = left block through sites 1..i, F[i] = right block from i..N
+    std::vector<tnh_buf *> E(N + 2, nullptr), F(N + 2, nullptr);
+    for (int i = 1; i <= N; ++i) {
+      env_left(ctx, E[i - 1], psi[i - 1], Wid, &E[i], TNH_PREC_F64);
+      owned.push_back(E[i]);
+    }
+    for (int i = N; i >= 1; --i) {
+      env_right(ctx, F[i + 1], psi[i - 1], Wid, &F[i], TNH_PREC_F64);
+      owned.push_back(F[i]);
+    }
+    const double *op = ops;
+    for (int t = 0; t < nterms; ++t) {
+      const int s = first_site[t], r = range[t];
+      TNH_REQUIRE(s >= 1 && r >= 1 && s + r - 1 <= N, TNH_ERR_ARG,
+                  "inner: term " + std::to_string(t) + " out of range");
+      tnh_buf *X = nullptr;
+      const tnh_buf *prev = E[s - 1];
+      for (int k = 0; k < r; ++k) {
+        tnh_buf *W = op_tensor(ctx, d, op + 2 * size_t(d) * d * k);
+        tnh_buf *Y = nullptr;
+        try {
+          env_left(ctx, prev, psi[s - 1 + k], W, &Y, TNH_PREC_F64);
+        } catch (...) {
+          buf_delete(ctx, W);
+          buf_delete(ctx, X);
+          throw;
+        }
+        buf_delete(ctx, W);
+        buf_delete(ctx, X);
+        X = Y;
+        prev = X;
+      }
+      op += 2 * size_t(d) * d * r;
+      const tnh_buf *R = F[s + r];
+      double re = 0, im = 0;
+      if (R) {
+        cdot_host(ctx, X, R, &re, &im);
+      } else {
+        // beyond the right end: the block is the 1x1x1 ones block
+        TNH_REQUIRE(X->numel == 1, TNH_ERR_SHAPE, "inner: open right bond");
+        double h[2] = {0, 0};
+        to_host(ctx, h, X->ptr, is_complex(X->dtype) ? 16 : 8);
+        re = h[0];
+        im = is_complex(X->dtype) ? h[1] : 0.0;
+      }
+      buf_delete(ctx, X);
+      out[2 * t] = re;
+      out[2 * t + 1] = im;
+    }
+  });
+  for (auto *b : owned) buf_delete(ctx, b);
+  return rc;
+}

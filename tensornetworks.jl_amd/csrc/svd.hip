@@ -734,6 +734,39 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
   hipLaunchKernelGGL(gather_cols_kernel, dim3(unsigned((int64_t(p) * k + 255) / 256)), dim3(256), 0,
                      ctx->stream, V, p, pp, dperm, nullptr, k, basis, int64_t(p) * k, cplx);
   TNH_HIP(hipGetLastError());
+  // One Newton-Schulz step toward the nearest orthonormal basis, U <- U (3I - U^H U) / 2:
+  // the accumulated Jacobi rotations leave U^H U - I ~ 1e-13, this squares it away.
+  {
+    const int64_t pk = int64_t(p) * k, kk = int64_t(k) * k;
+    double *X = static_cast<double *>(ctx->workspace(621, sizeof(double) * kk * planes));
+    double *W = static_cast<double *>(ctx->workspace(622, sizeof(double) * pk * planes));
+    COperand Ub, Uh;
+    Ub.re = basis;
+    Ub.im = cplx ? basis + pk : nullptr;
+    Uh = Ub;
+    Uh.conj = true;
+    Gemm g1;  // X = U^H U
+    g1.M = k;
+    g1.N = k;
+    g1.K = p;
+    g1.transA = true;
+    g1.lda = p;
+    g1.ldb = p;
+    g1.ldc = k;
+    cgemm(ctx, g1, Uh, Ub, X, cplx ? X + kk : nullptr, TNH_F64, TNH_PREC_F64);
+    Gemm g2;  // W = U X
+    g2.M = p;
+    g2.N = k;
+    g2.K = k;
+    g2.lda = p;
+    g2.ldb = k;
+    g2.ldc = p;
+    COperand Xo;
+    Xo.re = X;
+    Xo.im = cplx ? X + kk : nullptr;
+    cgemm(ctx, g2, Ub, Xo, W, cplx ? W + pk : nullptr, TNH_F64, TNH_PREC_F64);
+    axpby(ctx, pk * planes, -0.5, W, 1.5, basis);  // U = 1.5 U - 0.5 W
+  }
   TNH_HIP(hipStreamSynchronize(ctx->stream));  // perm lives on the host stack
   return basis;
 }

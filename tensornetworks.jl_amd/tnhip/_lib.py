@@ -70,6 +70,9 @@ SIGNATURES = {
     "tnh_buf_device_ptr": (C.c_void_p, [buf_p]),
     "tnh_buf_convert": (C.c_int, [ctx_p, buf_p, C.c_int, C.POINTER(buf_p)]),
     "tnh_gemm": (C.c_int, [ctx_p, C.c_int, C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_double, buf_p, C.c_int64, buf_p, C.c_int64, C.c_double, buf_p, C.c_int64, C.c_int]),
+    "tnh_buf_norm": (C.c_int, [ctx_p, buf_p, C.POINTER(C.c_double)]),
+    "tnh_buf_scale": (C.c_int, [ctx_p, buf_p, C.c_double, C.c_double]),
+    "tnh_apply_site_op": (C.c_int, [ctx_p, C.POINTER(buf_p), buf_p]),
     "tnh_heff_apply": (C.c_int, [ctx_p, buf_p, buf_p, buf_p, buf_p, buf_p, buf_p, C.c_double, C.c_double, C.c_int]),
     "tnh_env_left": (C.c_int, [ctx_p, buf_p, buf_p, buf_p, C.POINTER(buf_p), C.c_int]),
     "tnh_env_right": (C.c_int, [ctx_p, buf_p, buf_p, buf_p, C.POINTER(buf_p), C.c_int]),
@@ -80,6 +83,7 @@ SIGNATURES = {
     "tnh_two_site_theta": (C.c_int, [ctx_p, buf_p, buf_p, C.POINTER(buf_p)]),
     "tnh_mps_movecenter": (C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]),
     "tnh_applygate2": (C.c_int, [ctx_p, C.POINTER(buf_p), C.POINTER(buf_p), buf_p, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "tnh_inner_oplist": (C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "tnh_dmrg_create": (C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.c_int, C.POINTER(buf_p), C.POINTER(C.c_double), C.POINTER(C.c_void_p)]),
     "tnh_dmrg_destroy": (C.c_int, [C.c_void_p]),
     "tnh_dmrg_sweep": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(DmrgParams), C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_double)]),
