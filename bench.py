@@ -134,6 +134,8 @@ def main(argv=None):
     ap.add_argument("--no-svd", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--svd-only", action="store_true",
+                    help="run only the truncated SVD of the 2chi x 2chi theta matrix (1 warm-up + 1 timed)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB FP64 device copy (reformat kernel), the "
                          "known-size reference for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -163,6 +165,19 @@ def main(argv=None):
     chi, D, d = a.chi, 5, 2
     ctx = tnhip.Context.default()
     Lt, Rt, tht = make_inputs(chi, D, a.seed, torch)
+    if a.svd_only:
+        M = Tensor.wrap(tht.data_ptr(), (2 * chi, 2 * chi), np.float64, owner=tht, ctx=ctx)
+        times = []
+        for _ in range(2):
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            U, S, Vt, disc, fb = ops.svd_trunc(M, 0.0, chi, 1, want=(True, True, True))
+            ctx.synchronize()
+            times.append(time.perf_counter() - t0)
+            del U, S, Vt
+        print(json.dumps({"svd_2chi": {"n": 2 * chi, "kept": chi, "seconds_warm": times[1],
+                                       "seconds_first": times[0], "discarded": disc, "fallback": fb}}))
+        return 0
     Lt32, Rt32 = Lt.float(), Rt.float()
     torch.cuda.synchronize()
     W = heisenberg_bulk_w()

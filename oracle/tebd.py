@@ -12,7 +12,8 @@ from .exact import OPS
 
 def local_tensor(terms, st_ops, site, N, d=2):
     """sitetensor(oplist, st, site), src/structures/mps/oplist.jl:164-183, for terms of range
-    <= 2: G[o1,i1,o2,i2] (or G[o,i] for a one-site block at the last site)."""
+    <= 2: G[o1,i1,o2,i2] (or G[o,i] for a one-site block at the last site). An operator is
+    a name in st_ops or a d x d matrix."""
     rng = max([1] + [s[-1] - s[0] + 1 for _, s, _ in terms])
     r = min(rng, N - site + 1)
     G = None
@@ -22,7 +23,11 @@ def local_tensor(terms, st_ops, site, N, d=2):
         mats = []
         for k in range(r):
             s = site + k
-            mats.append(st_ops[ops[sites.index(s)]] if s in sites else st_ops["id"])
+            if s in sites:
+                o = ops[sites.index(s)]
+                mats.append(st_ops[o] if isinstance(o, str) else np.asarray(o))
+            else:
+                mats.append(st_ops["id"])
         T = mats[0]
         for m in mats[1:]:
             T = np.multiply.outer(T, m)

@@ -1,0 +1,148 @@
+"""Quantum-jump Monte Carlo on the GPU: qjmc_gates, qjmc_simulation, qjmc_emission_rates
+and the QJMC observers of src/algorithms/mps/qjmc.jl.
+
+Per trajectory the MPS stays on the device: the Trotter gates of exp(dt (-iH - sum L^H L / 2))
+run through tnh_applygate2 / tnh_apply_site_op, the emission rates c^2 <psi|L^H L|psi> come
+from tnh_inner_oplist, and a jump is applyop! followed by the two centre sweeps of
+qjmc.jl:103-107. Random numbers come from a caller-supplied source (`rand`, a callable
+returning one uniform in [0, 1)), consumed in the order of the reference's rand() calls so
+that a trajectory is reproducible from its uniforms.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .models import GMPS, OpList, Sitetypes
+from .tebd import DeviceMPS, applygates, entropy, inner, trotterize
+
+
+def qjmc_gates(st: Sitetypes, H: OpList, jumpops: OpList, dt, order=2, evol="imag"):
+    """qjmc_gates(st, H, jumpops, dt; kwargs), qjmc.jl:9-26: the escape operators
+    c^2 dag(O) O and the gates of Heff = -i H - escape / 2. Returns (Heff, gates)."""
+    escapeops = OpList(jumpops.length)
+    for ops, sites, c in zip(jumpops.ops, jumpops.sites, jumpops.coeffs):
+        names = [st.opprod([st.dag(o), o]) for o in ops]
+        escapeops.add(names, sites, c ** 2)
+    Heff = (-1j) * H + (-0.5) * escapeops
+    gates = trotterize(st, Heff, dt, order=order, evol=evol)
+    return Heff, gates
+
+
+def _escape_oplist(st: Sitetypes, jumpops: OpList) -> OpList:
+    ol = OpList(jumpops.length)
+    for ops, sites, c in zip(jumpops.ops, jumpops.sites, jumpops.coeffs):
+        ol.add([st.opprod([st.dag(o), o]) for o in ops], sites, c ** 2)
+    return ol
+
+
+def qjmc_emission_rates(st: Sitetypes, psi: DeviceMPS, jumpops: OpList) -> np.ndarray:
+    """qjmc_emission_rates(st, psi, jumpops), qjmc.jl:170-220: |c^2 <psi|dag(O) O|psi>| for
+    every jump term (psi normalised), computed on the GPU."""
+    return np.abs(inner(st, psi, _escape_oplist(st, jumpops)))
+
+
+def _applyop(st: Sitetypes, psi: DeviceMPS, ops, sites):
+    """applyop!(st, psi, ops, sites), mps.jl:141-151 (coefficient 1)."""
+    for o, s in zip(ops, sites):
+        psi.apply_site_op(s, st.op(o))
+
+
+def qjmc_simulation(st: Sitetypes, psi, H: OpList, jumpops: OpList, tmax, dt, observers=(), save=0.0,
+                    cutoff=1e-12, mindim=1, maxdim=0, classical=True, verbose=True, rand=None,
+                    order=2, evol="imag"):
+    """qjmc_simulation(st, psi, H, jumpops, tmax, dt, observers; kwargs), qjmc.jl:28-167.
+    psi: a GMPS (updated in place at the end) or a DeviceMPS. Returns (jumps, jumptimes)
+    with 1-based jump indices, as the reference stores them."""
+    if rand is None:
+        gen = np.random.default_rng()
+        rand = gen.random
+    host = psi if isinstance(psi, GMPS) else None
+    dpsi = DeviceMPS.from_gmps(psi) if host is not None else psi
+    save = dt if save == 0 else save
+    steps = int(math.ceil(round(tmax / dt, 5)))
+    savesteps = int(math.ceil(round(save / dt)))
+    jumps, jumptimes = [], []
+    _, gates = qjmc_gates(st, H, jumpops, dt, order=order, evol=evol)
+    N = len(dpsi)
+    time = 0.0
+    for ob in observers:
+        ob.measure(time, dpsi, jumps, jumptimes)
+
+    def jump_to(rates, er):
+        r = rand()
+        cum = np.cumsum(rates) / er
+        hit = np.nonzero(r < cum)[0]
+        if hit.size == 0:
+            raise ValueError("no jump channel selected")
+        idx = int(hit[0])
+        dpsi.movecenter(1)
+        _applyop(st, dpsi, jumpops.ops[idx], jumpops.sites[idx])
+        dpsi.movecenter(N)
+        dpsi.movecenter(1, cutoff, maxdim, mindim)
+        dpsi.normalize()
+        jumps.append(idx + 1)
+        jumptimes.append(time + dt)
+
+    for i in range(1, steps + 1):
+        applygates(dpsi, gates, cutoff=cutoff, maxdim=maxdim, mindim=mindim)
+        r = rand()
+        if not classical:
+            prob = dpsi.norm() ** 2
+            dpsi.normalize()
+            if r > prob:
+                rates = qjmc_emission_rates(st, dpsi, jumpops)
+                jump_to(rates, rates.sum())
+        else:
+            dpsi.normalize()
+            rates = qjmc_emission_rates(st, dpsi, jumpops)
+            er = rates.sum()
+            prob = math.exp(-er * dt)
+            if rand() > prob:
+                jump_to(rates, er)
+        time += dt
+        if i % savesteps == 0:
+            if verbose:
+                print(f"time={time:.5f}, jumps={len(jumps)}, maxbonddim={dpsi.maxbonddim()}, "
+                      f"activity={len(jumps) / time:.5f} ")
+            for ob in observers:
+                ob.measure(time, dpsi, jumps, jumptimes)
+    if host is not None:
+        g = dpsi.to_gmps()
+        host.tensors, host.center = g.tensors, g.center
+    return jumps, jumptimes
+
+
+class QJMCOperators:
+    """QJMCOperators(oplist, st), qjmc.jl:238-260: inner(st, psi, oplist, psi) at every
+    save point."""
+
+    def __init__(self, oplist: OpList, st: Sitetypes):
+        self.oplist, self.st = oplist, st
+        self.times, self.measurements = [], []
+
+    def measure(self, time, psi, jumps, jumptimes):
+        self.times.append(time)
+        self.measurements.append(inner(self.st, psi, self.oplist))
+
+
+class QJMCActivity:
+    """QJMCActivity(), qjmc.jl:264-285: time and number of jumps so far."""
+
+    def __init__(self):
+        self.time, self.jumps = 0.0, 0
+
+    def measure(self, time, psi, jumps, jumptimes):
+        self.time, self.jumps = time, len(jumps)
+
+
+class QJMCEntropy:
+    """QJMCEntropy(), qjmc.jl:293-310: the entanglement entropy of every bond."""
+
+    def __init__(self):
+        self.times, self.measurements = [], []
+
+    def measure(self, time, psi, jumps, jumptimes):
+        self.times.append(time)
+        self.measurements.append([entropy(psi, i) for i in range(1, len(psi))])

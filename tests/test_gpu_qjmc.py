@@ -1,0 +1,82 @@
+"""GPU parity of quantum-jump Monte Carlo (qjmc_simulation, qjmc.jl:28-167) against the
+dense oracle trajectory on the same supplied uniforms. Without truncation (cutoff 0) the MPS
+trajectory is the dense one up to the SVD's accuracy, so the jump records must agree
+exactly and <sz_i> at every save point to 1e-8."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def model(N, g, gamma):
+    from tnhip import OpList
+
+    H = OpList(N)
+    for i in range(1, N):
+        H.add(["z", "z"], [i, i + 1], -1.0)
+    for i in range(1, N + 1):
+        H.add("x", i, -g)
+    J = OpList(N)
+    for i in range(1, N + 1):
+        J.add("s-", i, gamma ** 0.5)
+    return H, J
+
+
+def terms(ol):
+    return [(list(o), list(s), c) for o, s, c in zip(ol.ops, ol.sites, ol.coeffs)]
+
+
+def zlist(N):
+    from tnhip import OpList
+
+    zs = OpList(N)
+    for i in range(1, N + 1):
+        zs.add("z", i)
+    return zs
+
+
+@pytest.mark.parametrize("classical", [True, False])
+def test_qjmc_trajectories_match_dense(ctx, classical):
+    from tnhip import productMPS, spinhalf
+    from tnhip.qjmc import QJMCOperators, QJMCActivity, qjmc_simulation
+    from oracle.qjmc import qjmc_dense
+
+    N, g, gamma, dt, tmax = 6, 0.5, 0.5, 0.05, 1.0
+    H, J = model(N, g, gamma)
+    U = np.random.default_rng(11).random((3, 200))
+    v0 = np.zeros(2 ** N, complex)
+    v0[0] = 1.0
+    jr, tr, times, meas = qjmc_dense(v0, terms(H), terms(J), N, dt, tmax, U, save=0.25,
+                                     obs=[("z", i) for i in range(1, N + 1)], classical=classical)
+    assert sum(len(j) for j in jr) >= 2  # the case exercises jumps
+    for m in range(U.shape[0]):
+        st = spinhalf()
+        psi = productMPS(st, ["up"] * N)
+        it = iter(U[m])
+        obs, act = QJMCOperators(zlist(N), st), QJMCActivity()
+        jumps, jtimes = qjmc_simulation(st, psi, H, J, tmax, dt, [obs, act], save=0.25, cutoff=0.0,
+                                        classical=classical, verbose=False, rand=lambda: float(next(it)))
+        assert jumps == jr[m], (m, jumps, jr[m])
+        assert np.allclose(jtimes, tr[m], atol=1e-12)
+        assert act.jumps == len(jumps)
+        got = np.array(obs.measurements).real
+        assert got.shape == meas[m].shape
+        assert np.allclose(got, meas[m].real, atol=1e-8)
+
+
+def test_emission_rates_match_dense(ctx, rng):
+    from tnhip import spinhalf
+    from tnhip.qjmc import qjmc_emission_rates
+    from tnhip.tebd import DeviceMPS
+    from oracle.mps import random_mps, to_dense, normalize
+    from oracle.qjmc import emission_rates_dense
+
+    N = 5
+    _, J = model(N, 0.5, 0.3)
+    J.add(["s-", "s-"], [2, 3], 0.7)
+    psi = random_mps(N, 2, 4, rng)
+    normalize(psi)
+    v = to_dense(psi)
+    ref = emission_rates_dense(v[:, None], terms(J), N)[0]
+    got = qjmc_emission_rates(spinhalf(), DeviceMPS(psi.tensors, psi.center), J)
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-14)
