@@ -215,6 +215,18 @@ int tnh_applygate2(tnh_ctx *ctx, tnh_buf **A1, tnh_buf **A2, const tnh_buf *G, i
 int tnh_inner_oplist(tnh_ctx *ctx, int N, const tnh_buf *const *psi, int nterms,
                      const int *first_site, const int *range, const double *ops, double *out);
 
+/* <psi|H|psi> for an MPO H = W[0..N-1] (inner(psi, H, psi), src/structures/mps/mpo.jl:181-217,
+ * no normalisation) into (energy_re, energy_im), and, when dW is not NULL, its gradient
+ * with respect to every MPO tensor: dW[i] (shape (D_l, d, d, D_r), allocated or reused like
+ * tnh_env_left's output) holds dE/dW_i[w, s, t, w'] = sum L[a,w,a'] conj(A[a,s,b]) A[a',t,b']
+ * R[b,w',b'], so that E = sum W_i . dW_i for every i and couplings entering W linearly get
+ * their gradient by the chain rule. The reference has no differentiation; this is the
+ * backward pass of the environment sweep of buildleft!/buildright! (projmps.jl:50-95).
+ * The forward sweep keeps left environments only every `block` sites and the backward
+ * sweep recomputes them one block at a time. FP64. */
+int tnh_mpo_expect_grad(tnh_ctx *ctx, int N, tnh_buf *const *psi, tnh_buf *const *W, int block,
+                        double *energy_re, double *energy_im, tnh_buf **dW);
+
 /* ---------------------------------------------------- device-resident DMRG */
 
 typedef struct tnh_dmrg_params {

@@ -84,7 +84,8 @@ SIGNATURES = {
     "tnh_mps_movecenter": (C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]),
     "tnh_applygate2": (C.c_int, [ctx_p, C.POINTER(buf_p), C.POINTER(buf_p), buf_p, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "tnh_inner_oplist": (C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
-    "tnh_dmrg_create": (C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.c_int, C.POINTER(buf_p), C.POINTER(C.c_double), C.POINTER(C.c_void_p)]),
+    "tnh_mpo_expect_grad": (C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.POINTER(buf_p), C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(buf_p)]),
+    "tnh_dmrg_create":(C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.c_int, C.POINTER(buf_p), C.POINTER(C.c_double), C.POINTER(C.c_void_p)]),
     "tnh_dmrg_destroy": (C.c_int, [C.c_void_p]),
     "tnh_dmrg_sweep": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(DmrgParams), C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "tnh_dmrg_energy": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),

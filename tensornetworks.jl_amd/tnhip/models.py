@@ -169,7 +169,7 @@ def _realify(arrs):
     return [np.asfortranarray(a) for a in arrs]
 
 
-def mpo_tensors(st: Sitetypes, H: OpList):
+def mpo_tensors(st: Sitetypes, H: OpList, slots: list | None = None):
     """Finite-state-automaton MPO of an operator list, after MPO(st, H),
     src/structures/mps/mpo.jl:323-440: W[w, s_out, s_in, w'] with channel 0 = "nothing
     placed yet", the last channel = "term complete", one fresh channel per bond a
@@ -177,7 +177,9 @@ def mpo_tensors(st: Sitetypes, H: OpList):
     terms in the [0, last] corner. The reference then compresses by SVD with cutoff
     1e-15 (:442-457); that changes only the gauge of W for the Hamiltonians used here,
     so it is not applied (energies and the operator are identical: see the tests).
-    Returns real tensors when every coefficient and operator is real."""
+    Returns real tensors when every coefficient and operator is real. If `slots` is a list,
+    it receives, per term, (site, row, col, op) such that the term's coefficient c enters
+    only as W_site[row, :, :, col] += c * op (1-based site): dW/dc for the chain rule."""
     N, d = H.length, st.dim
     nchan = [0] * (N + 1)  # channels crossing bond b (between sites b and b+1), 1-based b
     plan = []
@@ -207,6 +209,8 @@ def mpo_tensors(st: Sitetypes, H: OpList):
             i = sites[0]
             W = Ws[i - 1]
             W[0, :, :, W.shape[3] - 1] += coeff * st.op(ops[0])
+            if slots is not None:
+                slots.append((i, 0, W.shape[3] - 1, st.op(ops[0])))
             continue
         for i in range(sites[0], sites[-1] + 1):
             W = Ws[i - 1]
@@ -214,6 +218,8 @@ def mpo_tensors(st: Sitetypes, H: OpList):
             row = 0 if i == sites[0] else chans[i - 1]
             col = W.shape[3] - 1 if i == sites[-1] else chans[i]
             W[row, :, :, col] += (coeff if i == sites[0] else 1.0) * opm
+            if slots is not None and i == sites[0]:
+                slots.append((i, row, col, opm))
     return _realify(Ws)
 
 

@@ -82,3 +82,33 @@ def test_sitetensor_two_site_gate_layout():
     ref = -np.einsum("ab,cd->abcd", Z, Z) - 0.5 * np.einsum("ab,cd->abcd", X, I)
     assert np.allclose(G, ref)
     assert H.sitetensor(st, 3) is None
+
+
+def test_mpo_coefficient_slots_are_the_derivative():
+    """W(c) - W(c with c_t -> c_t + 1) is nonzero only at the recorded slot, equal to op:
+    the slots are dW/dc_t, which the coupling gradient contracts with dE/dW."""
+    from tnhip.models import spinhalf, OpList, mpo_tensors
+
+    st = spinhalf()
+    N = 5
+    H = OpList(N)
+    for i in range(1, N):
+        H.add(["z", "z"], [i, i + 1], -1.0 - 0.1 * i)
+    for i in range(1, N + 1):
+        H.add("x", i, -0.3 * i)
+    H.add(["y", "x"], [2, 4], 0.4)
+    slots = []
+    W0 = mpo_tensors(st, H, slots)
+    assert len(slots) == len(H.coeffs)
+    for t, (site, row, col, op) in enumerate(slots):
+        Hp = H * 1.0
+        Hp.coeffs[t] += 1.0
+        W1 = mpo_tensors(st, Hp)
+        for i in range(N):
+            diff = np.asarray(W1[i], dtype=complex) - np.asarray(W0[i], dtype=complex)
+            if i == site - 1:
+                expect = np.zeros_like(diff)
+                expect[row, :, :, col] = op
+                assert np.allclose(diff, expect), t
+            else:
+                assert np.abs(diff).max() == 0.0, (t, i)
