@@ -105,6 +105,115 @@ def cpu_baseline(budget_s=12.0):
     return best
 
 
+def heisenberg_oplist(N):
+    """H = sum S.S with Pauli spin operators: (s+ s- + s- s+) / 2 + z z / 4 (real MPO, D = 5)."""
+    from tnhip.models import OpList
+
+    H = OpList(N)
+    for i in range(1, N):
+        H.add(["s+", "s-"], [i, i + 1], 0.5)
+        H.add(["s-", "s+"], [i, i + 1], 0.5)
+        H.add(["z", "z"], [i, i + 1], 0.25)
+    return H
+
+
+def tfim_oplist(N, g=1.0):
+    """H = -sum z z - g sum x (MPO D = 3)."""
+    from tnhip.models import OpList
+
+    H = OpList(N)
+    for i in range(1, N):
+        H.add(["z", "z"], [i, i + 1], -1.0)
+    for i in range(1, N + 1):
+        H.add("x", i, -g)
+    return H
+
+
+def dmrg_heisenberg_n32(cpu=True):
+    """Two-site DMRG, Heisenberg N = 32, random chi = 8 start (seed 0), chi_max 64, cutoff
+    1e-10, 8 passes, Krylov dimension 4, FP64 on the GPU; the FP64 CPU port on the same
+    initial tensors beside it."""
+    from tnhip import MPO, randomMPS, spinhalf
+    from tnhip.mps import dmrg
+
+    N = 32
+    H = heisenberg_oplist(N)
+    psi = randomMPS(2, N, 8, rng=np.random.default_rng(0))
+    start = [t.copy() for t in psi.tensors], psi.center
+    M = MPO(spinhalf(), H)
+    hist = []
+    t0 = time.perf_counter()
+    psi, e = dmrg(psi, M, maxsweeps=8, cutoff=1e-10, maxdim=64, krylovdim=4, verbose=False, history=hist)
+    gpu_s = time.perf_counter() - t0
+    out = {"gpu_seconds": gpu_s, "energy": e, "energy_per_site": e / N, "sweeps": len(hist),
+           "maxbonddim": hist[-1]["maxbonddim"]}
+    if cpu:
+        from oracle.dmrg import dmrg as dmrg_cpu
+        from oracle.mps import MPS
+
+        p = MPS([t.copy() for t in start[0]], start[1])
+        t0 = time.perf_counter()
+        _, ec = dmrg_cpu(p, [np.asarray(w) for w in M.tensors], maxsweeps=8, cutoff=1e-10, maxdim=64,
+                         krylovdim=4)
+        out["cpu_port_seconds"] = time.perf_counter() - t0
+        out["cpu_port_energy"] = ec
+        out["abs_dE_per_site_vs_cpu"] = abs(e - ec) / N
+    return out
+
+
+def dmrg_tfim_n100(chi, sweeps):
+    """Critical TFIM N = 100 (g = 1), random chi = 16 start (seed 0), cutoff 1e-12: an
+    all-FP64 run and an FP32-MFMA run with FP64 refinement passes, the same number of
+    passes each. Reports per-pass times and the energy errors per site."""
+    from tnhip import MPO, randomMPS, spinhalf
+    from tnhip.mps import dmrg
+
+    N = 100
+    e_exact = 1.0 - 1.0 / np.sin(np.pi / (4 * N + 2))  # open critical TFIM closed form
+    M = MPO(spinhalf(), tfim_oplist(N))
+    res = {}
+    for prec in ("f64", "f32"):
+        psi = randomMPS(2, N, 16, rng=np.random.default_rng(0))
+        hist = []
+        t0 = time.perf_counter()
+        psi, e = dmrg(psi, M, maxsweeps=sweeps, minsweeps=sweeps, tol=0.0, cutoff=1e-12, maxdim=chi,
+                      verbose=False, precision=prec, refine_sweeps=2, refine_tol=0.0, history=hist)
+        res[prec] = {"seconds": time.perf_counter() - t0, "energy": e,
+                     "abs_dE_per_site_vs_exact": abs(e - e_exact) / N,
+                     "pass_seconds": [round(h["seconds"], 4) for h in hist],
+                     "pass_precision": [h["precision"] for h in hist],
+                     "maxbonddim": hist[-1]["maxbonddim"]}
+    res["abs_dE_per_site_mixed_vs_f64"] = abs(res["f32"]["energy"] - res["f64"]["energy"]) / N
+    lo = [s for s, p in zip(res["f32"]["pass_seconds"], res["f32"]["pass_precision"]) if p == "f32"]
+    hi = res["f64"]["pass_seconds"][: len(lo)]
+    if lo:
+        res["f32_pass_speedup_vs_f64"] = sum(hi) / sum(lo)
+    res["config"] = {"N": N, "g": 1.0, "chi_max": chi, "passes": sweeps, "cutoff": 1e-12, "krylovdim": 3,
+                     "E0_exact": e_exact}
+    return res
+
+
+def bond_step(ctx, ops, Tensor, dL, dR, dW, dx, prec, chi, torch):
+    """One two-site DMRG bond step at chi on the synthetic workload: Lanczos (Krylov
+    dimension 3, 2 restarts, as dmrg.jl's defaults), the truncated split to chi, and the
+    left environment update. Returns seconds per phase."""
+    th = dx.astype(np.float64)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    e, nmv = ops.eigsolve_ground(dL, dW, dW, dR, th, krylovdim=3, maxiter=2, tol=1e-14, precision=prec)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    A1, A2, s, disc = ops.replacesites2(th, 0, True, 1e-12, chi, 1)
+    ctx.synchronize()
+    t2 = time.perf_counter()
+    Ln = ops.env_left(dL, A1, dW, precision=prec)
+    ctx.synchronize()
+    t3 = time.perf_counter()
+    del Ln, A1, A2, th
+    return {"lanczos_s": t1 - t0, "matvecs": nmv, "split_svd_s": t2 - t1, "env_update_s": t3 - t2,
+            "bond_step_s": t3 - t0, "energy": e, "kept": int(s.size)}
+
+
 def run_matvecs(ctx, T, ops, dL, dR, dW, dx, out, prec, steps, warmup, torch):
     for _ in range(warmup):
         ops.heff_apply(dL, dW, dW, dR, dx, out=out, precision=prec)
@@ -133,6 +242,11 @@ def main(argv=None):
     ap.add_argument("--chi", type=int, default=2048)
     ap.add_argument("--no-svd", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-dmrg", action="store_true", help="skip the DMRG legs")
+    ap.add_argument("--dmrg-chi", type=int, default=128, help="chi_max of the N=100 TFIM sweep leg")
+    ap.add_argument("--dmrg-passes", type=int, default=6)
+    ap.add_argument("--dmrg-only", action="store_true",
+                    help="run only the N=100 TFIM sweep leg (FP64 and FP32+refinement)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--svd-only", action="store_true",
                     help="run only the truncated SVD of the 2chi x 2chi theta matrix (1 warm-up + 1 timed)")
@@ -164,6 +278,9 @@ def main(argv=None):
 
     chi, D, d = a.chi, 5, 2
     ctx = tnhip.Context.default()
+    if a.dmrg_only:
+        print(json.dumps({"dmrg_tfim_N100": dmrg_tfim_n100(a.dmrg_chi, a.dmrg_passes)}))
+        return 0
     Lt, Rt, tht = make_inputs(chi, D, a.seed, torch)
     if a.svd_only:
         M = Tensor.wrap(tht.data_ptr(), (2 * chi, 2 * chi), np.float64, owner=tht, ctx=ctx)
@@ -235,6 +352,19 @@ def main(argv=None):
         ctx.synchronize()
         extra["svd_2chi"] = {"seconds": time.perf_counter() - t0, "kept": S.shape[0], "discarded": disc,
                              "fallback": fb, "n": 2 * chi}
+        del U, S, Vt
+    if not a.no_dmrg:
+        # DMRG bond step at chi (FP64 vs FP32-MFMA matvecs; FP64 Lanczos vectors and SVD)
+        steps_ = {}
+        for prec in ("f64", "f32"):
+            dL, dR = (dL64, dR64) if prec == "f64" else (dL32, dR32)
+            steps_[prec] = bond_step(ctx, ops, Tensor, dL, dR, dW, dx, prec, chi, torch)
+        steps_["bond_step_speedup_f32_vs_f64"] = steps_["f64"]["bond_step_s"] / steps_["f32"]["bond_step_s"]
+        steps_["lanczos_speedup_f32_vs_f64"] = steps_["f64"]["lanczos_s"] / steps_["f32"]["lanczos_s"]
+        steps_["half_sweep_estimate_s_N100"] = {p: 99 * steps_[p]["bond_step_s"] for p in ("f64", "f32")}
+        extra["dmrg_bond_step_chi%d" % chi] = steps_
+        extra["dmrg_tfim_N100"] = dmrg_tfim_n100(a.dmrg_chi, a.dmrg_passes)
+        extra["dmrg_heisenberg_N32"] = dmrg_heisenberg_n32(cpu=not a.no_cpu)
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline()

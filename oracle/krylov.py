@@ -40,6 +40,8 @@ def eigsolve_ground(matvec, x0, krylovdim=3, maxiter=2, tol=1e-14):
         for j in range(start, kdim):
             w = mv(V[j])
             nmv += 1
+            if np.iscomplexobj(w) and not np.iscomplexobj(V):  # complex operator, real start
+                V, H = V.astype(np.complex128), H.astype(np.complex128)
             for _ in range(2):  # classical Gram-Schmidt, twice
                 c = V[: j + 1].conj() @ w
                 H[: j + 1, j] += c

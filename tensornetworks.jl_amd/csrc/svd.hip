@@ -10,9 +10,11 @@
 // Block Jacobi: columns are cut into blocks of JB = 16; a round-robin (circle)
 // schedule pairs all blocks per round, and every pair (32 columns) is handled by one
 // workgroup that diagonalises its 32 x 32 sub-problem in LDS with a parallel cyclic
-// Jacobi (16 disjoint rotations per step, exact zeroing of the rotated entry). The
-// rotations are then applied to the full columns (and rows, two-sided) of G / M and
-// to the accumulated unitary V. Pairs already orthogonal to tolerance are skipped.
+// Jacobi: 16 disjoint rotations per step, applied in one in-place pass over the 2 x 2
+// blocks they couple (exact zeroing of the rotated entry), at most `max_inner` inner
+// sweeps unless the pair is the whole matrix. The accumulated 32 x 32 rotation of each
+// pair is then applied to the full columns (and rows, two-sided) of G / M and to the
+// accumulated unitary V. Pairs already orthogonal to tolerance are skipped.
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -26,8 +28,10 @@ namespace {
 
 constexpr int JB = 16;
 constexpr int JP = 2 * JB;
+constexpr int NS = JP / 2;  // rotations per inner step
 constexpr int kGramRows = 64;
-constexpr int kMaxInner = 15;
+constexpr int kGramPer = JP * JP / 256;  // Gram entries per thread
+constexpr int kMaxInnerCap = 32;
 
 // circle-method pairing of n (even) players in round `step`, pair `i`
 __host__ __device__ inline void circle_pair(int n, int step, int i, int *p, int *q) {
@@ -71,7 +75,9 @@ __global__ __launch_bounds__(256) void jac_gram_kernel(const double *__restrict_
   int chunk = (rows + nsplit - 1) / nsplit;
   chunk = (chunk + kGramRows - 1) / kGramRows * kGramRows;
   const int r0 = split * chunk, r1 = min(rows, r0 + chunk);
-  double ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
+  double ar[kGramPer], ai[kGramPer];
+#pragma unroll
+  for (int q = 0; q < kGramPer; ++q) ar[q] = ai[q] = 0.0;
   for (int rb = r0; rb < r1; rb += kGramRows) {
     for (int e = t; e < kGramRows * JP; e += 256) {
       const int rr = e % kGramRows, k = e / kGramRows;
@@ -86,7 +92,7 @@ __global__ __launch_bounds__(256) void jac_gram_kernel(const double *__restrict_
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < kGramPer; ++q) {
       const int e = t + 256 * q, i = e % JP, j = e / JP;
       double sr = 0, si = 0;
       for (int rr = 0; rr < kGramRows; ++rr) {
@@ -101,35 +107,42 @@ __global__ __launch_bounds__(256) void jac_gram_kernel(const double *__restrict_
   }
   double *out = partial + int64_t(pair * nsplit + split) * 2 * JP * JP;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < kGramPer; ++q) {
     const int e = t + 256 * q;
     out[e] = ar[q];
     out[JP * JP + e] = ai[q];
   }
 }
 
-// Diagonalise the 32x32 Hermitian sub-problem of each pair. Two-sided mode reads
+// Diagonalise the 32 x 32 Hermitian sub-problem of each pair. Two-sided mode reads
 // G[P,P] from the global matrix; one-sided mode sums the partial Grams. Writes the
 // accumulated unitary Q (and the diagonalised block D in two-sided mode), an
 // `active` flag (0 = pair already converged, nothing to apply) and the pair's
 // off-diagonal measure into conv (atomic max over the sweep).
+//
+// Inner step: the 16 rotations of one circle-method round act on disjoint index pairs,
+// so G <- J^H G J splits into 16 x 16 independent 2 x 2 blocks {p_I, q_I} x {p_J, q_J};
+// each thread updates one block in place (columns by J_J, then rows by J_I^H), and the
+// rotated diagonal blocks take their exact values. One barrier after the rotation
+// parameters, one after the update. CPLX = false skips the imaginary planes.
+template <bool CPLX>
 __global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__ G, int64_t imG,
                                                       int n, const double *__restrict__ partial,
-                                                      int nsplit, bool two_sided, bool cplx,
+                                                      int nsplit, bool two_sided,
                                                       const int *__restrict__ pairs, int cols,
-                                                      double tol, double abstol,
+                                                      double tol, double abstol, int max_inner,
                                                       double *__restrict__ Qout,
                                                       double *__restrict__ Dout,
                                                       int *__restrict__ active,
                                                       unsigned long long *__restrict__ conv) {
   const int pair = blockIdx.x, t = threadIdx.x;
-  __shared__ double gr[JP][JP + 1], gi[JP][JP + 1], qr[JP][JP + 1], qi[JP][JP + 1];
+  __shared__ double gr[JP][JP + 1], gi[CPLX ? JP : 1][JP + 1], qr[JP][JP + 1], qi[CPLX ? JP : 1][JP + 1];
   __shared__ int colmap[JP];
-  __shared__ int rp[JP / 2], rq[JP / 2], rrot[JP / 2];
-  __shared__ double rcs[JP / 2], rsn[JP / 2], rer[JP / 2], rei[JP / 2], ra[JP / 2],
-      rc[JP / 2], rtb[JP / 2];
-  __shared__ int any;
+  __shared__ int rp[NS], rq[NS], rrot[NS];
+  __shared__ double rcs[NS], rsn[NS], rer[NS], rei[NS], ra[NS], rc[NS], rtb[NS];
+  __shared__ int anyrot[kMaxInnerCap];
   if (t < JP) colmap[t] = pair_col(pairs + 2 * pair, t, cols);
+  if (t < kMaxInnerCap) anyrot[t] = 0;
   __syncthreads();
   for (int e = t; e < JP * JP; e += 256) {
     const int i = e % JP, j = e / JP;
@@ -138,28 +151,29 @@ __global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__
       const int ci = colmap[i], cj = colmap[j];
       if (ci >= 0 && cj >= 0) {
         vr = G[ci + int64_t(cj) * n];
-        if (cplx) vi = G[imG + ci + int64_t(cj) * n];
+        if (CPLX) vi = G[imG + ci + int64_t(cj) * n];
       }
     } else {
       for (int s = 0; s < nsplit; ++s) {
         const double *pp = partial + int64_t(pair * nsplit + s) * 2 * JP * JP;
         vr += pp[e];
-        vi += pp[JP * JP + e];
+        if (CPLX) vi += pp[JP * JP + e];
       }
     }
-    if (i == j) vi = 0;
     gr[i][j] = vr;
-    gi[i][j] = vi;
     qr[i][j] = (i == j) ? 1.0 : 0.0;
-    qi[i][j] = 0.0;
+    if (CPLX) {
+      gi[i][j] = (i == j) ? 0.0 : vi;
+      qi[i][j] = 0.0;
+    }
   }
   __syncthreads();
-  // off-diagonal measure: max |g_ij| / sqrt(|g_ii g_jj|) over i != j
+  // off-diagonal measure: max |g_ij| / sqrt(|g_ii g_jj|) over i != j above the noise floor
   double m = 0;
   for (int e = t; e < JP * JP; e += 256) {
     const int i = e % JP, j = e / JP;
     if (i == j) continue;
-    const double b = hypot(gr[i][j], gi[i][j]);
+    const double b = CPLX ? hypot(gr[i][j], gi[i][j]) : fabs(gr[i][j]);
     if (b <= abstol) continue;
     const double s = sqrt(fabs(gr[i][i] * gr[j][j]));
     m = fmax(m, s > 0 ? b / s : 1.0);
@@ -170,16 +184,15 @@ __global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__
     if (t == 0) active[pair] = 0;
     return;
   }
-  for (int sw = 0; sw < kMaxInner; ++sw) {
-    if (t == 0) any = 0;
-    __syncthreads();
+  const int sweeps = max_inner < kMaxInnerCap ? max_inner : kMaxInnerCap;
+  for (int sw = 0; sw < sweeps; ++sw) {
     for (int step = 0; step < JP - 1; ++step) {
-      if (t < JP / 2) {
+      if (t < NS) {
         int p, q;
         circle_pair(JP, step, t, &p, &q);
         const double a = gr[p][p], c = gr[q][q];
-        const double br = gr[p][q], bi = gi[p][q];
-        const double bb = hypot(br, bi);
+        const double br = gr[p][q], bi = CPLX ? gi[p][q] : 0.0;
+        const double bb = CPLX ? hypot(br, bi) : fabs(br);
         const bool rot = bb > abstol && bb > tol * sqrt(fabs(a * c));
         double cs = 1, sn = 0, tt = 0, er = 1, ei = 0;
         if (rot) {
@@ -189,7 +202,7 @@ __global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__
           sn = tt * cs;
           er = br / bb;  // e^{-i phi} = conj(b) / |b|
           ei = -bi / bb;
-          any = 1;
+          anyrot[sw] = 1;
         }
         rp[t] = p;
         rq[t] = q;
@@ -203,88 +216,122 @@ __global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__
         rtb[t] = tt * bb;
       }
       __syncthreads();
-      // columns: [g_p g_q] <- [g_p g_q] J,  J = [[cs, sn], [-sn e, cs e]]
-      for (int task = t; task < (JP / 2) * JP; task += 256) {
-        const int k = task / JP, r = task % JP;
-        if (!rrot[k]) continue;
-        const int p = rp[k], q = rq[k];
-        const double cs = rcs[k], sn = rsn[k], er = rer[k], ei = rei[k];
-        {
-          const double pr_ = gr[r][p], pi_ = gi[r][p], qr_ = gr[r][q], qi_ = gi[r][q];
-          const double eqr = er * qr_ - ei * qi_, eqi = er * qi_ + ei * qr_;
-          gr[r][p] = cs * pr_ - sn * eqr;
-          gi[r][p] = cs * pi_ - sn * eqi;
-          gr[r][q] = sn * pr_ + cs * eqr;
-          gi[r][q] = sn * pi_ + cs * eqi;
+      // G: one 2 x 2 block (I rows, J columns) per thread
+      for (int task = t; task < NS * NS; task += 256) {
+        const int I = task % NS, J = task / NS;
+        const bool rI = rrot[I], rJ = rrot[J];
+        if (!rI && !rJ) continue;
+        const int pI = rp[I], qI = rq[I], pJ = rp[J], qJ = rq[J];
+        if (I == J) {  // rotated diagonal block: exact values
+          gr[pI][qI] = gr[qI][pI] = 0.0;
+          gr[pI][pI] = ra[I] - rtb[I];
+          gr[qI][qI] = rc[I] + rtb[I];
+          if (CPLX) gi[pI][qI] = gi[qI][pI] = gi[pI][pI] = gi[qI][qI] = 0.0;
+          continue;
         }
-        {
-          const double pr_ = qr[r][p], pi_ = qi[r][p], qr_ = qr[r][q], qi_ = qi[r][q];
-          const double eqr = er * qr_ - ei * qi_, eqi = er * qi_ + ei * qr_;
-          qr[r][p] = cs * pr_ - sn * eqr;
+        // x = [[g(pI,pJ), g(pI,qJ)], [g(qI,pJ), g(qI,qJ)]]
+        double x00r = gr[pI][pJ], x01r = gr[pI][qJ], x10r = gr[qI][pJ], x11r = gr[qI][qJ];
+        double x00i = 0, x01i = 0, x10i = 0, x11i = 0;
+        if (CPLX) {
+          x00i = gi[pI][pJ];
+          x01i = gi[pI][qJ];
+          x10i = gi[qI][pJ];
+          x11i = gi[qI][qJ];
+        }
+        if (rJ) {  // columns: [x_p x_q] <- [x_p x_q] [[cs, sn], [-sn e, cs e]]
+          const double cs = rcs[J], sn = rsn[J], er = rer[J], ei = rei[J];
+          double eqr = er * x01r - ei * x01i, eqi = er * x01i + ei * x01r;
+          double npr = cs * x00r - sn * eqr, npi = cs * x00i - sn * eqi;
+          double nqr = sn * x00r + cs * eqr, nqi = sn * x00i + cs * eqi;
+          x00r = npr; x00i = npi; x01r = nqr; x01i = nqi;
+          eqr = er * x11r - ei * x11i;
+          eqi = er * x11i + ei * x11r;
+          npr = cs * x10r - sn * eqr;
+          npi = cs * x10i - sn * eqi;
+          nqr = sn * x10r + cs * eqr;
+          nqi = sn * x10i + cs * eqi;
+          x10r = npr; x10i = npi; x11r = nqr; x11i = nqi;
+        }
+        if (rI) {  // rows: [x_p; x_q] <- J^H [x_p; x_q]  (conj(e))
+          const double cs = rcs[I], sn = rsn[I], er = rer[I], ei = -rei[I];
+          double eqr = er * x10r - ei * x10i, eqi = er * x10i + ei * x10r;
+          double npr = cs * x00r - sn * eqr, npi = cs * x00i - sn * eqi;
+          double nqr = sn * x00r + cs * eqr, nqi = sn * x00i + cs * eqi;
+          x00r = npr; x00i = npi; x10r = nqr; x10i = nqi;
+          eqr = er * x11r - ei * x11i;
+          eqi = er * x11i + ei * x11r;
+          npr = cs * x01r - sn * eqr;
+          npi = cs * x01i - sn * eqi;
+          nqr = sn * x01r + cs * eqr;
+          nqi = sn * x01i + cs * eqi;
+          x01r = npr; x01i = npi; x11r = nqr; x11i = nqi;
+        }
+        gr[pI][pJ] = x00r;
+        gr[pI][qJ] = x01r;
+        gr[qI][pJ] = x10r;
+        gr[qI][qJ] = x11r;
+        if (CPLX) {
+          gi[pI][pJ] = x00i;
+          gi[pI][qJ] = x01i;
+          gi[qI][pJ] = x10i;
+          gi[qI][qJ] = x11i;
+        }
+      }
+      // Q <- Q J, one row of a rotated column pair per task
+      for (int task = t; task < JP * NS; task += 256) {
+        const int r = task % JP, J = task / JP;
+        if (!rrot[J]) continue;
+        const int p = rp[J], q = rq[J];
+        const double cs = rcs[J], sn = rsn[J], er = rer[J], ei = rei[J];
+        const double pr_ = qr[r][p], qr_ = qr[r][q];
+        const double pi_ = CPLX ? qi[r][p] : 0.0, qi_ = CPLX ? qi[r][q] : 0.0;
+        const double eqr = er * qr_ - ei * qi_, eqi = er * qi_ + ei * qr_;
+        qr[r][p] = cs * pr_ - sn * eqr;
+        qr[r][q] = sn * pr_ + cs * eqr;
+        if (CPLX) {
           qi[r][p] = cs * pi_ - sn * eqi;
-          qr[r][q] = sn * pr_ + cs * eqr;
           qi[r][q] = sn * pi_ + cs * eqi;
         }
       }
       __syncthreads();
-      // rows: [g_p; g_q] <- J^H [g_p; g_q]
-      for (int task = t; task < (JP / 2) * JP; task += 256) {
-        const int k = task / JP, c = task % JP;
-        if (!rrot[k]) continue;
-        const int p = rp[k], q = rq[k];
-        const double cs = rcs[k], sn = rsn[k], er = rer[k], ei = -rei[k];  // conj(e)
-        const double pr_ = gr[p][c], pi_ = gi[p][c], qr_ = gr[q][c], qi_ = gi[q][c];
-        const double eqr = er * qr_ - ei * qi_, eqi = er * qi_ + ei * qr_;
-        gr[p][c] = cs * pr_ - sn * eqr;
-        gi[p][c] = cs * pi_ - sn * eqi;
-        gr[q][c] = sn * pr_ + cs * eqr;
-        gi[q][c] = sn * pi_ + cs * eqi;
-      }
-      __syncthreads();
-      if (t < JP / 2 && rrot[t]) {
-        const int p = rp[t], q = rq[t];
-        gr[p][q] = gi[p][q] = gr[q][p] = gi[q][p] = 0.0;
-        gr[p][p] = ra[t] - rtb[t];
-        gr[q][q] = rc[t] + rtb[t];
-        gi[p][p] = gi[q][q] = 0.0;
-      }
-      __syncthreads();
     }
-    if (!any) break;
+    // anyrot[sw] is complete after the step's final barrier and never rewritten
+    if (!anyrot[sw]) break;
   }
   double *Q = Qout + int64_t(pair) * 2 * JP * JP;
   for (int e = t; e < JP * JP; e += 256) {
     const int i = e % JP, j = e / JP;
     Q[e] = qr[i][j];
-    Q[JP * JP + e] = qi[i][j];
+    Q[JP * JP + e] = CPLX ? qi[i][j] : 0.0;
   }
   if (two_sided) {
     double *D = Dout + int64_t(pair) * 2 * JP * JP;
     for (int e = t; e < JP * JP; e += 256) {
       const int i = e % JP, j = e / JP;
       D[e] = gr[i][j];
-      D[JP * JP + e] = gi[i][j];
+      D[JP * JP + e] = CPLX ? gi[i][j] : 0.0;
     }
   }
   if (t == 0) active[pair] = 1;
 }
 
 // Y[:, P] <- Y[:, P] Q_P for Y in {X (rows_x rows), V (rows_v rows)}.
+template <bool CPLX>
 __global__ __launch_bounds__(256) void jac_colupd_kernel(double *__restrict__ X, int rows_x,
                                                          int64_t imX, double *__restrict__ V,
-                                                         int rows_v, int64_t imV, bool cplx,
+                                                         int rows_v, int64_t imV,
                                                          const int *__restrict__ pairs, int cols,
                                                          const double *__restrict__ Qin,
                                                          const int *__restrict__ active) {
   const int pair = blockIdx.y;
   if (!active[pair]) return;
-  __shared__ double qr[JP * JP], qi[JP * JP];
+  __shared__ double qr[JP * JP], qi[CPLX ? JP * JP : 1];
   __shared__ int colmap[JP];
   const int t = threadIdx.x;
   const double *Q = Qin + int64_t(pair) * 2 * JP * JP;
   for (int e = t; e < JP * JP; e += 256) {
     qr[e] = Q[e];
-    qi[e] = Q[JP * JP + e];
+    if (CPLX) qi[e] = Q[JP * JP + e];
   }
   if (t < JP) colmap[t] = pair_col(pairs + 2 * pair, t, cols);
   __syncthreads();
@@ -294,12 +341,12 @@ __global__ __launch_bounds__(256) void jac_colupd_kernel(double *__restrict__ X,
     const int rows = which ? rows_v : rows_x;
     const int64_t im = which ? imV : imX;
     if (!Y || r >= rows) continue;
-    double yr[JP], yi[JP];
+    double yr[JP], yi[CPLX ? JP : 1];
 #pragma unroll
     for (int k = 0; k < JP; ++k) {
       const int c = colmap[k];
       yr[k] = c >= 0 ? Y[r + int64_t(c) * rows] : 0.0;
-      yi[k] = (c >= 0 && cplx) ? Y[im + r + int64_t(c) * rows] : 0.0;
+      if (CPLX) yi[k] = c >= 0 ? Y[im + r + int64_t(c) * rows] : 0.0;
     }
     for (int j = 0; j < JP; ++j) {
       const int c = colmap[j];
@@ -307,32 +354,37 @@ __global__ __launch_bounds__(256) void jac_colupd_kernel(double *__restrict__ X,
       double sr = 0, si = 0;
 #pragma unroll
       for (int k = 0; k < JP; ++k) {
-        const double a = qr[k + JP * j], b = qi[k + JP * j];
-        sr += yr[k] * a - yi[k] * b;
-        si += yr[k] * b + yi[k] * a;
+        const double a = qr[k + JP * j];
+        sr += yr[k] * a;
+        if (CPLX) {
+          const double b = qi[k + JP * j];
+          sr -= yi[k] * b;
+          si += yr[k] * b + yi[k] * a;
+        }
       }
       Y[r + int64_t(c) * rows] = sr;
-      if (cplx) Y[im + r + int64_t(c) * rows] = si;
+      if (CPLX) Y[im + r + int64_t(c) * rows] = si;
     }
   }
 }
 
 // Two-sided mode: G[P, :] <- Q_P^H G[P, :]; entries in P x P take the diagonalised block D.
+template <bool CPLX>
 __global__ __launch_bounds__(256) void jac_rowupd_kernel(double *__restrict__ G, int n, int64_t imG,
-                                                         bool cplx, const int *__restrict__ pairs,
+                                                         const int *__restrict__ pairs,
                                                          const double *__restrict__ Qin,
                                                          const double *__restrict__ Din,
                                                          const int *__restrict__ active) {
   const int pair = blockIdx.y;
   if (!active[pair]) return;
-  __shared__ double qr[JP * JP], qi[JP * JP];
+  __shared__ double qr[JP * JP], qi[CPLX ? JP * JP : 1];
   __shared__ int colmap[JP];
   const int t = threadIdx.x;
   const double *Q = Qin + int64_t(pair) * 2 * JP * JP;
   const double *D = Din + int64_t(pair) * 2 * JP * JP;
   for (int e = t; e < JP * JP; e += 256) {
     qr[e] = Q[e];
-    qi[e] = Q[JP * JP + e];
+    if (CPLX) qi[e] = Q[JP * JP + e];
   }
   if (t < JP) colmap[t] = pair_col(pairs + 2 * pair, t, n);
   __syncthreads();
@@ -346,16 +398,16 @@ __global__ __launch_bounds__(256) void jac_rowupd_kernel(double *__restrict__ G,
       const int rj = colmap[j];
       if (rj < 0) continue;
       G[rj + int64_t(c) * n] = D[j + JP * local];
-      if (cplx) G[imG + rj + int64_t(c) * n] = D[JP * JP + j + JP * local];
+      if (CPLX) G[imG + rj + int64_t(c) * n] = D[JP * JP + j + JP * local];
     }
     return;
   }
-  double gr[JP], gi[JP];
+  double gr[JP], gi[CPLX ? JP : 1];
 #pragma unroll
   for (int k = 0; k < JP; ++k) {
     const int rk = colmap[k];
     gr[k] = rk >= 0 ? G[rk + int64_t(c) * n] : 0.0;
-    gi[k] = (rk >= 0 && cplx) ? G[imG + rk + int64_t(c) * n] : 0.0;
+    if (CPLX) gi[k] = rk >= 0 ? G[imG + rk + int64_t(c) * n] : 0.0;
   }
   for (int j = 0; j < JP; ++j) {
     const int rj = colmap[j];
@@ -363,12 +415,16 @@ __global__ __launch_bounds__(256) void jac_rowupd_kernel(double *__restrict__ G,
     double sr = 0, si = 0;
 #pragma unroll
     for (int k = 0; k < JP; ++k) {
-      const double a = qr[k + JP * j], b = -qi[k + JP * j];  // conj(Q[k][j])
-      sr += a * gr[k] - b * gi[k];
-      si += a * gi[k] + b * gr[k];
+      const double a = qr[k + JP * j];
+      sr += a * gr[k];
+      if (CPLX) {
+        const double b = -qi[k + JP * j];  // conj(Q[k][j])
+        sr -= b * gi[k];
+        si += a * gi[k] + b * gr[k];
+      }
     }
     G[rj + int64_t(c) * n] = sr;
-    if (cplx) G[imG + rj + int64_t(c) * n] = si;
+    if (CPLX) G[imG + rj + int64_t(c) * n] = si;
   }
 }
 
@@ -525,20 +581,34 @@ static JacobiResult jacobi(tnh_ctx *ctx, bool two_sided, double *X, int rows, in
       ctx->workspace(605, sizeof(unsigned long long) * (max_sweeps + 1)));
   TNH_HIP(hipMemsetAsync(conv, 0, sizeof(unsigned long long) * (max_sweeps + 1), ctx->stream));
   const int rmax = std::max(rows, n);
+  // a single pair covering every column is the whole problem: solve it to convergence
+  const int max_inner = (s.nb <= 2) ? kMaxInnerCap : 3;
   for (int sw = 0; sw < max_sweeps; ++sw) {
     for (int r = 0; r < s.rounds; ++r) {
       const int *pr = s.dev + size_t(r) * s.npairs * 2;
       if (!two_sided)
         hipLaunchKernelGGL(jac_gram_kernel, dim3(s.npairs, nsplit), dim3(256), 0, ctx->stream, X,
                            rows, imX, n, cplx, pr, nsplit, partial);
-      hipLaunchKernelGGL(jac_eig_kernel, dim3(s.npairs), dim3(256), 0, ctx->stream, X, imX, n,
-                         partial, nsplit, two_sided, cplx, pr, n, tol, abstol, Q, D, active,
-                         conv + sw);
-      hipLaunchKernelGGL(jac_colupd_kernel, dim3((rmax + 255) / 256, s.npairs), dim3(256), 0,
-                         ctx->stream, X, rows, imX, V, n, imV, cplx, pr, n, Q, active);
-      if (two_sided)
-        hipLaunchKernelGGL(jac_rowupd_kernel, dim3((n + 255) / 256, s.npairs), dim3(256), 0,
-                           ctx->stream, X, n, imX, cplx, pr, Q, D, active);
+      if (cplx)
+        hipLaunchKernelGGL(jac_eig_kernel<true>, dim3(s.npairs), dim3(256), 0, ctx->stream, X, imX,
+                           n, partial, nsplit, two_sided, pr, n, tol, abstol, max_inner, Q, D,
+                           active, conv + sw);
+      else
+        hipLaunchKernelGGL(jac_eig_kernel<false>, dim3(s.npairs), dim3(256), 0, ctx->stream, X, imX,
+                           n, partial, nsplit, two_sided, pr, n, tol, abstol, max_inner, Q, D,
+                           active, conv + sw);
+      if (cplx)
+        hipLaunchKernelGGL(jac_colupd_kernel<true>, dim3((rmax + 255) / 256, s.npairs), dim3(256), 0,
+                           ctx->stream, X, rows, imX, V, n, imV, pr, n, Q, active);
+      else
+        hipLaunchKernelGGL(jac_colupd_kernel<false>, dim3((rmax + 255) / 256, s.npairs), dim3(256), 0,
+                           ctx->stream, X, rows, imX, V, n, imV, pr, n, Q, active);
+      if (two_sided && cplx)
+        hipLaunchKernelGGL(jac_rowupd_kernel<true>, dim3((n + 255) / 256, s.npairs), dim3(256), 0,
+                           ctx->stream, X, n, imX, pr, Q, D, active);
+      else if (two_sided)
+        hipLaunchKernelGGL(jac_rowupd_kernel<false>, dim3((n + 255) / 256, s.npairs), dim3(256), 0,
+                           ctx->stream, X, n, imX, pr, Q, D, active);
     }
     TNH_HIP(hipGetLastError());
     unsigned long long bits = 0;
@@ -652,7 +722,10 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
     hipLaunchKernelGGL(eye_kernel, dim3(unsigned((pp + 255) / 256)), dim3(256), 0, ctx->stream, V, p,
                        pp, cplx);
     const double abstol = 4.0 * 2.220446049250313e-16 * std::sqrt(double(p)) * dmax;
-    JacobiResult jr = jacobi(ctx, true, G, p, p, cplx, V, 1e-14, abstol, 40);
+    // relative off-diagonal tolerance 4 sqrt(p) eps (at least 1e-14), the scale of the
+    // rounding that p-term inner products leave
+    const double tol = std::max(1e-14, 4.0 * 2.220446049250313e-16 * std::sqrt(double(p)));
+    JacobiResult jr = jacobi(ctx, true, G, p, p, cplx, V, tol, abstol, 40);
     if (jr.converged) {
       double *dv = static_cast<double *>(ctx->workspace(612, sizeof(double) * p));
       hipLaunchKernelGGL(diag_kernel, dim3((p + 255) / 256), dim3(256), 0, ctx->stream, G, p, dv);

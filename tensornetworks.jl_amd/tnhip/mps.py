@@ -6,6 +6,7 @@ src/algorithms/mps/dmrg.jl. Site tensors travel as Fortran-ordered arrays."""
 from __future__ import annotations
 
 import ctypes as C
+import time
 
 import numpy as np
 
@@ -204,7 +205,10 @@ def dmrg(psi: GMPS, *Hs: GMPS, nsites=2, krylovdim=3, kryloviter=2, ishermitian=
     while not converged:
         if prec != "f64" and (sweeps >= maxsweeps - refine_sweeps > 0 if maxsweeps else False):
             prec = "f64"
+        t_sweep = time.perf_counter()
+        pass_prec = prec
         cost, D, disc = st.sweep(direction, krylovdim, kryloviter, 1e-14, cutoff, maxdim, mindim, prec)
+        t_sweep = time.perf_counter() - t_sweep  # the sweep returns host scalars: synchronised
         if prec == "f64":
             fp64_passes += 1
         direction = not direction
@@ -230,7 +234,7 @@ def dmrg(psi: GMPS, *Hs: GMPS, nsites=2, krylovdim=3, kryloviter=2, ishermitian=
         lastD = D
         if history is not None:
             history.append({"sweep": sweeps, "energy": cost, "maxbonddim": D, "discarded": disc,
-                            "precision": prec})
+                            "precision": pass_prec, "seconds": t_sweep})
         if verbose:
             print(f"Sweep={sweeps}, energy={cost:.12f}, maxbonddim={D} ")
     psi.tensors = st.tensors()
