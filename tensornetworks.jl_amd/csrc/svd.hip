@@ -368,63 +368,117 @@ __global__ __launch_bounds__(256) void jac_colupd_kernel(double *__restrict__ X,
   }
 }
 
-// Two-sided mode: G[P, :] <- Q_P^H G[P, :]; entries in P x P take the diagonalised block D.
+// Two-sided mode, both sides in one pass: G[P, K] <- Q_P^H G[P, K] Q_K for every pair of
+// pairs (P = blockIdx.x rows, K = blockIdx.y columns), Q of an inactive pair being the
+// identity; the diagonal tiles take the diagonalised blocks D. One 32 x 32 tile and the
+// two rotations in LDS; G is read and written once per round.
 template <bool CPLX>
-__global__ __launch_bounds__(256) void jac_rowupd_kernel(double *__restrict__ G, int n, int64_t imG,
-                                                         const int *__restrict__ pairs,
-                                                         const double *__restrict__ Qin,
-                                                         const double *__restrict__ Din,
-                                                         const int *__restrict__ active) {
-  const int pair = blockIdx.y;
-  if (!active[pair]) return;
-  __shared__ double qr[JP * JP], qi[CPLX ? JP * JP : 1];
-  __shared__ int colmap[JP];
-  const int t = threadIdx.x;
-  const double *Q = Qin + int64_t(pair) * 2 * JP * JP;
-  const double *D = Din + int64_t(pair) * 2 * JP * JP;
-  for (int e = t; e < JP * JP; e += 256) {
-    qr[e] = Q[e];
-    if (CPLX) qi[e] = Q[JP * JP + e];
+__global__ __launch_bounds__(256) void jac_twoside_kernel(double *__restrict__ G, int n, int64_t imG,
+                                                          const int *__restrict__ pairs,
+                                                          const double *__restrict__ Qin,
+                                                          const double *__restrict__ Din,
+                                                          const int *__restrict__ active) {
+  const int P = blockIdx.x, K = blockIdx.y, t = threadIdx.x;
+  const bool aP = active[P] != 0, aK = active[K] != 0;
+  if (!aP && !aK) return;
+  __shared__ double tr[JP][JP + 1], ti[CPLX ? JP : 1][JP + 1];
+  __shared__ double pr_[JP][JP + 1], pi_[CPLX ? JP : 1][JP + 1];
+  __shared__ double kr_[JP][JP + 1], ki_[CPLX ? JP : 1][JP + 1];
+  __shared__ int rowmap[JP], colmap[JP];
+  if (t < JP) {
+    rowmap[t] = pair_col(pairs + 2 * P, t, n);
+    colmap[t] = pair_col(pairs + 2 * K, t, n);
   }
-  if (t < JP) colmap[t] = pair_col(pairs + 2 * pair, t, n);
   __syncthreads();
-  const int c = blockIdx.x * 256 + t;
-  if (c >= n) return;
-  int local = -1;
-  for (int k = 0; k < JP; ++k)
-    if (colmap[k] == c) local = k;
-  if (local >= 0) {
-    for (int j = 0; j < JP; ++j) {
-      const int rj = colmap[j];
-      if (rj < 0) continue;
-      G[rj + int64_t(c) * n] = D[j + JP * local];
-      if (CPLX) G[imG + rj + int64_t(c) * n] = D[JP * JP + j + JP * local];
+  if (P == K) {  // aP holds here
+    const double *D = Din + int64_t(P) * 2 * JP * JP;
+    for (int e = t; e < JP * JP; e += 256) {
+      const int i = e % JP, j = e / JP;
+      const int r = rowmap[i], c = colmap[j];
+      if (r < 0 || c < 0) continue;
+      G[r + int64_t(c) * n] = D[e];
+      if (CPLX) G[imG + r + int64_t(c) * n] = D[JP * JP + e];
     }
     return;
   }
-  double gr[JP], gi[CPLX ? JP : 1];
-#pragma unroll
-  for (int k = 0; k < JP; ++k) {
-    const int rk = colmap[k];
-    gr[k] = rk >= 0 ? G[rk + int64_t(c) * n] : 0.0;
-    if (CPLX) gi[k] = rk >= 0 ? G[imG + rk + int64_t(c) * n] : 0.0;
-  }
-  for (int j = 0; j < JP; ++j) {
-    const int rj = colmap[j];
-    if (rj < 0) continue;
-    double sr = 0, si = 0;
-#pragma unroll
-    for (int k = 0; k < JP; ++k) {
-      const double a = qr[k + JP * j];
-      sr += a * gr[k];
-      if (CPLX) {
-        const double b = -qi[k + JP * j];  // conj(Q[k][j])
-        sr -= b * gi[k];
-        si += a * gi[k] + b * gr[k];
-      }
+  const double *QP = Qin + int64_t(P) * 2 * JP * JP;
+  const double *QK = Qin + int64_t(K) * 2 * JP * JP;
+  for (int e = t; e < JP * JP; e += 256) {
+    const int i = e % JP, j = e / JP;
+    const int r = rowmap[i], c = colmap[j];
+    const bool ok = r >= 0 && c >= 0;
+    tr[i][j] = ok ? G[r + int64_t(c) * n] : 0.0;
+    if (CPLX) ti[i][j] = ok ? G[imG + r + int64_t(c) * n] : 0.0;
+    if (aP) {
+      pr_[i][j] = QP[e];
+      if (CPLX) pi_[i][j] = QP[JP * JP + e];
     }
-    G[rj + int64_t(c) * n] = sr;
-    if (CPLX) G[imG + rj + int64_t(c) * n] = si;
+    if (aK) {
+      kr_[i][j] = QK[e];
+      if (CPLX) ki_[i][j] = QK[JP * JP + e];
+    }
+  }
+  __syncthreads();
+  constexpr int PER = JP * JP / 256;  // entries per thread
+  double sr[PER], si[PER];
+  if (aK) {  // T <- T Q_K
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = t + 256 * q, i = e % JP, j = e / JP;
+      double ar = 0, ai = 0;
+      for (int k = 0; k < JP; ++k) {
+        const double a = tr[i][k], c = kr_[k][j];
+        ar += a * c;
+        if (CPLX) {
+          const double b = ti[i][k], d = ki_[k][j];
+          ar -= b * d;
+          ai += a * d + b * c;
+        }
+      }
+      sr[q] = ar;
+      si[q] = ai;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = t + 256 * q, i = e % JP, j = e / JP;
+      tr[i][j] = sr[q];
+      if (CPLX) ti[i][j] = si[q];
+    }
+    __syncthreads();
+  }
+  if (aP) {  // T <- Q_P^H T
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = t + 256 * q, i = e % JP, j = e / JP;
+      double ar = 0, ai = 0;
+      for (int k = 0; k < JP; ++k) {
+        const double a = pr_[k][i], c = tr[k][j];  // conj(Q_P[k][i]) T[k][j]
+        ar += a * c;
+        if (CPLX) {
+          const double b = -pi_[k][i], d = ti[k][j];
+          ar -= b * d;
+          ai += a * d + b * c;
+        }
+      }
+      sr[q] = ar;
+      si[q] = ai;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = t + 256 * q, i = e % JP, j = e / JP;
+      sr[q] = tr[i][j];
+      si[q] = CPLX ? ti[i][j] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = t + 256 * q, i = e % JP, j = e / JP;
+    const int r = rowmap[i], c = colmap[j];
+    if (r < 0 || c < 0) continue;
+    G[r + int64_t(c) * n] = sr[q];
+    if (CPLX) G[imG + r + int64_t(c) * n] = si[q];
   }
 }
 
@@ -581,8 +635,10 @@ static JacobiResult jacobi(tnh_ctx *ctx, bool two_sided, double *X, int rows, in
       ctx->workspace(605, sizeof(unsigned long long) * (max_sweeps + 1)));
   TNH_HIP(hipMemsetAsync(conv, 0, sizeof(unsigned long long) * (max_sweeps + 1), ctx->stream));
   const int rmax = std::max(rows, n);
-  // a single pair covering every column is the whole problem: solve it to convergence
-  const int max_inner = (s.nb <= 2) ? kMaxInnerCap : 3;
+  // a single pair covering every column is the whole problem: solve it to convergence.
+  // Otherwise one inner sweep per pair and round: measured fastest overall (the outer
+  // sweeps revisit every pair, so fully converged sub-problems buy little).
+  const int max_inner = (s.nb <= 2) ? kMaxInnerCap : 1;
   for (int sw = 0; sw < max_sweeps; ++sw) {
     for (int r = 0; r < s.rounds; ++r) {
       const int *pr = s.dev + size_t(r) * s.npairs * 2;
@@ -597,17 +653,20 @@ static JacobiResult jacobi(tnh_ctx *ctx, bool two_sided, double *X, int rows, in
         hipLaunchKernelGGL(jac_eig_kernel<false>, dim3(s.npairs), dim3(256), 0, ctx->stream, X, imX,
                            n, partial, nsplit, two_sided, pr, n, tol, abstol, max_inner, Q, D,
                            active, conv + sw);
+      // two-sided: G takes both rotations in one tiled pass, V its columns; one-sided: X and
+      // V take the column rotations
+      double *Xc = two_sided ? nullptr : X;
       if (cplx)
         hipLaunchKernelGGL(jac_colupd_kernel<true>, dim3((rmax + 255) / 256, s.npairs), dim3(256), 0,
-                           ctx->stream, X, rows, imX, V, n, imV, pr, n, Q, active);
+                           ctx->stream, Xc, rows, imX, V, n, imV, pr, n, Q, active);
       else
         hipLaunchKernelGGL(jac_colupd_kernel<false>, dim3((rmax + 255) / 256, s.npairs), dim3(256), 0,
-                           ctx->stream, X, rows, imX, V, n, imV, pr, n, Q, active);
+                           ctx->stream, Xc, rows, imX, V, n, imV, pr, n, Q, active);
       if (two_sided && cplx)
-        hipLaunchKernelGGL(jac_rowupd_kernel<true>, dim3((n + 255) / 256, s.npairs), dim3(256), 0,
+        hipLaunchKernelGGL(jac_twoside_kernel<true>, dim3(s.npairs, s.npairs), dim3(256), 0,
                            ctx->stream, X, n, imX, pr, Q, D, active);
       else if (two_sided)
-        hipLaunchKernelGGL(jac_rowupd_kernel<false>, dim3((n + 255) / 256, s.npairs), dim3(256), 0,
+        hipLaunchKernelGGL(jac_twoside_kernel<false>, dim3(s.npairs, s.npairs), dim3(256), 0,
                            ctx->stream, X, n, imX, pr, Q, D, active);
     }
     TNH_HIP(hipGetLastError());
