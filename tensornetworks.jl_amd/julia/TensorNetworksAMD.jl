@@ -215,6 +215,39 @@ function dmrg_gpu(psi::GMPS, Hs::GMPS...; nsites::Int = 2, krylovdim::Int = 3,
     return psi, cost
 end
 
-export Context, GPUProjMPS, dmrg_gpu
+# ------------------------------------------ <psi|H|psi> and dE/dW (tnh_mpo_expect_grad)
+"""
+    mpo_energy_grad(psi::GMPS, H::GMPS; block=8) -> (E, dW::Vector{Array{ComplexF64,4}})
+
+inner(psi, H, psi) (src/structures/mps/mpo.jl:181-217) and its gradient with respect to
+every MPO tensor: E = sum(H[i] .* dW[i]) at each site. Environments are checkpointed every
+`block` sites on the device.
+"""
+function mpo_energy_grad(psi::GMPS, H::GMPS; block::Int = 8, ctx::Context = default_context())
+    N = length(psi)
+    dpsi = [upload(ctx, psi[i]) for i = 1:N]
+    dW = [upload(ctx, H[i]) for i = 1:N]
+    hp = [t.h for t in dpsi]; hw = [t.h for t in dW]
+    out = fill(Ptr{Cvoid}(C_NULL), N)
+    er = Ref{Cdouble}(0); ei = Ref{Cdouble}(0)
+    GC.@preserve hp hw out check(ccall((:tnh_mpo_expect_grad, libtnhip), Cint,
+            (Ptr{Cvoid}, Cint, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Cint, Ptr{Cdouble}, Ptr{Cdouble},
+             Ptr{Ptr{Cvoid}}),
+            ctx.h, N, hp, hw, block, er, ei, out), ctx.h, "tnh_mpo_expect_grad")
+    grads = [download(DeviceTensor(ctx, out[i])) for i = 1:N]
+    return complex(er[], ei[]), grads
+end
+
+# A ChainRulesCore rule for a coupling-parametrised energy E(c) = inner(psi, H(c), psi), where
+# H(c) = MPO(st, oplist(c)) places each c_t at one MPO entry (site_t, row_t, col_t) times the
+# operator op_t, reads (with ChainRulesCore loaded by the caller):
+#
+#   function ChainRulesCore.rrule(::typeof(energy), c, psi, slots)
+#       E, dW = mpo_energy_grad(psi, build_mpo(c))
+#       g = [sum(dW[s][r, :, :, k] .* op) for (s, r, k, op) in slots]
+#       return real(E), Ē -> (NoTangent(), Ē .* real.(g), NoTangent(), NoTangent())
+#   end
+
+export Context, GPUProjMPS, dmrg_gpu, mpo_energy_grad
 
 end # module
