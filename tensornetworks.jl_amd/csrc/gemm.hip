@@ -7,7 +7,9 @@ namespace tnh {
 namespace gk {
 void launch_f64(hipStream_t, const Params &, int, bool, bool, int, int, int, bool);
 void launch_f32(hipStream_t, const Params &, int, bool, bool, int, int, int, bool);
-void launch_bf16x3(hipStream_t, const Params &, int, bool, bool, int, int, int, bool);
+void launch_bf16p(hipStream_t, const Params &, int, int, bool);
+void split_bf16(hipStream_t, const void *, int, bool, int64_t, int64_t, int, int, int, int,
+                unsigned short *);
 }  // namespace gk
 
 void gemm(tnh_ctx *ctx, const Gemm &g, int prec) {
@@ -68,9 +70,35 @@ void gemm(tnh_ctx *ctx, const Gemm &g, int prec) {
     case TNH_PREC_F32:
       gk::launch_f32(ctx->stream, p, g.batch, g.transA, g.transB, g.tA, g.tB, g.tC, big);
       break;
-    case TNH_PREC_BF16X3:
-      gk::launch_bf16x3(ctx->stream, p, g.batch, g.transA, g.transB, g.tA, g.tB, g.tC, big);
+    case TNH_PREC_BF16X3: {
+      // split both operands once into k-contiguous hi/lo bf16 planes (k padded to 32)
+      const int Kp = (g.K + 31) / 32 * 32;
+      if (Kp == 0) {  // empty sum: C = beta C through the FP32 kernel
+        gk::launch_f32(ctx->stream, p, g.batch, g.transA, g.transB, g.tA, g.tB, g.tC, big);
+        break;
+      }
+      const int64_t pa = int64_t(g.M) * Kp, pb = int64_t(g.N) * Kp;
+      auto *Ap = static_cast<unsigned short *>(
+          ctx->workspace(950, sizeof(unsigned short) * 2 * pa * g.batch));
+      auto *Bp = static_cast<unsigned short *>(
+          ctx->workspace(951, sizeof(unsigned short) * 2 * pb * g.batch));
+      // op(A)[m, k]: rows (m) contiguous unless A is transposed; op(B)[k, n]: k contiguous
+      // unless B is transposed
+      gk::split_bf16(ctx->stream, g.A, g.tA, !g.transA, g.lda, g.sA, g.M, g.K, Kp, g.batch, Ap);
+      gk::split_bf16(ctx->stream, g.B, g.tB, g.transB, g.ldb, g.sB, g.N, g.K, Kp, g.batch, Bp);
+      gk::Params q = p;
+      q.A = Ap;
+      q.B = Bp;
+      q.K = Kp;
+      q.lda = Kp;
+      q.ldb = Kp;
+      q.sA = 2 * pa;
+      q.sB = 2 * pb;
+      q.planeA = pa;
+      q.planeB = pb;
+      gk::launch_bf16p(ctx->stream, q, g.batch, g.tC, big);
       break;
+    }
     default:
       throw Error(TNH_ERR_ARG, "unknown precision " + std::to_string(prec));
   }

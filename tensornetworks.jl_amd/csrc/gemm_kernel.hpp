@@ -5,7 +5,9 @@
 // Three arithmetic modes share one LDS-tiled skeleton:
 //   MODE_F64    : v_mfma_f64_16x16x4f64, FP64 in LDS, FP64 accumulate (BK = 16).
 //   MODE_F32    : v_mfma_f32_16x16x4f32, FP32 in LDS, FP32 accumulate (BK = 32).
-//   MODE_BF16X3 : each FP32 operand is split into hi + lo bf16 when staged to LDS;
+//   MODE_BF16P  : split-BF16x3 on operands split beforehand into hi + lo bf16 planes
+//                 stored k-contiguous ([row][k], k padded to a multiple of 32): tiles are
+//                 streamed with 16-byte loads and LDS stores, and
 //                 v_mfma_f32_16x16x32_bf16 accumulates hi*hi + hi*lo + lo*hi in FP32.
 //
 // Tiling: 256 threads = 4 waves (2 x 2); a block computes BM x BN, a wave
@@ -19,7 +21,7 @@
 //   row-contiguous operand -> [k][row], row stride ROWS + 16 elements: the 16 lanes of a
 //                            lane group read 16 consecutive rows, and the +16 padding puts
 //                            the 4 lane groups on disjoint bank ranges (FP32).
-// (bf16 always uses [row][k], splitting into hi/lo planes as it stages.)
+// (bf16 planes always use [row][k].)
 // Global loads are 16-byte vectors when the host has checked alignment and divisibility
 // (Params::vecA / vecB), else bounds-checked scalars. Tiles are prefetched into registers
 // one k-step ahead and LDS is double buffered: one barrier per k-step.
@@ -37,7 +39,7 @@
 namespace tnh {
 namespace gk {
 
-enum { MODE_F64 = 0, MODE_F32 = 1, MODE_BF16X3 = 2 };
+enum { MODE_F64 = 0, MODE_F32 = 1, MODE_BF16P = 2 };
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef float v2f __attribute__((ext_vector_type(2)));
@@ -58,6 +60,7 @@ struct Params {
   double alpha, beta;
   int tiles_m, tiles_n;
   int vecA, vecB, vecC;
+  int64_t planeA, planeB;  // MODE_BF16P: element offset of the lo plane from the hi plane
 };
 
 template <int MODE>
@@ -75,7 +78,7 @@ struct ModeTraits<MODE_F32> {
   static constexpr int KPAD = 4;
 };
 template <>
-struct ModeTraits<MODE_BF16X3> {
+struct ModeTraits<MODE_BF16P> {
   using S = unsigned short;  // bf16 bits, two planes (hi, lo)
   static constexpr int BK = 32;
   static constexpr int KPAD = 8;
@@ -107,11 +110,10 @@ template <int MODE, typename T, bool KCG, int ROWS>
 struct Tile {
   using S = typename ModeTraits<MODE>::S;
   static constexpr int BK = ModeTraits<MODE>::BK;
-  static constexpr bool LDS_KC = KCG || MODE == MODE_BF16X3;
+  static constexpr bool LDS_KC = KCG;
   static constexpr int LDK = BK + ModeTraits<MODE>::KPAD;  // [row][k] stride
   static constexpr int LDR = ROWS + 16;                    // [k][row] stride
-  static constexpr int PLANES = (MODE == MODE_BF16X3) ? 2 : 1;
-  static constexpr int ELEMS = (LDS_KC ? ROWS * LDK : BK * LDR) * PLANES;
+  static constexpr int ELEMS = LDS_KC ? ROWS * LDK : BK * LDR;
   static constexpr int VW = 16 / sizeof(T);  // elements per 16-byte vector
   static constexpr int NE = ROWS * BK / 256;  // elements per thread
   static constexpr int NV = NE / VW;          // vectors per thread
@@ -171,18 +173,7 @@ struct Tile {
   }
 
   __device__ __forceinline__ void store(S *__restrict__ lds, bool vec) const {
-    if constexpr (MODE == MODE_BF16X3) {
-#pragma unroll
-      for (int e = 0; e < NE; ++e) {
-        int r, k;
-        pos(vec, e, r, k);
-        const float x = float(v[e]);
-        const unsigned short hi = f2bf(x);
-        const unsigned short lo = f2bf(x - bf2f(hi));
-        lds[r * LDK + k] = hi;
-        lds[ROWS * LDK + r * LDK + k] = lo;
-      }
-    } else if (vec) {
+    if (vec) {
       // a vector's elements are contiguous in the LDS layout too
 #pragma unroll
       for (int e = 0; e < NV; ++e) {
@@ -204,6 +195,50 @@ struct Tile {
         pos(false, e, r, k);
         (LDS_KC ? lds[r * LDK + k] : lds[k * LDR + r]) = S(v[e]);
       }
+    }
+  }
+};
+
+// One pre-split operand tile (ROWS x 32 bf16, hi and lo planes) from [row][k] planes with
+// row stride ld (k padded, so every 16-byte vector lies inside a row); rows beyond nrows
+// load the tile base and are zeroed (branch-free).
+template <int ROWS>
+struct TileP {
+  static constexpr int BK = 32;
+  static constexpr int LDK = BK + 8;  // [row][k] stride: 80 bytes keeps 16-byte alignment
+  static constexpr int ELEMS = 2 * ROWS * LDK;
+  static constexpr int NV = ROWS * BK / 8 / 256;  // 16-byte vectors per plane per thread
+  static_assert(NV >= 1, "tile too small");
+  v8us hi[NV], lo[NV];
+
+  __device__ __forceinline__ static void pos(int e, int &r, int &k) {
+    const int vi = threadIdx.x + 256 * e;
+    k = (vi % (BK / 8)) * 8;
+    r = vi / (BK / 8);
+  }
+  __device__ __forceinline__ void load(const unsigned short *__restrict__ base, int64_t ld,
+                                       int64_t plane, int row0, int nrows, int k0) {
+#pragma unroll
+    for (int e = 0; e < NV; ++e) {
+      int r, k;
+      pos(e, r, k);
+      const int gr = row0 + r;
+      const bool in = gr < nrows;
+      const int64_t off = in ? (int64_t(k0 + k) + int64_t(gr) * ld) : 0;
+      const v8us h = *reinterpret_cast<const v8us *>(base + off);
+      const v8us l = *reinterpret_cast<const v8us *>(base + plane + off);
+      const v8us z = {0, 0, 0, 0, 0, 0, 0, 0};
+      hi[e] = in ? h : z;
+      lo[e] = in ? l : z;
+    }
+  }
+  __device__ __forceinline__ void store(unsigned short *__restrict__ lds) const {
+#pragma unroll
+    for (int e = 0; e < NV; ++e) {
+      int r, k;
+      pos(e, r, k);
+      *reinterpret_cast<v8us *>(&lds[r * LDK + k]) = hi[e];
+      *reinterpret_cast<v8us *>(&lds[ROWS * LDK + r * LDK + k]) = lo[e];
     }
   }
 };
@@ -242,8 +277,9 @@ __global__ __launch_bounds__(256) void gemm_kernel(Params p) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
   // A is k-contiguous in memory when transposed; B when not transposed.
-  using TileA = Tile<MODE, TA, TRA, BM>;
-  using TileB = Tile<MODE, TB, !TRB, BN>;
+  constexpr bool PRE = MODE == MODE_BF16P;
+  using TileA = typename std::conditional<PRE, TileP<BM>, Tile<MODE, TA, TRA, BM>>::type;
+  using TileB = typename std::conditional<PRE, TileP<BN>, Tile<MODE, TB, !TRB, BN>>::type;
   constexpr int A_ELEMS = TileA::ELEMS, B_ELEMS = TileB::ELEMS;
 
   __shared__ S lds[2 * (A_ELEMS + B_ELEMS)];
@@ -283,22 +319,37 @@ __global__ __launch_bounds__(256) void gemm_kernel(Params p) {
   TileB tb;
   const int nk = (p.K + BK - 1) / BK;
   if (nk > 0) {
-    ta.load(A, p.lda, m0, p.M, 0, p.K, va);
-    tb.load(B, p.ldb, n0, p.N, 0, p.K, vb);
+    if constexpr (PRE) {
+      ta.load(A, p.lda, p.planeA, m0, p.M, 0);
+      tb.load(B, p.ldb, p.planeB, n0, p.N, 0);
+    } else {
+      ta.load(A, p.lda, m0, p.M, 0, p.K, va);
+      tb.load(B, p.ldb, n0, p.N, 0, p.K, vb);
+    }
   }
 
   for (int kt = 0; kt < nk; ++kt) {
     S *As = lds + (kt & 1) * (A_ELEMS + B_ELEMS);
     S *Bs = As + A_ELEMS;
-    ta.store(As, va);
-    tb.store(Bs, vb);
+    if constexpr (PRE) {
+      ta.store(As);
+      tb.store(Bs);
+    } else {
+      ta.store(As, va);
+      tb.store(Bs, vb);
+    }
     __syncthreads();
     if (kt + 1 < nk) {
-      ta.load(A, p.lda, m0, p.M, (kt + 1) * BK, p.K, va);
-      tb.load(B, p.ldb, n0, p.N, (kt + 1) * BK, p.K, vb);
+      if constexpr (PRE) {
+        ta.load(A, p.lda, p.planeA, m0, p.M, (kt + 1) * BK);
+        tb.load(B, p.ldb, p.planeB, n0, p.N, (kt + 1) * BK);
+      } else {
+        ta.load(A, p.lda, m0, p.M, (kt + 1) * BK, p.K, va);
+        tb.load(B, p.ldb, n0, p.N, (kt + 1) * BK, p.K, vb);
+      }
     }
 
-    if constexpr (MODE == MODE_BF16X3) {
+    if constexpr (MODE == MODE_BF16P) {
       constexpr int LD = TileA::LDK;
       v8us ah[FM], al[FM], bh[FN], bl[FN];
 #pragma unroll
