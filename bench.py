@@ -247,6 +247,8 @@ def main(argv=None):
     ap.add_argument("--dmrg-passes", type=int, default=6)
     ap.add_argument("--dmrg-only", action="store_true",
                     help="run only the N=100 TFIM sweep leg (FP64 and FP32+refinement)")
+    ap.add_argument("--dmrg-small-only", action="store_true",
+                    help="run only the N=32 Heisenberg DMRG leg (GPU, and the CPU port unless --no-cpu)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--svd-only", action="store_true",
                     help="run only the truncated SVD of the 2chi x 2chi theta matrix (1 warm-up + 1 timed)")
@@ -280,6 +282,9 @@ def main(argv=None):
     ctx = tnhip.Context.default()
     if a.dmrg_only:
         print(json.dumps({"dmrg_tfim_N100": dmrg_tfim_n100(a.dmrg_chi, a.dmrg_passes)}))
+        return 0
+    if a.dmrg_small_only:
+        print(json.dumps({"dmrg_heisenberg_N32": dmrg_heisenberg_n32(cpu=not a.no_cpu)}))
         return 0
     Lt, Rt, tht = make_inputs(chi, D, a.seed, torch)
     if a.svd_only:

@@ -790,7 +790,7 @@ int tnh_apply_site_op(tnh_ctx *ctx, tnh_buf **A, const tnh_buf *G) {
         if (gc) wc[d * d + i + d * o] = hg[d * d + o + d * i];
       }
     double *Wc = static_cast<double *>(ctx->workspace(850, wc.size() * sizeof(double)));
-    TNH_HIP(hipMemcpyAsync(Wc, wc.data(), wc.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    to_device(ctx, Wc, wc.data(), wc.size() * sizeof(double));
     tnh_buf *out = nullptr;
     const int odt = is_double(a->dtype) ? (oc ? TNH_C128 : TNH_F64) : (oc ? TNH_C64 : TNH_F32);
     buf_ensure(ctx, &out, odt, 3, a->dims);
@@ -814,7 +814,6 @@ int tnh_apply_site_op(tnh_ctx *ctx, tnh_buf **A, const tnh_buf *G) {
     for (int i = 0; i < d; ++i) s.offI[i] = s.offJ[i] = cl * i;
     try {
       small_contract(ctx, s);
-      TNH_HIP(hipStreamSynchronize(ctx->stream));  // wc lives on the host stack
     } catch (...) {
       buf_delete(ctx, out);
       throw;

@@ -1,6 +1,7 @@
 // Context, device buffers and host<->device transfers of libtnhip.
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
 #include <sstream>
 
 #include "tnh_internal.hpp"
@@ -131,7 +132,12 @@ tnh_ctx::~tnh_ctx() {
   for (auto &kv : ws)
     if (kv.second.first) (void)hipFree(kv.second.first);
   for (auto &kv : cache) (void)hipFree(kv.second);
+  for (auto &kv : schedules)
+    if (kv.second.first) (void)hipFree(kv.second.first);
   if (host_scalars) (void)hipHostFree(host_scalars);
+  if (pin_down) (void)hipHostFree(pin_down);
+  if (pin_up) (void)hipHostFree(pin_up);
+  if (up_done) (void)hipEventDestroy(up_done);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -186,8 +192,47 @@ std::string shape_str(const tnh_buf *b) {
 }
 
 void to_host(tnh_ctx *ctx, void *host, const void *dev, size_t bytes) {
-  TNH_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  if (bytes == 0) return;
+  if (ctx->pin_down_cap < bytes) {
+    if (ctx->pin_down) TNH_HIP(hipHostFree(ctx->pin_down));
+    ctx->pin_down = nullptr;
+    ctx->pin_down_cap = 0;
+    const size_t cap = std::max<size_t>(bytes, size_t(1) << 16);
+    TNH_HIP(hipHostMalloc(&ctx->pin_down, cap, hipHostMallocDefault));
+    ctx->pin_down_cap = cap;
+  }
+  TNH_HIP(hipMemcpyAsync(ctx->pin_down, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
   TNH_HIP(hipStreamSynchronize(ctx->stream));
+  std::memcpy(host, ctx->pin_down, bytes);
+  // every queued upload has completed: the upload arena is free again
+  ctx->pin_up_off = 0;
+  ctx->up_pending = false;
+}
+
+void to_device(tnh_ctx *ctx, void *dev, const void *host, size_t bytes) {
+  if (bytes == 0) return;
+  const size_t need = (bytes + 255) & ~size_t(255);
+  if (ctx->pin_up_off + need > ctx->pin_up_cap) {
+    // wait until the queued uploads have left the arena, then start over (growing it)
+    if (ctx->up_pending) TNH_HIP(hipEventSynchronize(ctx->up_done));
+    ctx->up_pending = false;
+    ctx->pin_up_off = 0;
+    if (ctx->pin_up_cap < need) {
+      if (ctx->pin_up) TNH_HIP(hipHostFree(ctx->pin_up));
+      ctx->pin_up = nullptr;
+      ctx->pin_up_cap = 0;
+      const size_t cap = std::max<size_t>(need, size_t(1) << 20);
+      TNH_HIP(hipHostMalloc(&ctx->pin_up, cap, hipHostMallocDefault));
+      ctx->pin_up_cap = cap;
+    }
+  }
+  char *src = static_cast<char *>(ctx->pin_up) + ctx->pin_up_off;
+  std::memcpy(src, host, bytes);
+  TNH_HIP(hipMemcpyAsync(dev, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  if (!ctx->up_done) TNH_HIP(hipEventCreateWithFlags(&ctx->up_done, hipEventDisableTiming));
+  TNH_HIP(hipEventRecord(ctx->up_done, ctx->stream));
+  ctx->up_pending = true;
+  ctx->pin_up_off += need;
 }
 
 }  // namespace tnh

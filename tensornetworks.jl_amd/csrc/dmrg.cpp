@@ -78,7 +78,7 @@ void copy_in(tnh_ctx *ctx, const tnh_buf *src, bool cplx, tnh_buf **dst) {
   buf_ensure(ctx, dst, dt, src->ndim, src->dims);
   if (src->numel == 0) return;
   if (src->dtype == dt) {
-    TNH_HIP(hipMemcpyAsync((*dst)->ptr, src->ptr, src->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    copy_device(ctx, (*dst)->ptr, src->ptr, src->bytes);
   } else {
     convert(ctx, src->ptr, src->dtype, (*dst)->ptr, dt, src->numel);
   }

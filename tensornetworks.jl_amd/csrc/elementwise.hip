@@ -62,6 +62,13 @@ __global__ void fill_kernel(T *__restrict__ x, int64_t n, T v) {
     x[i] = v;
 }
 
+template <typename T>
+__global__ void copy_kernel(T *__restrict__ dst, const T *__restrict__ src, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+
 __global__ void axpby_kernel(int64_t n, double a, const double *__restrict__ x, double b,
                              double *__restrict__ y) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
@@ -154,6 +161,26 @@ void fill(tnh_ctx *ctx, void *x, int t, int64_t n, double v) {
   else
     hipLaunchKernelGGL(fill_kernel<float>, dim3(g), dim3(kThreads), 0, ctx->stream,
                        static_cast<float *>(x), m, float(v));
+  TNH_HIP(hipGetLastError());
+}
+
+void copy_device(tnh_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (bytes == 0 || dst == src) return;
+  const uintptr_t a = reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src);
+  if ((a & 15u) == 0 && bytes % 16 == 0) {  // 16-byte elements
+    const int64_t n = int64_t(bytes / 16);
+    hipLaunchKernelGGL(copy_kernel<double2>, dim3(grid_for(ctx, n)), dim3(kThreads), 0, ctx->stream,
+                       static_cast<double2 *>(dst), static_cast<const double2 *>(src), n);
+  } else if ((a & 7u) == 0 && bytes % 8 == 0) {
+    const int64_t n = int64_t(bytes / 8);
+    hipLaunchKernelGGL(copy_kernel<double>, dim3(grid_for(ctx, n)), dim3(kThreads), 0, ctx->stream,
+                       static_cast<double *>(dst), static_cast<const double *>(src), n);
+  } else {
+    const int64_t n = int64_t(bytes);
+    hipLaunchKernelGGL(copy_kernel<unsigned char>, dim3(grid_for(ctx, n)), dim3(kThreads), 0,
+                       ctx->stream, static_cast<unsigned char *>(dst),
+                       static_cast<const unsigned char *>(src), n);
+  }
   TNH_HIP(hipGetLastError());
 }
 

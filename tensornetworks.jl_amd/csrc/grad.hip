@@ -93,8 +93,7 @@ tnh_buf *ones_block(tnh_ctx *ctx) {
   int64_t dims[3] = {1, 1, 1};
   tnh_buf *b = buf_new(ctx, TNH_F64, 3, dims);
   const double one = 1.0;
-  TNH_HIP(hipMemcpyAsync(b->ptr, &one, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-  TNH_HIP(hipStreamSynchronize(ctx->stream));
+  to_device(ctx, b->ptr, &one, sizeof(double));
   return b;
 }
 
@@ -113,8 +112,8 @@ void site_grad(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *A, const tnh_buf *
   const bool pc = xc;  // A^H X is complex iff A or X is
   const bool gc = pc || is_complex(R->dtype);
   const int64_t nx = ca * Dl * d * cb, np = d * cb * Dl * d * cb;
-  double *X = static_cast<double *>(ctx->workspace(900, sizeof(double) * nx * (xc ? 2 : 1)));
-  double *P = static_cast<double *>(ctx->workspace(901, sizeof(double) * np * (pc ? 2 : 1)));
+  double *X = static_cast<double *>(ctx->workspace(905, sizeof(double) * nx * (xc ? 2 : 1)));
+  double *P = static_cast<double *>(ctx->workspace(906, sizeof(double) * np * (pc ? 2 : 1)));
   // X[(a,w),(t,b')] = L[(a,w),a'] A[a',(t,b')]
   Gemm g1;
   g1.M = int(ca * Dl);
@@ -138,7 +137,7 @@ void site_grad(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *A, const tnh_buf *
   xo.im = xc ? X + nx : nullptr;
   cgemm(ctx, g2, opnd(A, true), xo, P, pc ? P + np : nullptr, TNH_F64, TNH_PREC_F64);
   const int nout = int(Dl * d * d * Dr);
-  double *part = static_cast<double *>(ctx->workspace(902, sizeof(double) * 2 * nout * cb));
+  double *part = static_cast<double *>(ctx->workspace(907, sizeof(double) * 2 * nout * cb));
   hipLaunchKernelGGL(site_grad_partial_kernel, dim3(unsigned(cb)), dim3(256), 0, ctx->stream, P,
                      pc ? np : 0, static_cast<const double *>(R->ptr), is_complex(R->dtype) ? R->numel : 0,
                      int(d), int(cb), Dl, Dr, part);

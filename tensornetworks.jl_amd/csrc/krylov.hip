@@ -270,8 +270,7 @@ double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, in
   };
 
   // v0 = theta / |theta|
-  TNH_HIP(hipMemcpyAsync(V, theta->ptr, sizeof(double) * vstride, hipMemcpyDeviceToDevice,
-                         ctx->stream));
+  copy_device(ctx, V, theta->ptr, sizeof(double) * vstride);
   {
     double nr = norm_of(V);
     TNH_REQUIRE(std::isfinite(nr) && nr > 0, TNH_ERR_NUMERIC, "eigsolve: initial vector has zero or non-finite norm");
@@ -314,8 +313,7 @@ double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, in
         breakdown = true;
         break;
       }
-      TNH_HIP(hipMemcpyAsync(V + (j + 1) * vstride, Wv, sizeof(double) * vstride,
-                             hipMemcpyDeviceToDevice, ctx->stream));
+      copy_device(ctx, V + (j + 1) * vstride, Wv, sizeof(double) * vstride);
       scale(ctx, V + (j + 1) * vstride, TNH_F64, vstride, 1.0 / beta);
     }
     std::vector<cd> Hm(size_t(m) * m);
@@ -335,9 +333,8 @@ double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, in
                        cplx);
     TNH_HIP(hipGetLastError());
     const double rn = norm_of(Wv);
-    TNH_HIP(hipMemcpyAsync(V + vstride, V + int64_t(m) * vstride, sizeof(double) * vstride,
-                           hipMemcpyDeviceToDevice, ctx->stream));
-    TNH_HIP(hipMemcpyAsync(V, Wv, sizeof(double) * vstride, hipMemcpyDeviceToDevice, ctx->stream));
+    copy_device(ctx, V + vstride, V + int64_t(m) * vstride, sizeof(double) * vstride);
+    copy_device(ctx, V, Wv, sizeof(double) * vstride);
     scale(ctx, V, TNH_F64, vstride, 1.0 / rn);
     std::fill(H.begin(), H.end(), cd(0));
     h(0, 0) = theta_e;

@@ -25,9 +25,7 @@ tnh_buf *op_tensor(tnh_ctx *ctx, int d, const double *interleaved) {
     planar[i] = interleaved[2 * i];
     if (cplx) planar[size_t(d) * d + i] = interleaved[2 * i + 1];
   }
-  TNH_HIP(hipMemcpyAsync(b->ptr, planar.data(), planar.size() * sizeof(double),
-                         hipMemcpyHostToDevice, ctx->stream));
-  TNH_HIP(hipStreamSynchronize(ctx->stream));
+  to_device(ctx, b->ptr, planar.data(), planar.size() * sizeof(double));
   return b;
 }
 

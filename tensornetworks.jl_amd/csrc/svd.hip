@@ -545,6 +545,12 @@ __global__ void gather_cols_kernel(const double *src, int rows, int64_t imS, con
   if (cplx) out[imO + e] = f * src[imS + s];
 }
 
+// X[i, c] *= f[i] for a column-major matrix with `rows` rows (tot elements over all planes)
+__global__ void scale_rows_kernel(double *X, int rows, int64_t tot, const double *f) {
+  const int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (e < tot) X[e] *= f[e % rows];
+}
+
 // out[j, r] = scale_j * conj(src[r, perm[j]])  (k x rows), via a 32 x 32 LDS tile
 __global__ void gather_cols_adjoint_kernel(const double *src, int rows, int64_t imS,
                                            const int *perm, const double *scale, int k,
@@ -596,18 +602,22 @@ static Schedule schedule_for(tnh_ctx *ctx, int cols) {
   if (s.nbp < 2) s.nbp = 2;
   s.rounds = s.nbp - 1;
   s.npairs = s.nbp / 2;
-  std::vector<int> h(size_t(s.rounds) * s.npairs * 2);
-  for (int r = 0; r < s.rounds; ++r)
-    for (int i = 0; i < s.npairs; ++i) {
-      int p, q;
-      circle_pair(s.nbp, r, i, &p, &q);
-      h[(size_t(r) * s.npairs + i) * 2] = p < s.nb ? p : -1;
-      h[(size_t(r) * s.npairs + i) * 2 + 1] = q < s.nb ? q : -1;
-    }
-  s.dev = static_cast<int *>(ctx->workspace(600, sizeof(int) * h.size()));
-  TNH_HIP(hipMemcpyAsync(s.dev, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, ctx->stream));
-  // the host vector dies at return: make the copy complete first
-  TNH_HIP(hipStreamSynchronize(ctx->stream));
+  auto it = ctx->schedules.find(s.nb);
+  if (it == ctx->schedules.end()) {  // built once per block count, kept on the device
+    std::vector<int> h(size_t(s.rounds) * s.npairs * 2);
+    for (int r = 0; r < s.rounds; ++r)
+      for (int i = 0; i < s.npairs; ++i) {
+        int p, q;
+        circle_pair(s.nbp, r, i, &p, &q);
+        h[(size_t(r) * s.npairs + i) * 2] = p < s.nb ? p : -1;
+        h[(size_t(r) * s.npairs + i) * 2 + 1] = q < s.nb ? q : -1;
+      }
+    void *d = nullptr;
+    TNH_HIP(hipMalloc(&d, sizeof(int) * h.size()));
+    to_device(ctx, d, h.data(), sizeof(int) * h.size());
+    it = ctx->schedules.emplace(s.nb, std::make_pair(d, sizeof(int) * h.size())).first;
+  }
+  s.dev = static_cast<int *>(it->second.first);
   return s;
 }
 
@@ -804,19 +814,18 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
     const int rows = side == 0 ? n : m;
     double *X = static_cast<double *>(ctx->workspace(613, sizeof(double) * int64_t(rows) * p * planes));
     if (side == 1) {
-      TNH_HIP(hipMemcpyAsync(X, Mp, sizeof(double) * int64_t(m) * n * planes,
-                             hipMemcpyDeviceToDevice, ctx->stream));
+      copy_device(ctx, X, Mp, sizeof(double) * int64_t(m) * n * planes);
     } else {
       // X = M^H (n x m): X[c, r] = conj(M[r, c]), i.e. the adjoint gather of all n
       // columns of the m-row matrix M
       std::vector<int> id(n);
       std::iota(id.begin(), id.end(), 0);
       int *dperm = static_cast<int *>(ctx->workspace(614, sizeof(int) * n));
-      TNH_HIP(hipMemcpyAsync(dperm, id.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+      to_device(ctx, dperm, id.data(), sizeof(int) * n);
       dim3 grid((m + 31) / 32, (n + 31) / 32);
       hipLaunchKernelGGL(gather_cols_adjoint_kernel, grid, dim3(256), 0, ctx->stream, Mp, m,
                          int64_t(m) * n, dperm, nullptr, n, X, int64_t(n) * m, cplx);
-      TNH_HIP(hipStreamSynchronize(ctx->stream));
+      TNH_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(eye_kernel, dim3(unsigned((pp + 255) / 256)), dim3(256), 0, ctx->stream, V, p,
                        pp, cplx);
@@ -861,7 +870,7 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
   o->k = truncation_rank(o->s, cutoff, maxdim, mindim, &o->discarded);
   const int k = o->k;
   int *dperm = static_cast<int *>(ctx->workspace(615, sizeof(int) * k));
-  TNH_HIP(hipMemcpyAsync(dperm, perm.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx->stream));
+  to_device(ctx, dperm, perm.data(), sizeof(int) * k);
   double *basis = static_cast<double *>(ctx->workspace(620, sizeof(double) * int64_t(p) * k * planes));
   hipLaunchKernelGGL(gather_cols_kernel, dim3(unsigned((int64_t(p) * k + 255) / 256)), dim3(256), 0,
                      ctx->stream, V, p, pp, dperm, nullptr, k, basis, int64_t(p) * k, cplx);
@@ -899,7 +908,6 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
     cgemm(ctx, g2, Ub, Xo, W, cplx ? W + pk : nullptr, TNH_F64, TNH_PREC_F64);
     axpby(ctx, pk * planes, -0.5, W, 1.5, basis);  // U = 1.5 U - 0.5 W
   }
-  TNH_HIP(hipStreamSynchronize(ctx->stream));  // perm lives on the host stack
   return basis;
 }
 
@@ -980,7 +988,7 @@ int split_matrix(tnh_ctx *ctx, const tnh_buf *src, int m, int n, int side, doubl
   Mo.im = cplx ? M + int64_t(m) * n : nullptr;
   if (side == 0) {
     // A1 = U_k (left-orthonormal), A2 = U_k^H M (= S V^H)
-    TNH_HIP(hipMemcpyAsync((*A1)->ptr, basis, (*A1)->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    copy_device(ctx, (*A1)->ptr, basis, (*A1)->bytes);
     Gemm g;
     g.M = k;
     g.N = n;
@@ -999,7 +1007,7 @@ int split_matrix(tnh_ctx *ctx, const tnh_buf *src, int m, int n, int side, doubl
     std::vector<int> id(k);
     std::iota(id.begin(), id.end(), 0);
     int *dperm = static_cast<int *>(ctx->workspace(631, sizeof(int) * k));
-    TNH_HIP(hipMemcpyAsync(dperm, id.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx->stream));
+    to_device(ctx, dperm, id.data(), sizeof(int) * k);
     dim3 grid((n + 31) / 32, (k + 31) / 32);
     hipLaunchKernelGGL(gather_cols_adjoint_kernel, grid, dim3(256), 0, ctx->stream, basis, n,
                        int64_t(n) * k, dperm, nullptr, k, static_cast<double *>((*A2)->ptr),
@@ -1016,7 +1024,6 @@ int split_matrix(tnh_ctx *ctx, const tnh_buf *src, int m, int n, int side, doubl
     Vk.re = basis;
     Vk.im = cplx ? basis + int64_t(n) * k : nullptr;
     cgemm(ctx, g, Mo, Vk, (*A1)->re(), cplx ? (*A1)->im() : nullptr, TNH_F64, TNH_PREC_F64);
-    TNH_HIP(hipStreamSynchronize(ctx->stream));  // id lives on the host stack
   }
   return k;
 }
@@ -1038,11 +1045,11 @@ void svd_trunc(tnh_ctx *ctx, const tnh_buf *Mb, double cutoff, int maxdim, int m
   std::vector<double> sv(o.s.begin(), o.s.begin() + k), inv(k);
   for (int i = 0; i < k; ++i) inv[i] = sv[i] > 0 ? 1.0 / sv[i] : 0.0;
   double *dinv = static_cast<double *>(ctx->workspace(632, sizeof(double) * k));
-  TNH_HIP(hipMemcpyAsync(dinv, inv.data(), sizeof(double) * k, hipMemcpyHostToDevice, ctx->stream));
+  to_device(ctx, dinv, inv.data(), sizeof(double) * k);
   std::vector<int> id(k);
   std::iota(id.begin(), id.end(), 0);
   int *dperm = static_cast<int *>(ctx->workspace(631, sizeof(int) * k));
-  TNH_HIP(hipMemcpyAsync(dperm, id.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx->stream));
+  to_device(ctx, dperm, id.data(), sizeof(int) * k);
   COperand Mo;
   Mo.re = M;
   Mo.im = cplx ? M + int64_t(m) * n : nullptr;
@@ -1052,7 +1059,7 @@ void svd_trunc(tnh_ctx *ctx, const tnh_buf *Mb, double cutoff, int maxdim, int m
     if (U) {
       int64_t dU[2] = {m, k};
       buf_ensure(ctx, U, dt, 2, dU);
-      TNH_HIP(hipMemcpyAsync((*U)->ptr, basis, (*U)->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+      copy_device(ctx, (*U)->ptr, basis, (*U)->bytes);
     }
     if (Vt) {
       int64_t dV[2] = {k, n};
@@ -1070,15 +1077,11 @@ void svd_trunc(tnh_ctx *ctx, const tnh_buf *Mb, double cutoff, int maxdim, int m
       Ub.im = cplx ? basis + int64_t(m) * k : nullptr;
       Ub.conj = true;
       cgemm(ctx, g, Ub, Mo, (*Vt)->re(), cplx ? (*Vt)->im() : nullptr, TNH_F64, TNH_PREC_F64);
-      // row scaling by 1/s: gather columns of the transpose is awkward; scale via a
-      // diagonal GEMM-free pass: Vt[i, :] *= inv[i]
-      std::vector<double> h(size_t(k) * n * (cplx ? 2 : 1));
-      to_host(ctx, h.data(), (*Vt)->ptr, (*Vt)->bytes);
-      for (int pl = 0; pl < (cplx ? 2 : 1); ++pl)
-        for (int c = 0; c < n; ++c)
-          for (int i = 0; i < k; ++i) h[size_t(pl) * k * n + i + size_t(c) * k] *= inv[i];
-      TNH_HIP(hipMemcpyAsync((*Vt)->ptr, h.data(), (*Vt)->bytes, hipMemcpyHostToDevice, ctx->stream));
-      TNH_HIP(hipStreamSynchronize(ctx->stream));
+      // row scaling Vt[i, :] *= 1 / s_i (both planes)
+      const int64_t tot = int64_t(k) * n * (cplx ? 2 : 1);
+      hipLaunchKernelGGL(scale_rows_kernel, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, ctx->stream,
+                         static_cast<double *>((*Vt)->ptr), k, tot, dinv);
+      TNH_HIP(hipGetLastError());
     }
   } else {
     // V = basis (n x k): Vt = basis^H; U = M V diag(1/s)
@@ -1113,9 +1116,8 @@ void svd_trunc(tnh_ctx *ctx, const tnh_buf *Mb, double cutoff, int maxdim, int m
   if (S) {
     int64_t dS[1] = {k};
     buf_ensure(ctx, S, TNH_F64, 1, dS);
-    TNH_HIP(hipMemcpyAsync((*S)->ptr, sv.data(), sizeof(double) * k, hipMemcpyHostToDevice, ctx->stream));
+    to_device(ctx, (*S)->ptr, sv.data(), sizeof(double) * k);
   }
-  TNH_HIP(hipStreamSynchronize(ctx->stream));  // host vectors die at return
   if (k_out) *k_out = k;
   if (disc_out) *disc_out = o.discarded;
   if (fb_out) *fb_out = o.fallback ? 1 : 0;

@@ -109,6 +109,18 @@ struct tnh_ctx {
   // Pinned host staging for small device->host scalar transfers.
   double *host_scalars = nullptr;
   static constexpr int kHostScalars = 4096;
+  // Pinned staging of to_host (downloads) and to_device (uploads). Uploads take
+  // consecutive pieces of an arena whose copies are queued on the stream; the arena is
+  // reset whenever the stream has been synchronised (every to_host), and a full arena
+  // first waits for the last queued upload.
+  void *pin_down = nullptr;
+  size_t pin_down_cap = 0;
+  void *pin_up = nullptr;
+  size_t pin_up_cap = 0, pin_up_off = 0;
+  hipEvent_t up_done = nullptr;
+  bool up_pending = false;
+  // Device copies of the block-Jacobi round-robin schedules, keyed by column count.
+  std::map<int, std::pair<void *, size_t>> schedules;
 
   ~tnh_ctx();
 };
@@ -121,8 +133,11 @@ void buf_delete(tnh_ctx *ctx, tnh_buf *b);
 // Reallocate *b to (dtype, dims) unless it already has that shape and dtype.
 void buf_ensure(tnh_ctx *ctx, tnh_buf **b, int dtype, int ndim, const int64_t *dims);
 std::string shape_str(const tnh_buf *b);
-// Copy n real scalars of type dt from device to host (synchronises the stream).
+// Copy bytes from device to host through pinned staging (synchronises the stream).
 void to_host(tnh_ctx *ctx, void *host, const void *dev, size_t bytes);
+// Queue a copy of bytes from host to device on the stream; the host data is copied into
+// pinned staging before return, so it may be freed at once (no synchronisation needed).
+void to_device(tnh_ctx *ctx, void *dev, const void *host, size_t bytes);
 
 // ------------------------------------------------------------ GEMM (gemm.hip)
 // Column-major C[M,N] = alpha * op(A)[M,K] op(B)[K,N] + beta * C, strided batch.
@@ -172,6 +187,8 @@ void convert(tnh_ctx *ctx, const void *src, int tsrc, void *dst, int tdst, int64
 void scale(tnh_ctx *ctx, void *x, int t, int64_t n, double a);
 void fill(tnh_ctx *ctx, void *x, int t, int64_t n, double v);
 void axpby(tnh_ctx *ctx, int64_t n, double a, const double *x, double b, double *y);
+// Device-to-device copy by a kernel on the context's stream.
+void copy_device(tnh_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 // FP64 reductions with deterministic two-stage sums; result stays on the device
 // (dst) so that chains of Krylov steps need no host round trip.
