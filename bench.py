@@ -193,6 +193,69 @@ def dmrg_tfim_n100(chi, sweeps):
     return res
 
 
+def gradient_tfim_n64(chi=512, block=8):
+    """E(J, h) = <psi|H(J, h)|psi> and its 127 coupling derivatives for the inhomogeneous TFIM,
+    N = 64, J_i ~ U(0.8, 1.2), h_i ~ U(0.5, 1.5) (seed 1), on a random right-canonical MPS of
+    bond dimension chi (seed 0; the timing does not depend on the state), checkpoints every
+    `block` sites. Accuracy: the gradient against inner()'s <O_t> (Hellmann-Feynman form)."""
+    from tnhip import randomMPS, spinhalf
+    from tnhip.grad import coupling_gradient
+    from tnhip.tebd import DeviceMPS, inner
+
+    N = 64
+    r = np.random.default_rng(1)
+    J, h = r.uniform(0.8, 1.2, N - 1), r.uniform(0.5, 1.5, N)
+    from tnhip.models import OpList
+
+    H = OpList(N)
+    for i in range(1, N):
+        H.add(["z", "z"], [i, i + 1], -J[i - 1])
+    for i in range(1, N + 1):
+        H.add("x", i, -h[i - 1])
+    st = spinhalf()
+    psi = DeviceMPS.from_gmps(randomMPS(2, N, chi, rng=np.random.default_rng(0)))
+    coupling_gradient(st, psi, H, block=block)  # warm-up
+    psi.ctx.synchronize()
+    t0 = time.perf_counter()
+    E, g = coupling_gradient(st, psi, H, block=block)
+    el = time.perf_counter() - t0
+    hf = inner(st, psi, H) / np.array(H.coeffs)
+    return {"seconds": el, "energy": float(E.real), "n_couplings": len(H.coeffs), "chi": psi.maxbonddim(),
+            "block": block, "max_abs_diff_vs_inner": float(np.abs(g - hf).max())}
+
+
+def qjmc_tfim_n40(steps=20, chi_max=256):
+    """One quantum-jump trajectory of the dissipative TFIM, N = 40, g = 0.5, jump operators
+    s-_i at rate 0.1, dt = 0.01, second-order Trotter, cutoff 1e-8, chi_max 256, from all
+    spins up, `steps` steps on fixed uniforms (seed 0). Reports trajectory steps per second."""
+    from tnhip import productMPS, spinhalf
+    from tnhip.models import OpList
+    from tnhip.qjmc import qjmc_simulation
+    from tnhip.tebd import DeviceMPS
+
+    N, g, gamma, dt = 40, 0.5, 0.1, 0.01
+    H = OpList(N)
+    for i in range(1, N):
+        H.add(["z", "z"], [i, i + 1], -1.0)
+    for i in range(1, N + 1):
+        H.add("x", i, -g)
+    Jo = OpList(N)
+    for i in range(1, N + 1):
+        Jo.add("s-", i, gamma ** 0.5)
+    st = spinhalf()
+    psi = DeviceMPS.from_gmps(productMPS(st, ["up"] * N))
+    u = iter(np.random.default_rng(0).random(4 * steps + 16))
+    t0 = time.perf_counter()
+    jumps, _ = qjmc_simulation(st, psi, H, Jo, steps * dt, dt, [], cutoff=1e-8, maxdim=chi_max, verbose=False,
+                               rand=lambda: float(next(u)))
+    psi.ctx.synchronize()
+    el = time.perf_counter() - t0
+    return {"seconds": el, "steps": steps, "trajectory_steps_per_s": steps / el, "jumps": len(jumps),
+            "maxbonddim": psi.maxbonddim(),
+            "note": "one trajectory at a time; 512 trajectories x 200 steps would take "
+                    f"{512 * 200 * el / steps / 3600:.1f} h at this rate (early-time bond dims)"}
+
+
 def bond_step(ctx, ops, Tensor, dL, dR, dW, dx, prec, chi, torch):
     """One two-site DMRG bond step at chi on the synthetic workload: Lanczos (Krylov
     dimension 3, 2 restarts, as dmrg.jl's defaults), the truncated split to chi, and the
@@ -247,6 +310,8 @@ def main(argv=None):
     ap.add_argument("--dmrg-passes", type=int, default=6)
     ap.add_argument("--dmrg-only", action="store_true",
                     help="run only the N=100 TFIM sweep leg (FP64 and FP32+refinement)")
+    ap.add_argument("--no-extra-legs", action="store_true", help="skip the gradient and QJMC legs")
+    ap.add_argument("--legs-only", action="store_true", help="run only the gradient and QJMC legs")
     ap.add_argument("--dmrg-small-only", action="store_true",
                     help="run only the N=32 Heisenberg DMRG leg (GPU, and the CPU port unless --no-cpu)")
     ap.add_argument("--seed", type=int, default=0)
@@ -285,6 +350,9 @@ def main(argv=None):
         return 0
     if a.dmrg_small_only:
         print(json.dumps({"dmrg_heisenberg_N32": dmrg_heisenberg_n32(cpu=not a.no_cpu)}))
+        return 0
+    if a.legs_only:
+        print(json.dumps({"gradient_tfim_N64": gradient_tfim_n64(), "qjmc_tfim_N40": qjmc_tfim_n40()}))
         return 0
     Lt, Rt, tht = make_inputs(chi, D, a.seed, torch)
     if a.svd_only:
@@ -370,6 +438,9 @@ def main(argv=None):
         extra["dmrg_bond_step_chi%d" % chi] = steps_
         extra["dmrg_tfim_N100"] = dmrg_tfim_n100(a.dmrg_chi, a.dmrg_passes)
         extra["dmrg_heisenberg_N32"] = dmrg_heisenberg_n32(cpu=not a.no_cpu)
+    if not a.no_extra_legs:
+        extra["gradient_tfim_N64"] = gradient_tfim_n64()
+        extra["qjmc_tfim_N40"] = qjmc_tfim_n40()
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline()

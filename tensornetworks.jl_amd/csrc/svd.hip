@@ -138,7 +138,7 @@ __global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__
   const int pair = blockIdx.x, t = threadIdx.x;
   __shared__ double gr[JP][JP + 1], gi[CPLX ? JP : 1][JP + 1], qr[JP][JP + 1], qi[CPLX ? JP : 1][JP + 1];
   __shared__ int colmap[JP];
-  __shared__ int rp[NS], rq[NS], rrot[NS];
+  __shared__ int rrot[NS];
   __shared__ double rcs[NS], rsn[NS], rer[NS], rei[NS], ra[NS], rc[NS], rtb[NS];
   __shared__ int anyrot[kMaxInnerCap];
   if (t < JP) colmap[t] = pair_col(pairs + 2 * pair, t, cols);
@@ -196,16 +196,15 @@ __global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__
         const bool rot = bb > abstol && bb > tol * sqrt(fabs(a * c));
         double cs = 1, sn = 0, tt = 0, er = 1, ei = 0;
         if (rot) {
-          const double tau = (c - a) / (2 * bb);
+          const double ib = 1.0 / bb;
+          const double tau = 0.5 * (c - a) * ib;
           tt = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1 + tau * tau));
-          cs = 1 / sqrt(1 + tt * tt);
+          cs = rsqrt(1 + tt * tt);
           sn = tt * cs;
-          er = br / bb;  // e^{-i phi} = conj(b) / |b|
-          ei = -bi / bb;
+          er = br * ib;  // e^{-i phi} = conj(b) / |b|
+          ei = -bi * ib;
           anyrot[sw] = 1;
         }
-        rp[t] = p;
-        rq[t] = q;
         rrot[t] = rot;
         rcs[t] = cs;
         rsn[t] = sn;
@@ -221,7 +220,9 @@ __global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__
         const int I = task % NS, J = task / NS;
         const bool rI = rrot[I], rJ = rrot[J];
         if (!rI && !rJ) continue;
-        const int pI = rp[I], qI = rq[I], pJ = rp[J], qJ = rq[J];
+        int pI, qI, pJ, qJ;  // the circle pairing, recomputed instead of read from LDS
+        circle_pair(JP, step, I, &pI, &qI);
+        circle_pair(JP, step, J, &pJ, &qJ);
         if (I == J) {  // rotated diagonal block: exact values
           gr[pI][qI] = gr[qI][pI] = 0.0;
           gr[pI][pI] = ra[I] - rtb[I];
@@ -281,7 +282,8 @@ __global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__
       for (int task = t; task < JP * NS; task += 256) {
         const int r = task % JP, J = task / JP;
         if (!rrot[J]) continue;
-        const int p = rp[J], q = rq[J];
+        int p, q;
+        circle_pair(JP, step, J, &p, &q);
         const double cs = rcs[J], sn = rsn[J], er = rer[J], ei = rei[J];
         const double pr_ = qr[r][p], qr_ = qr[r][q];
         const double pi_ = CPLX ? qi[r][p] : 0.0, qi_ = CPLX ? qi[r][q] : 0.0;
