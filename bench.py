@@ -224,13 +224,16 @@ def gradient_tfim_n64(chi=512, block=8):
             "block": block, "max_abs_diff_vs_inner": float(np.abs(g - hf).max())}
 
 
-def qjmc_tfim_n40(steps=20, chi_max=256):
-    """One quantum-jump trajectory of the dissipative TFIM, N = 40, g = 0.5, jump operators
+def qjmc_tfim_n40(steps=20, ntraj=64, nworkers=8, chi_max=256):
+    """Quantum-jump trajectories of the dissipative TFIM, N = 40, g = 0.5, jump operators
     s-_i at rate 0.1, dt = 0.01, second-order Trotter, cutoff 1e-8, chi_max 256, from all
-    spins up, `steps` steps on fixed uniforms (seed 0). Reports trajectory steps per second."""
+    spins up, on fixed uniforms (seed 0): `ntraj` trajectories of `steps` steps through the
+    native runner (nworkers threads, one stream each), <sz_i> saved every step; and one
+    trajectory through the Python host loop for comparison. Reports trajectory steps per
+    second (a bounded sample of the 512 x 200 workload: early times, small bond dims)."""
     from tnhip import productMPS, spinhalf
     from tnhip.models import OpList
-    from tnhip.qjmc import qjmc_simulation
+    from tnhip.qjmc import qjmc_simulation, qjmc_trajectories
     from tnhip.tebd import DeviceMPS
 
     N, g, gamma, dt = 40, 0.5, 0.1, 0.01
@@ -242,18 +245,34 @@ def qjmc_tfim_n40(steps=20, chi_max=256):
     Jo = OpList(N)
     for i in range(1, N + 1):
         Jo.add("s-", i, gamma ** 0.5)
+    zs = OpList(N)
+    for i in range(1, N + 1):
+        zs.add("z", i)
     st = spinhalf()
+    U = np.random.default_rng(0).random((ntraj, 4 * steps + 16))
+    # host loop, one trajectory
     psi = DeviceMPS.from_gmps(productMPS(st, ["up"] * N))
-    u = iter(np.random.default_rng(0).random(4 * steps + 16))
+    u = iter(U[0])
     t0 = time.perf_counter()
     jumps, _ = qjmc_simulation(st, psi, H, Jo, steps * dt, dt, [], cutoff=1e-8, maxdim=chi_max, verbose=False,
                                rand=lambda: float(next(u)))
     psi.ctx.synchronize()
+    host_el = time.perf_counter() - t0
+    # native runner, ntraj trajectories
+    qjmc_trajectories(st, ["up"] * N, H, Jo, 2 * dt, dt, U[:nworkers], observables=zs, cutoff=1e-8,
+                      maxdim=chi_max, nworkers=nworkers)  # warm-up: worker contexts, code paths
+    t0 = time.perf_counter()
+    out = qjmc_trajectories(st, ["up"] * N, H, Jo, steps * dt, dt, U, observables=zs, cutoff=1e-8,
+                            maxdim=chi_max, nworkers=nworkers)
     el = time.perf_counter() - t0
-    return {"seconds": el, "steps": steps, "trajectory_steps_per_s": steps / el, "jumps": len(jumps),
-            "maxbonddim": psi.maxbonddim(),
-            "note": "one trajectory at a time; 512 trajectories x 200 steps would take "
-                    f"{512 * 200 * el / steps / 3600:.1f} h at this rate (early-time bond dims)"}
+    rate = ntraj * steps / el
+    return {"native": {"seconds": el, "trajectories": ntraj, "steps": steps, "workers": nworkers,
+                       "trajectory_steps_per_s": rate, "jumps": int(sum(out["njumps"])),
+                       "maxbonddim": int(max(out["maxbonddim"])),
+                       "sz_avg_final": float(out["measurements"][:, -1, :].real.mean())},
+            "host_loop": {"seconds": host_el, "steps": steps, "trajectory_steps_per_s": steps / host_el,
+                          "jumps": len(jumps)},
+            "projected_512x200_hours": 512 * 200 / rate / 3600}
 
 
 def bond_step(ctx, ops, Tensor, dL, dR, dW, dx, prec, chi, torch):

@@ -227,6 +227,47 @@ int tnh_inner_oplist(tnh_ctx *ctx, int N, const tnh_buf *const *psi, int nterms,
 int tnh_mpo_expect_grad(tnh_ctx *ctx, int N, tnh_buf *const *psi, tnh_buf *const *W, int block,
                         double *energy_re, double *energy_im, tnh_buf **dW);
 
+/* ------------------------------------------------ quantum-jump trajectories */
+
+/* One QJMC problem: the Trotter gates of qjmc_gates (src/algorithms/mps/qjmc.jl:9-26, i.e.
+ * trotterize of -iH - sum c^2 L^H L / 2), single-site jump operators, one-site observables
+ * and the initial product state. Matrices are column-major, interleaved complex. */
+typedef struct tnh_qjmc_problem {
+  int N, d;                     /* sites, physical dimension */
+  int nlayers;                  /* gate layers of the GateList, applied in order */
+  const int *layer_count;       /* gates per layer */
+  const int *gate_site;         /* per gate (all layers in order): first site, 1-based */
+  const int *gate_range;        /* per gate: 1 (d x d) or 2 (G[o1,i1,o2,i2], d^4) */
+  const int64_t *gate_offset;   /* per gate: offset into gate_data in complex numbers */
+  const double *gate_data;
+  int njump;                    /* jump terms c_t L_t on site jump_site[t] */
+  const int *jump_site;
+  const double *jump_coeff;     /* c_t (the rate is c_t^2 <L_t^H L_t>) */
+  const double *jump_op;        /* d x d per term */
+  int nobs;                     /* observables <O_o> on site obs_site[o], measured at t = 0 */
+  const int *obs_site;          /* and every `savesteps` steps (QJMCOperators) */
+  const double *obs_op;         /* d x d per observable */
+  const double *init_state;     /* d complex amplitudes per site */
+  double dt;
+  int nsteps, savesteps;
+  double cutoff;
+  int maxdim, mindim;
+} tnh_qjmc_problem;
+
+/* Run ntraj trajectories of qjmc_simulation (qjmc.jl:28-167, classical branch) with the
+ * step loop on the host side of this library and every tensor operation on the GPU:
+ * applygates!, normalise, emission rates, jump test exp(-sum(rates) dt) and channel
+ * choice, jump by applyop! and the centre sweeps of :103-107. Trajectory t consumes its
+ * uniforms uniforms[t * nuniform ...] in the reference's order: per step one discarded
+ * draw (:64), the jump test (:97), and the channel (:99) when a jump happens. nworkers
+ * host threads each own a context (one HIP stream) on ctx's device and take trajectories
+ * in turn. Outputs: meas[t][save][obs] (re, im) with nsteps / savesteps + 1 saves,
+ * njumps[t], the first maxjumps jump indices (1-based) and steps per trajectory, and the
+ * final maximum bond dimension maxbond[t]. */
+int tnh_qjmc_run(tnh_ctx *ctx, const tnh_qjmc_problem *problem, int ntraj, const double *uniforms,
+                 int64_t nuniform, int nworkers, double *meas, int *njumps, int *jumps,
+                 int *jump_steps, int maxjumps, int *maxbond);
+
 /* ---------------------------------------------------- device-resident DMRG */
 
 typedef struct tnh_dmrg_params {

@@ -51,6 +51,33 @@ class DmrgParams(C.Structure):
     ]
 
 
+class QjmcProblem(C.Structure):
+    _fields_ = [
+        ("N", C.c_int),
+        ("d", C.c_int),
+        ("nlayers", C.c_int),
+        ("layer_count", C.POINTER(C.c_int)),
+        ("gate_site", C.POINTER(C.c_int)),
+        ("gate_range", C.POINTER(C.c_int)),
+        ("gate_offset", C.POINTER(C.c_int64)),
+        ("gate_data", C.POINTER(C.c_double)),
+        ("njump", C.c_int),
+        ("jump_site", C.POINTER(C.c_int)),
+        ("jump_coeff", C.POINTER(C.c_double)),
+        ("jump_op", C.POINTER(C.c_double)),
+        ("nobs", C.c_int),
+        ("obs_site", C.POINTER(C.c_int)),
+        ("obs_op", C.POINTER(C.c_double)),
+        ("init_state", C.POINTER(C.c_double)),
+        ("dt", C.c_double),
+        ("nsteps", C.c_int),
+        ("savesteps", C.c_int),
+        ("cutoff", C.c_double),
+        ("maxdim", C.c_int),
+        ("mindim", C.c_int),
+    ]
+
+
 # name -> (restype, argtypes); every entry point of include/tnhip.h
 SIGNATURES = {
     "tnh_version": (C.c_char_p, []),
@@ -84,7 +111,8 @@ SIGNATURES = {
     "tnh_mps_movecenter": (C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]),
     "tnh_applygate2": (C.c_int, [ctx_p, C.POINTER(buf_p), C.POINTER(buf_p), buf_p, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "tnh_inner_oplist": (C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
-    "tnh_mpo_expect_grad": (C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.POINTER(buf_p), C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(buf_p)]),
+    "tnh_qjmc_run": (C.c_int, [ctx_p, C.POINTER(QjmcProblem), C.c_int, C.POINTER(C.c_double), C.c_int64, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
+    "tnh_mpo_expect_grad":(C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.POINTER(buf_p), C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(buf_p)]),
     "tnh_dmrg_create":(C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.c_int, C.POINTER(buf_p), C.POINTER(C.c_double), C.POINTER(C.c_void_p)]),
     "tnh_dmrg_destroy": (C.c_int, [C.c_void_p]),
     "tnh_dmrg_sweep": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(DmrgParams), C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_double)]),

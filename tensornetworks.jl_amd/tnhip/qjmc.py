@@ -10,6 +10,7 @@ that a trajectory is reproducible from its uniforms.
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 
 import numpy as np
@@ -112,6 +113,100 @@ def qjmc_simulation(st: Sitetypes, psi, H: OpList, jumpops: OpList, tmax, dt, ob
         g = dpsi.to_gmps()
         host.tensors, host.center = g.tensors, g.center
     return jumps, jumptimes
+
+
+def _cplx_flat(m):
+    """Column-major interleaved complex doubles of a matrix or tensor."""
+    a = np.asarray(m, dtype=np.complex128).reshape(-1, order="F")
+    return np.ascontiguousarray(a).view(np.float64)
+
+
+def qjmc_trajectories(st: Sitetypes, init_names, H: OpList, jumpops: OpList, tmax, dt, uniforms,
+                      observables: OpList | None = None, save=0.0, cutoff=1e-12, mindim=1, maxdim=0,
+                      order=2, nworkers=4, maxjumps=256, ctx=None):
+    """Many trajectories of qjmc_simulation (qjmc.jl:28-167, classical branch) run natively by
+    tnh_qjmc_run: the step loop in C++, every tensor operation on the GPU, `nworkers` threads
+    each with its own stream. All trajectories start from productMPS(st, init_names);
+    trajectory t consumes uniforms[t] in the reference's rand() order. Jump operators must
+    be single-site (multi-site ones run through qjmc_simulation). Returns a dict with
+    `jumps` and `jumptimes` (lists per trajectory, 1-based indices, as the reference stores
+    them), `measurements` (ntraj x nsave x nobs, coefficient * <O>), `times`, `maxbonddim`."""
+    from ._lib import QjmcProblem
+    from .core import Context
+
+    ctx = ctx or Context.default()
+    N, d = jumpops.length, st.dim
+    if len(init_names) != N or H.length != N:
+        raise ValueError("OpList and MPS have different lengths.")
+    for sites in jumpops.sites:
+        if len(sites) != 1:
+            raise NotImplementedError("multi-site jump operators: use qjmc_simulation")
+    obs = observables if observables is not None else OpList(N)
+    for sites in obs.sites:
+        if len(sites) != 1:
+            raise NotImplementedError("multi-site observables: use qjmc_simulation with QJMCOperators")
+    _, gates = qjmc_gates(st, H, jumpops, dt, order=order)
+    layer_count = [len(s) for s in gates.sites]
+    gsite, grange, goff, gdata, off = [], [], [], [], 0
+    for sites, gts in zip(gates.sites, gates.gates):
+        for s, g in zip(sites, gts):
+            gsite.append(int(s))
+            grange.append(g.ndim // 2)
+            goff.append(off)
+            flat = _cplx_flat(g)
+            gdata.append(flat)
+            off += flat.size // 2
+    gdata = np.concatenate(gdata) if gdata else np.zeros(2)
+    jsite = [s[0] for s in jumpops.sites]
+    jcoeff = np.array([float(np.real(c)) for c in jumpops.coeffs])
+    if any(abs(np.imag(c)) > 0 for c in jumpops.coeffs):
+        raise ValueError("jump coefficients must be real")
+    jop = np.concatenate([_cplx_flat(st.op(o[0])) for o in jumpops.ops]) if jumpops.ops else np.zeros(2)
+    osite = [s[0] for s in obs.sites]
+    oop = np.concatenate([_cplx_flat(st.op(o[0])) for o in obs.ops]) if obs.ops else np.zeros(2)
+    init = np.concatenate([_cplx_flat(st.state(n)) for n in init_names])
+    save = dt if save == 0 else save
+    nsteps = int(math.ceil(round(tmax / dt, 5)))
+    savesteps = int(math.ceil(round(save / dt)))
+    U = np.ascontiguousarray(np.atleast_2d(np.asarray(uniforms, dtype=np.float64)))
+    ntraj, nu = U.shape
+
+    def iarr(x):
+        return (C.c_int * max(1, len(x)))(*x)
+
+    keep = [np.ascontiguousarray(a) for a in (gdata, jcoeff, jop, oop, init)]
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    p = QjmcProblem()
+    p.N, p.d, p.nlayers = N, d, len(layer_count)
+    p.layer_count, p.gate_site, p.gate_range = iarr(layer_count), iarr(gsite), iarr(grange)
+    p.gate_offset = (C.c_int64 * max(1, len(goff)))(*goff)
+    p.gate_data = dp(keep[0])
+    p.njump, p.jump_site, p.jump_coeff, p.jump_op = len(jsite), iarr(jsite), dp(keep[1]), dp(keep[2])
+    p.nobs, p.obs_site, p.obs_op = len(osite), iarr(osite), dp(keep[3])
+    p.init_state = dp(keep[4])
+    p.dt, p.nsteps, p.savesteps = float(dt), nsteps, savesteps
+    p.cutoff, p.maxdim, p.mindim = float(cutoff), int(maxdim), int(mindim)
+    nsave = nsteps // savesteps + 1
+    meas = np.zeros((ntraj, nsave, max(1, len(osite)), 2))
+    njumps = np.zeros(ntraj, dtype=np.int32)
+    jumps = np.zeros((ntraj, maxjumps), dtype=np.int32)
+    jsteps = np.zeros((ntraj, maxjumps), dtype=np.int32)
+    maxb = np.zeros(ntraj, dtype=np.int32)
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))  # noqa: E731
+    ctx.call("tnh_qjmc_run", C.byref(p), ntraj, dp(U), nu, int(nworkers), dp(meas), ip(njumps), ip(jumps),
+             ip(jsteps), int(maxjumps), ip(maxb))
+    coeffs = np.array(obs.coeffs, dtype=complex) if obs.coeffs else np.zeros(0)
+    vals = meas[..., 0] + 1j * meas[..., 1]
+    vals = vals[:, :, : len(osite)] * coeffs[None, None, :]
+    nj = njumps.tolist()
+    return {
+        "jumps": [jumps[t, : min(nj[t], maxjumps)].tolist() for t in range(ntraj)],
+        "jumptimes": [(jsteps[t, : min(nj[t], maxjumps)] * dt).tolist() for t in range(ntraj)],
+        "njumps": nj,
+        "measurements": vals,
+        "times": np.arange(nsave) * savesteps * dt,
+        "maxbonddim": maxb.tolist(),
+    }
 
 
 class QJMCOperators:

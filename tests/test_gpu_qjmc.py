@@ -64,6 +64,40 @@ def test_qjmc_trajectories_match_dense(ctx, classical):
         assert np.allclose(got, meas[m].real, atol=1e-8)
 
 
+def test_native_runner_matches_dense(ctx):
+    """tnh_qjmc_run (C++ step loop, two worker threads with their own streams) against the
+    dense oracle on the same uniforms: identical jump records, <sz_i> to 1e-8."""
+    from tnhip import spinhalf
+    from tnhip.qjmc import qjmc_trajectories
+    from oracle.qjmc import qjmc_dense
+
+    N, g, gamma, dt, tmax = 6, 0.5, 0.5, 0.05, 1.0
+    H, J = model(N, g, gamma)
+    U = np.random.default_rng(12).random((5, 200))
+    v0 = np.zeros(2 ** N, complex)
+    v0[0] = 1.0
+    jr, tr, times, meas = qjmc_dense(v0, terms(H), terms(J), N, dt, tmax, U, save=0.25,
+                                     obs=[("z", i) for i in range(1, N + 1)])
+    assert sum(len(j) for j in jr) >= 2
+    out = qjmc_trajectories(spinhalf(), ["up"] * N, H, J, tmax, dt, U, observables=zlist(N), save=0.25,
+                            cutoff=0.0, nworkers=2)
+    assert out["jumps"] == jr
+    for m in range(U.shape[0]):
+        assert np.allclose(out["jumptimes"][m], tr[m], atol=1e-12)
+    assert np.allclose(out["times"], times)
+    assert np.allclose(out["measurements"].real, meas.real, atol=1e-8)
+
+
+def test_native_runner_rejects_short_uniforms(ctx):
+    from tnhip import spinhalf, TnhError
+    from tnhip.qjmc import qjmc_trajectories
+
+    N = 4
+    H, J = model(N, 0.5, 0.5)
+    with pytest.raises(TnhError):
+        qjmc_trajectories(spinhalf(), ["up"] * N, H, J, 1.0, 0.05, np.full((1, 3), 0.5), cutoff=0.0)
+
+
 def test_emission_rates_match_dense(ctx, rng):
     from tnhip import spinhalf
     from tnhip.qjmc import qjmc_emission_rates
