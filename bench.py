@@ -161,10 +161,13 @@ def dmrg_heisenberg_n32(cpu=True):
     return out
 
 
-def dmrg_tfim_n100(chi, sweeps):
+def dmrg_tfim_n100(chi, sweeps, refine_tol=None, precisions=("f64", "f32")):
     """Critical TFIM N = 100 (g = 1), random chi = 16 start (seed 0), cutoff 1e-12: an
-    all-FP64 run and an FP32-MFMA run with FP64 refinement passes, the same number of
-    passes each. Reports per-pass times and the energy errors per site."""
+    all-FP64 run and lower-precision runs (FP32-MFMA, split-BF16x3) with FP64 refinement
+    passes, the same number of passes each. The lower precision switches to FP64 when the
+    relative energy change of a pass falls below refine_tol (None: the dmrg default for the
+    precision; 0: only for the last two passes). Reports per-pass times and energies and
+    the energy errors per site."""
     from tnhip import MPO, randomMPS, spinhalf
     from tnhip.mps import dmrg
 
@@ -172,24 +175,29 @@ def dmrg_tfim_n100(chi, sweeps):
     e_exact = 1.0 - 1.0 / np.sin(np.pi / (4 * N + 2))  # open critical TFIM closed form
     M = MPO(spinhalf(), tfim_oplist(N))
     res = {}
-    for prec in ("f64", "f32"):
+    for prec in precisions:
         psi = randomMPS(2, N, 16, rng=np.random.default_rng(0))
         hist = []
         t0 = time.perf_counter()
         psi, e = dmrg(psi, M, maxsweeps=sweeps, minsweeps=sweeps, tol=0.0, cutoff=1e-12, maxdim=chi,
-                      verbose=False, precision=prec, refine_sweeps=2, refine_tol=0.0, history=hist)
+                      verbose=False, precision=prec, refine_sweeps=2, refine_tol=refine_tol, history=hist)
         res[prec] = {"seconds": time.perf_counter() - t0, "energy": e,
                      "abs_dE_per_site_vs_exact": abs(e - e_exact) / N,
                      "pass_seconds": [round(h["seconds"], 4) for h in hist],
+                     "pass_energy": [h["energy"] for h in hist],
                      "pass_precision": [h["precision"] for h in hist],
                      "maxbonddim": hist[-1]["maxbonddim"]}
-    res["abs_dE_per_site_mixed_vs_f64"] = abs(res["f32"]["energy"] - res["f64"]["energy"]) / N
-    lo = [s for s, p in zip(res["f32"]["pass_seconds"], res["f32"]["pass_precision"]) if p == "f32"]
-    hi = res["f64"]["pass_seconds"][: len(lo)]
-    if lo:
-        res["f32_pass_speedup_vs_f64"] = sum(hi) / sum(lo)
+    for prec in precisions:
+        if prec == "f64":
+            continue
+        res[f"abs_dE_per_site_{prec}_vs_f64"] = abs(res[prec]["energy"] - res["f64"]["energy"]) / N
+        lo = [s for s, p in zip(res[prec]["pass_seconds"], res[prec]["pass_precision"]) if p == prec]
+        hi = res["f64"]["pass_seconds"][: len(lo)]
+        if lo:
+            res[f"{prec}_pass_speedup_vs_f64"] = sum(hi) / sum(lo)
+        res[f"{prec}_run_speedup_vs_f64"] = res["f64"]["seconds"] / res[prec]["seconds"]
     res["config"] = {"N": N, "g": 1.0, "chi_max": chi, "passes": sweeps, "cutoff": 1e-12, "krylovdim": 3,
-                     "E0_exact": e_exact}
+                     "refine_tol": refine_tol, "E0_exact": e_exact}
     return res
 
 
@@ -325,8 +333,10 @@ def main(argv=None):
     ap.add_argument("--no-svd", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dmrg", action="store_true", help="skip the DMRG legs")
-    ap.add_argument("--dmrg-chi", type=int, default=128, help="chi_max of the N=100 TFIM sweep leg")
-    ap.add_argument("--dmrg-passes", type=int, default=6)
+    ap.add_argument("--dmrg-chi", type=int, default=2048, help="chi_max of the N=100 TFIM sweep leg")
+    ap.add_argument("--dmrg-passes", type=int, default=24)
+    ap.add_argument("--refine-tol", type=float, default=None,
+                    help="relative energy change that switches a lower-precision DMRG run to FP64")
     ap.add_argument("--dmrg-only", action="store_true",
                     help="run only the N=100 TFIM sweep leg (FP64 and FP32+refinement)")
     ap.add_argument("--no-extra-legs", action="store_true", help="skip the gradient and QJMC legs")
@@ -365,7 +375,8 @@ def main(argv=None):
     chi, D, d = a.chi, 5, 2
     ctx = tnhip.Context.default()
     if a.dmrg_only:
-        print(json.dumps({"dmrg_tfim_N100": dmrg_tfim_n100(a.dmrg_chi, a.dmrg_passes)}))
+        print(json.dumps({"dmrg_tfim_N100": dmrg_tfim_n100(a.dmrg_chi, a.dmrg_passes, a.refine_tol,
+                                                           ("f64", "f32", "bf16x3"))}))
         return 0
     if a.dmrg_small_only:
         print(json.dumps({"dmrg_heisenberg_N32": dmrg_heisenberg_n32(cpu=not a.no_cpu)}))
@@ -455,7 +466,7 @@ def main(argv=None):
         steps_["lanczos_speedup_f32_vs_f64"] = steps_["f64"]["lanczos_s"] / steps_["f32"]["lanczos_s"]
         steps_["half_sweep_estimate_s_N100"] = {p: 99 * steps_[p]["bond_step_s"] for p in ("f64", "f32")}
         extra["dmrg_bond_step_chi%d" % chi] = steps_
-        extra["dmrg_tfim_N100"] = dmrg_tfim_n100(a.dmrg_chi, a.dmrg_passes)
+        extra["dmrg_tfim_N100"] = dmrg_tfim_n100(a.dmrg_chi, a.dmrg_passes, a.refine_tol, ("f64", "f32", "bf16x3"))
         extra["dmrg_heisenberg_N32"] = dmrg_heisenberg_n32(cpu=not a.no_cpu)
     if not a.no_extra_legs:
         extra["gradient_tfim_N64"] = gradient_tfim_n64()
