@@ -114,77 +114,24 @@ __global__ __launch_bounds__(256) void jac_gram_kernel(const double *__restrict_
   }
 }
 
-// Diagonalise the 32 x 32 Hermitian sub-problem of each pair. Two-sided mode reads
-// G[P,P] from the global matrix; one-sided mode sums the partial Grams. Writes the
-// accumulated unitary Q (and the diagonalised block D in two-sided mode), an
-// `active` flag (0 = pair already converged, nothing to apply) and the pair's
-// off-diagonal measure into conv (atomic max over the sweep).
-//
-// Inner step: the 16 rotations of one circle-method round act on disjoint index pairs,
-// so G <- J^H G J splits into 16 x 16 independent 2 x 2 blocks {p_I, q_I} x {p_J, q_J};
-// each thread updates one block in place (columns by J_J, then rows by J_I^H), and the
-// rotated diagonal blocks take their exact values. One barrier after the rotation
-// parameters, one after the update. CPLX = false skips the imaginary planes.
+// Cyclic parallel Jacobi on the JP x JP Hermitian matrix (gr, gi) held in LDS, accumulating
+// the rotations into (qr, qi): at most `sweeps` sweeps, stopping after a sweep that rotated
+// nothing (the return value says whether that happened). Every thread of the 256-thread
+// block calls it. One step applies the 16 rotations of a circle-method round: they act on
+// disjoint index pairs, so G <- J^H G J splits into 16 x 16 independent 2 x 2 blocks
+// {p_I, q_I} x {p_J, q_J}; each thread updates one block in place (columns by J_J, then rows
+// by J_I^H), and the rotated diagonal blocks take their exact values. One barrier after
+// the rotation parameters, one after the update. CPLX = false skips the imaginary planes.
 template <bool CPLX>
-__global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__ G, int64_t imG,
-                                                      int n, const double *__restrict__ partial,
-                                                      int nsplit, bool two_sided,
-                                                      const int *__restrict__ pairs, int cols,
-                                                      double tol, double abstol, int max_inner,
-                                                      double *__restrict__ Qout,
-                                                      double *__restrict__ Dout,
-                                                      int *__restrict__ active,
-                                                      unsigned long long *__restrict__ conv) {
-  const int pair = blockIdx.x, t = threadIdx.x;
-  __shared__ double gr[JP][JP + 1], gi[CPLX ? JP : 1][JP + 1], qr[JP][JP + 1], qi[CPLX ? JP : 1][JP + 1];
-  __shared__ int colmap[JP];
+__device__ bool jacobi_lds(double (*gr)[JP + 1], double (*gi)[JP + 1], double (*qr)[JP + 1],
+                           double (*qi)[JP + 1], double tol, double abstol, int sweeps) {
+  const int t = threadIdx.x;
   __shared__ int rrot[NS];
   __shared__ double rcs[NS], rsn[NS], rer[NS], rei[NS], ra[NS], rc[NS], rtb[NS];
   __shared__ int anyrot[kMaxInnerCap];
-  if (t < JP) colmap[t] = pair_col(pairs + 2 * pair, t, cols);
   if (t < kMaxInnerCap) anyrot[t] = 0;
   __syncthreads();
-  for (int e = t; e < JP * JP; e += 256) {
-    const int i = e % JP, j = e / JP;
-    double vr = 0, vi = 0;
-    if (two_sided) {
-      const int ci = colmap[i], cj = colmap[j];
-      if (ci >= 0 && cj >= 0) {
-        vr = G[ci + int64_t(cj) * n];
-        if (CPLX) vi = G[imG + ci + int64_t(cj) * n];
-      }
-    } else {
-      for (int s = 0; s < nsplit; ++s) {
-        const double *pp = partial + int64_t(pair * nsplit + s) * 2 * JP * JP;
-        vr += pp[e];
-        if (CPLX) vi += pp[JP * JP + e];
-      }
-    }
-    gr[i][j] = vr;
-    qr[i][j] = (i == j) ? 1.0 : 0.0;
-    if (CPLX) {
-      gi[i][j] = (i == j) ? 0.0 : vi;
-      qi[i][j] = 0.0;
-    }
-  }
-  __syncthreads();
-  // off-diagonal measure: max |g_ij| / sqrt(|g_ii g_jj|) over i != j above the noise floor
-  double m = 0;
-  for (int e = t; e < JP * JP; e += 256) {
-    const int i = e % JP, j = e / JP;
-    if (i == j) continue;
-    const double b = CPLX ? hypot(gr[i][j], gi[i][j]) : fabs(gr[i][j]);
-    if (b <= abstol) continue;
-    const double s = sqrt(fabs(gr[i][i] * gr[j][j]));
-    m = fmax(m, s > 0 ? b / s : 1.0);
-  }
-  m = block_max(m);
-  if (t == 0) atomicMax(conv, (unsigned long long)__double_as_longlong(m));
-  if (m <= tol) {
-    if (t == 0) active[pair] = 0;
-    return;
-  }
-  const int sweeps = max_inner < kMaxInnerCap ? max_inner : kMaxInnerCap;
+  if (sweeps > kMaxInnerCap) sweeps = kMaxInnerCap;
   for (int sw = 0; sw < sweeps; ++sw) {
     for (int step = 0; step < JP - 1; ++step) {
       if (t < NS) {
@@ -298,8 +245,72 @@ __global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__
       __syncthreads();
     }
     // anyrot[sw] is complete after the step's final barrier and never rewritten
-    if (!anyrot[sw]) break;
+    if (!anyrot[sw]) return true;
   }
+  return false;
+}
+
+// Diagonalise the 32 x 32 Hermitian sub-problem of each pair. Two-sided mode reads
+// G[P,P] from the global matrix; one-sided mode sums the partial Grams. Writes the
+// accumulated unitary Q (and the diagonalised block D in two-sided mode), an
+// `active` flag (0 = pair already converged, nothing to apply) and the pair's
+// off-diagonal measure into conv (atomic max over the sweep).
+template <bool CPLX>
+__global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__ G, int64_t imG,
+                                                      int n, const double *__restrict__ partial,
+                                                      int nsplit, bool two_sided,
+                                                      const int *__restrict__ pairs, int cols,
+                                                      double tol, double abstol, int max_inner,
+                                                      double *__restrict__ Qout,
+                                                      double *__restrict__ Dout,
+                                                      int *__restrict__ active,
+                                                      unsigned long long *__restrict__ conv) {
+  const int pair = blockIdx.x, t = threadIdx.x;
+  __shared__ double gr[JP][JP + 1], gi[CPLX ? JP : 1][JP + 1], qr[JP][JP + 1], qi[CPLX ? JP : 1][JP + 1];
+  __shared__ int colmap[JP];
+  if (t < JP) colmap[t] = pair_col(pairs + 2 * pair, t, cols);
+  __syncthreads();
+  for (int e = t; e < JP * JP; e += 256) {
+    const int i = e % JP, j = e / JP;
+    double vr = 0, vi = 0;
+    if (two_sided) {
+      const int ci = colmap[i], cj = colmap[j];
+      if (ci >= 0 && cj >= 0) {
+        vr = G[ci + int64_t(cj) * n];
+        if (CPLX) vi = G[imG + ci + int64_t(cj) * n];
+      }
+    } else {
+      for (int s = 0; s < nsplit; ++s) {
+        const double *pp = partial + int64_t(pair * nsplit + s) * 2 * JP * JP;
+        vr += pp[e];
+        if (CPLX) vi += pp[JP * JP + e];
+      }
+    }
+    gr[i][j] = vr;
+    qr[i][j] = (i == j) ? 1.0 : 0.0;
+    if (CPLX) {
+      gi[i][j] = (i == j) ? 0.0 : vi;
+      qi[i][j] = 0.0;
+    }
+  }
+  __syncthreads();
+  // off-diagonal measure: max |g_ij| / sqrt(|g_ii g_jj|) over i != j above the noise floor
+  double m = 0;
+  for (int e = t; e < JP * JP; e += 256) {
+    const int i = e % JP, j = e / JP;
+    if (i == j) continue;
+    const double b = CPLX ? hypot(gr[i][j], gi[i][j]) : fabs(gr[i][j]);
+    if (b <= abstol) continue;
+    const double s = sqrt(fabs(gr[i][i] * gr[j][j]));
+    m = fmax(m, s > 0 ? b / s : 1.0);
+  }
+  m = block_max(m);
+  if (t == 0) atomicMax(conv, (unsigned long long)__double_as_longlong(m));
+  if (m <= tol) {
+    if (t == 0) active[pair] = 0;
+    return;
+  }
+  (void)jacobi_lds<CPLX>(gr, gi, qr, qi, tol, abstol, max_inner);
   double *Q = Qout + int64_t(pair) * 2 * JP * JP;
   for (int e = t; e < JP * JP; e += 256) {
     const int i = e % JP, j = e / JP;
@@ -582,6 +593,44 @@ __global__ void gather_cols_adjoint_kernel(const double *src, int rows, int64_t 
   }
 }
 
+// The whole eigenproblem of a Hermitian p x p Gram matrix (p <= JP) in one workgroup: G is
+// made exactly Hermitian in LDS, the noise floor 4 eps sqrt(p) max_i G_ii is taken from its
+// diagonal, the cyclic Jacobi runs until a sweep rotates nothing, and V (p x p, planar), the
+// eigenvalues out[0..p) and the converged flag out[p] (1 or 0) are written.
+template <bool CPLX>
+__global__ __launch_bounds__(256) void tiny_eig_kernel(const double *__restrict__ G, int p, int64_t imG,
+                                                       double tol, double *__restrict__ V, int64_t imV,
+                                                       double *__restrict__ out) {
+  __shared__ double gr[JP][JP + 1], gi[CPLX ? JP : 1][JP + 1], qr[JP][JP + 1], qi[CPLX ? JP : 1][JP + 1];
+  const int t = threadIdx.x;
+  for (int e = t; e < JP * JP; e += 256) {
+    const int i = e % JP, j = e / JP;
+    double vr = 0, vi = 0;
+    if (i < p && j < p) {
+      vr = 0.5 * (G[i + int64_t(j) * p] + G[j + int64_t(i) * p]);
+      if (CPLX) vi = (i == j) ? 0.0 : 0.5 * (G[imG + i + int64_t(j) * p] - G[imG + j + int64_t(i) * p]);
+    }
+    gr[i][j] = vr;
+    qr[i][j] = (i == j) ? 1.0 : 0.0;
+    if (CPLX) {
+      gi[i][j] = vi;
+      qi[i][j] = 0.0;
+    }
+  }
+  __syncthreads();
+  double dm = t < JP ? fabs(gr[t][t]) : 0.0;
+  dm = block_max(dm);
+  const double abstol = 4.0 * 2.220446049250313e-16 * sqrt(double(p)) * dm;
+  const bool conv = jacobi_lds<CPLX>(gr, gi, qr, qi, tol, abstol, kMaxInnerCap);
+  for (int e = t; e < p * p; e += 256) {
+    const int i = e % p, j = e / p;
+    V[i + int64_t(j) * p] = qr[i][j];
+    if (CPLX) V[imV + i + int64_t(j) * p] = qi[i][j];
+  }
+  if (t < p) out[t] = gr[t][t];
+  if (t == 0) out[p] = conv ? 1.0 : 0.0;
+}
+
 struct Schedule {
   int nb = 0, nbp = 0, rounds = 0, npairs = 0;
   int *dev = nullptr;  // [round][pair][2]
@@ -756,6 +805,9 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
   double *V = static_cast<double *>(ctx->workspace(611, sizeof(double) * pp * planes));
   std::vector<double> val(p);
   bool ok = false;
+  // relative off-diagonal tolerance 4 sqrt(p) eps (at least 1e-14), the scale of the
+  // rounding that p-term inner products leave
+  const double tol = std::max(1e-14, 4.0 * 2.220446049250313e-16 * std::sqrt(double(p)));
   if (!force_fallback) {
     // G = M M^H (side 0) or M^H M (side 1)
     Gemm g;
@@ -777,6 +829,28 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
       A.conj = true;
     }
     cgemm(ctx, g, A, B, G, cplx ? G + pp : nullptr, TNH_F64, TNH_PREC_F64);
+    if (p <= JP) {
+      // the whole problem fits one workgroup: a single launch and a single download
+      double *dout = static_cast<double *>(ctx->workspace(616, sizeof(double) * (p + 1)));
+      if (cplx)
+        hipLaunchKernelGGL(tiny_eig_kernel<true>, dim3(1), dim3(256), 0, ctx->stream, G, p, pp, tol, V,
+                           pp, dout);
+      else
+        hipLaunchKernelGGL(tiny_eig_kernel<false>, dim3(1), dim3(256), 0, ctx->stream, G, p, pp, tol, V,
+                           pp, dout);
+      TNH_HIP(hipGetLastError());
+      std::vector<double> h(p + 1);
+      to_host(ctx, h.data(), dout, sizeof(double) * (p + 1));
+      if (h[p] == 1.0) {
+        ok = true;
+        for (int i = 0; i < p; ++i) {
+          if (!std::isfinite(h[i])) ok = false;
+          val[i] = std::sqrt(std::max(h[i], 0.0));
+        }
+      }
+    }
+  }
+  if (!force_fallback && !ok && p > JP) {
     hipLaunchKernelGGL(hermitize_kernel, dim3(unsigned((pp + 255) / 256)), dim3(256), 0, ctx->stream,
                        G, p, pp, cplx);
     // Entries below 4 eps sqrt(p) max_i G_ii are rounding noise: each rotation of a
@@ -793,9 +867,6 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
     hipLaunchKernelGGL(eye_kernel, dim3(unsigned((pp + 255) / 256)), dim3(256), 0, ctx->stream, V, p,
                        pp, cplx);
     const double abstol = 4.0 * 2.220446049250313e-16 * std::sqrt(double(p)) * dmax;
-    // relative off-diagonal tolerance 4 sqrt(p) eps (at least 1e-14), the scale of the
-    // rounding that p-term inner products leave
-    const double tol = std::max(1e-14, 4.0 * 2.220446049250313e-16 * std::sqrt(double(p)));
     JacobiResult jr = jacobi(ctx, true, G, p, p, cplx, V, tol, abstol, 40);
     if (jr.converged) {
       double *dv = static_cast<double *>(ctx->workspace(612, sizeof(double) * p));
