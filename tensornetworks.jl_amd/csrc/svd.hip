@@ -122,9 +122,22 @@ __global__ __launch_bounds__(256) void jac_gram_kernel(const double *__restrict_
 // {p_I, q_I} x {p_J, q_J}; each thread updates one block in place (columns by J_J, then rows
 // by J_I^H), and the rotated diagonal blocks take their exact values. One barrier after
 // the rotation parameters, one after the update. CPLX = false skips the imaginary planes.
+// With `cross` set, a sweep rotates only the JB x JB index pairs that couple the pair's two
+// column blocks (local indices < JB against >= JB): JB steps of the disjoint rotations
+// (i, JB + (i + step) mod JB) instead of the JP - 1 steps of the full cyclic order.
+__device__ inline void step_pair(bool cross, int step, int i, int *p, int *q) {
+  if (cross) {
+    *p = i;
+    *q = JB + ((i + step) % JB);
+  } else {
+    circle_pair(JP, step, i, p, q);
+  }
+}
+
 template <bool CPLX>
 __device__ bool jacobi_lds(double (*gr)[JP + 1], double (*gi)[JP + 1], double (*qr)[JP + 1],
-                           double (*qi)[JP + 1], double tol, double abstol, int sweeps) {
+                           double (*qi)[JP + 1], double tol, double abstol, int sweeps,
+                           bool cross = false) {
   const int t = threadIdx.x;
   __shared__ int rrot[NS];
   __shared__ double rcs[NS], rsn[NS], rer[NS], rei[NS], ra[NS], rc[NS], rtb[NS];
@@ -133,10 +146,11 @@ __device__ bool jacobi_lds(double (*gr)[JP + 1], double (*gi)[JP + 1], double (*
   __syncthreads();
   if (sweeps > kMaxInnerCap) sweeps = kMaxInnerCap;
   for (int sw = 0; sw < sweeps; ++sw) {
-    for (int step = 0; step < JP - 1; ++step) {
+    const int nsteps = cross ? JB : JP - 1;
+    for (int step = 0; step < nsteps; ++step) {
       if (t < NS) {
         int p, q;
-        circle_pair(JP, step, t, &p, &q);
+        step_pair(cross, step, t, &p, &q);
         const double a = gr[p][p], c = gr[q][q];
         const double br = gr[p][q], bi = CPLX ? gi[p][q] : 0.0;
         const double bb = CPLX ? hypot(br, bi) : fabs(br);
@@ -168,8 +182,8 @@ __device__ bool jacobi_lds(double (*gr)[JP + 1], double (*gi)[JP + 1], double (*
         const bool rI = rrot[I], rJ = rrot[J];
         if (!rI && !rJ) continue;
         int pI, qI, pJ, qJ;  // the circle pairing, recomputed instead of read from LDS
-        circle_pair(JP, step, I, &pI, &qI);
-        circle_pair(JP, step, J, &pJ, &qJ);
+        step_pair(cross, step, I, &pI, &qI);
+        step_pair(cross, step, J, &pJ, &qJ);
         if (I == J) {  // rotated diagonal block: exact values
           gr[pI][qI] = gr[qI][pI] = 0.0;
           gr[pI][pI] = ra[I] - rtb[I];
@@ -230,7 +244,7 @@ __device__ bool jacobi_lds(double (*gr)[JP + 1], double (*gi)[JP + 1], double (*
         const int r = task % JP, J = task / JP;
         if (!rrot[J]) continue;
         int p, q;
-        circle_pair(JP, step, J, &p, &q);
+        step_pair(cross, step, J, &p, &q);
         const double cs = rcs[J], sn = rsn[J], er = rer[J], ei = rei[J];
         const double pr_ = qr[r][p], qr_ = qr[r][q];
         const double pi_ = CPLX ? qi[r][p] : 0.0, qi_ = CPLX ? qi[r][q] : 0.0;
@@ -261,7 +275,7 @@ __global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__
                                                       int nsplit, bool two_sided,
                                                       const int *__restrict__ pairs, int cols,
                                                       double tol, double abstol, int max_inner,
-                                                      double *__restrict__ Qout,
+                                                      bool cross, double *__restrict__ Qout,
                                                       double *__restrict__ Dout,
                                                       int *__restrict__ active,
                                                       unsigned long long *__restrict__ conv) {
@@ -310,7 +324,7 @@ __global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__
     if (t == 0) active[pair] = 0;
     return;
   }
-  (void)jacobi_lds<CPLX>(gr, gi, qr, qi, tol, abstol, max_inner);
+  (void)jacobi_lds<CPLX>(gr, gi, qr, qi, tol, abstol, max_inner, cross);
   double *Q = Qout + int64_t(pair) * 2 * JP * JP;
   for (int e = t; e < JP * JP; e += 256) {
     const int i = e % JP, j = e / JP;
@@ -492,6 +506,85 @@ __global__ __launch_bounds__(256) void jac_twoside_kernel(double *__restrict__ G
     if (r < 0 || c < 0) continue;
     G[r + int64_t(c) * n] = sr[q];
     if (CPLX) G[imG + r + int64_t(c) * n] = si[q];
+  }
+}
+
+typedef double jv4d __attribute__((ext_vector_type(4)));
+
+// Real two-sided mode on the matrix cores, exploiting the symmetry of G: one workgroup per
+// tile of the upper triangle of pair tiles (P <= K, blockIdx.x = K (K + 1) / 2 + P) forms
+// Q_P^T G[P, K] Q_K with v_mfma_f64_16x16x4f64 (one 16 x 16 quarter per wave, the product
+// staged through LDS) and writes it and its transpose G[K, P], so G stays bitwise symmetric
+// and half the tiles are read. The Q of an inactive pair is the identity (exact in FP64).
+// v_mfma_f64_16x16x4f64 lane (g = lane / 16, rr = lane % 16) supplies A[rr][k_g] and
+// B[k_g][rr]; accumulator register r holds C[g + 4 r][rr].
+__global__ __launch_bounds__(256) void jac_twoside_sym_kernel(double *__restrict__ G, int n,
+                                                              const int *__restrict__ pairs,
+                                                              const double *__restrict__ Qin,
+                                                              const double *__restrict__ Din,
+                                                              const int *__restrict__ active) {
+  const int idx = blockIdx.x, t = threadIdx.x;
+  int K = int((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+  while ((K + 1) * (K + 2) / 2 <= idx) ++K;
+  while (K * (K + 1) / 2 > idx) --K;
+  const int P = idx - K * (K + 1) / 2;
+  const bool aP = active[P] != 0, aK = active[K] != 0;
+  if (!aP && !aK) return;
+  __shared__ double T[JP][JP + 1], QP[JP][JP + 1], QK[JP][JP + 1];
+  __shared__ int rowmap[JP], colmap[JP];
+  if (t < JP) {
+    rowmap[t] = pair_col(pairs + 2 * P, t, n);
+    colmap[t] = pair_col(pairs + 2 * K, t, n);
+  }
+  __syncthreads();
+  if (P == K) {  // aP holds here
+    const double *D = Din + int64_t(P) * 2 * JP * JP;
+    for (int e = t; e < JP * JP; e += 256) {
+      const int i = e % JP, j = e / JP;
+      const int r = rowmap[i], c = colmap[j];
+      if (r >= 0 && c >= 0) G[r + int64_t(c) * n] = D[e];
+    }
+    return;
+  }
+  const double *qp = Qin + int64_t(P) * 2 * JP * JP;
+  const double *qk = Qin + int64_t(K) * 2 * JP * JP;
+  for (int e = t; e < JP * JP; e += 256) {
+    const int i = e % JP, j = e / JP;
+    const int r = rowmap[i], c = colmap[j];
+    T[i][j] = (r >= 0 && c >= 0) ? G[r + int64_t(c) * n] : 0.0;
+    QP[i][j] = aP ? qp[e] : (i == j ? 1.0 : 0.0);
+    QK[i][j] = aK ? qk[e] : (i == j ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  const int lane = t & 63, w = t >> 6, g = lane >> 4, rr = lane & 15;
+  const int i0 = 16 * (w & 1), j0 = 16 * (w >> 1);
+  jv4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < JP / 4; ++s)  // U = T Q_K
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(T[i0 + rr][4 * s + g], QK[4 * s + g][j0 + rr], acc, 0, 0, 0);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) T[i0 + g + 4 * r][j0 + rr] = acc[r];
+  __syncthreads();
+  acc = jv4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < JP / 4; ++s)  // Q_P^T U
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(QP[4 * s + g][i0 + rr], T[4 * s + g][j0 + rr], acc, 0, 0, 0);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) T[i0 + g + 4 * r][j0 + rr] = acc[r];
+  __syncthreads();
+  for (int e = t; e < JP * JP; e += 256) {
+    {  // G[P, K], rows fastest
+      const int i = e % JP, j = e / JP;
+      const int r = rowmap[i], c = colmap[j];
+      if (r >= 0 && c >= 0) G[r + int64_t(c) * n] = T[i][j];
+    }
+    {  // G[K, P] = G[P, K]^T, its rows (the columns of the tile) fastest
+      const int j = e % JP, i = e / JP;
+      const int r = rowmap[i], c = colmap[j];
+      if (r >= 0 && c >= 0) G[c + int64_t(r) * n] = T[i][j];
+    }
   }
 }
 
@@ -703,16 +796,19 @@ static JacobiResult jacobi(tnh_ctx *ctx, bool two_sided, double *X, int rows, in
   for (int sw = 0; sw < max_sweeps; ++sw) {
     for (int r = 0; r < s.rounds; ++r) {
       const int *pr = s.dev + size_t(r) * s.npairs * 2;
+      // the first round of a sweep visits every block once: it also rotates the index pairs
+      // inside the blocks; the other rounds only the pairs across the two blocks of a pair
+      const bool cross = s.nb > 2 && r > 0;
       if (!two_sided)
         hipLaunchKernelGGL(jac_gram_kernel, dim3(s.npairs, nsplit), dim3(256), 0, ctx->stream, X,
                            rows, imX, n, cplx, pr, nsplit, partial);
       if (cplx)
         hipLaunchKernelGGL(jac_eig_kernel<true>, dim3(s.npairs), dim3(256), 0, ctx->stream, X, imX,
-                           n, partial, nsplit, two_sided, pr, n, tol, abstol, max_inner, Q, D,
+                           n, partial, nsplit, two_sided, pr, n, tol, abstol, max_inner, cross, Q, D,
                            active, conv + sw);
       else
         hipLaunchKernelGGL(jac_eig_kernel<false>, dim3(s.npairs), dim3(256), 0, ctx->stream, X, imX,
-                           n, partial, nsplit, two_sided, pr, n, tol, abstol, max_inner, Q, D,
+                           n, partial, nsplit, two_sided, pr, n, tol, abstol, max_inner, cross, Q, D,
                            active, conv + sw);
       // two-sided: G takes both rotations in one tiled pass, V its columns; one-sided: X and
       // V take the column rotations
@@ -727,8 +823,8 @@ static JacobiResult jacobi(tnh_ctx *ctx, bool two_sided, double *X, int rows, in
         hipLaunchKernelGGL(jac_twoside_kernel<true>, dim3(s.npairs, s.npairs), dim3(256), 0,
                            ctx->stream, X, n, imX, pr, Q, D, active);
       else if (two_sided)
-        hipLaunchKernelGGL(jac_twoside_kernel<false>, dim3(s.npairs, s.npairs), dim3(256), 0,
-                           ctx->stream, X, n, imX, pr, Q, D, active);
+        hipLaunchKernelGGL(jac_twoside_sym_kernel, dim3(s.npairs * (s.npairs + 1) / 2), dim3(256), 0,
+                           ctx->stream, X, n, pr, Q, D, active);
     }
     TNH_HIP(hipGetLastError());
     unsigned long long bits = 0;
