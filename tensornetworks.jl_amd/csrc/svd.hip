@@ -44,18 +44,22 @@ __host__ __device__ inline void circle_pair(int n, int step, int i, int *p, int 
   }
 }
 
+template <int B = JB>
 __device__ inline int pair_col(const int *pr, int k, int cols) {
-  const int blk = k < JB ? pr[0] : pr[1];
-  const int c = blk * JB + (k % JB);
+  const int blk = k < B ? pr[0] : pr[1];
+  const int c = blk * B + (k % B);
   return (blk >= 0 && c < cols) ? c : -1;
 }
 
+template <int NT = 256>
 __device__ inline double block_max(double v) {
-  __shared__ double red[4];
+  __shared__ double red[NT / 64];
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
-  double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  double m = red[0];
+#pragma unroll
+  for (int w = 1; w < NT / 64; ++w) m = fmax(m, red[w]);
   __syncthreads();
   return m;
 }
@@ -125,19 +129,22 @@ __global__ __launch_bounds__(256) void jac_gram_kernel(const double *__restrict_
 // With `cross` set, a sweep rotates only the JB x JB index pairs that couple the pair's two
 // column blocks (local indices < JB against >= JB): JB steps of the disjoint rotations
 // (i, JB + (i + step) mod JB) instead of the JP - 1 steps of the full cyclic order.
+template <int B>
 __device__ inline void step_pair(bool cross, int step, int i, int *p, int *q) {
   if (cross) {
     *p = i;
-    *q = JB + ((i + step) % JB);
+    *q = B + ((i + step) % B);
   } else {
-    circle_pair(JP, step, i, p, q);
+    circle_pair(2 * B, step, i, p, q);
   }
 }
 
-template <bool CPLX>
-__device__ bool jacobi_lds(double (*gr)[JP + 1], double (*gi)[JP + 1], double (*qr)[JP + 1],
-                           double (*qi)[JP + 1], double tol, double abstol, int sweeps,
+template <bool CPLX, int B, int NT>
+__device__ bool jacobi_lds(double (*gr)[2 * B + 1], double (*gi)[2 * B + 1], double (*qr)[2 * B + 1],
+                           double (*qi)[2 * B + 1], double tol, double abstol, int sweeps,
                            bool cross = false) {
+  constexpr int JP = 2 * B, NS = B;
+  static_assert(NT % NS == 0 && NT / NS <= NS, "thread layout of the step tasks");
   const int t = threadIdx.x;
   __shared__ int rrot[NS];
   __shared__ double rcs[NS], rsn[NS], rer[NS], rei[NS], ra[NS], rc[NS], rtb[NS];
@@ -146,11 +153,11 @@ __device__ bool jacobi_lds(double (*gr)[JP + 1], double (*gi)[JP + 1], double (*
   __syncthreads();
   if (sweeps > kMaxInnerCap) sweeps = kMaxInnerCap;
   for (int sw = 0; sw < sweeps; ++sw) {
-    const int nsteps = cross ? JB : JP - 1;
+    const int nsteps = cross ? B : JP - 1;
     for (int step = 0; step < nsteps; ++step) {
       if (t < NS) {
         int p, q;
-        step_pair(cross, step, t, &p, &q);
+        step_pair<B>(cross, step, t, &p, &q);
         const double a = gr[p][p], c = gr[q][q];
         const double br = gr[p][q], bi = CPLX ? gi[p][q] : 0.0;
         const double bb = CPLX ? hypot(br, bi) : fabs(br);
@@ -176,84 +183,87 @@ __device__ bool jacobi_lds(double (*gr)[JP + 1], double (*gi)[JP + 1], double (*
         rtb[t] = tt * bb;
       }
       __syncthreads();
-      // G: one 2 x 2 block (I rows, J columns) per thread
-      for (int task = t; task < NS * NS; task += 256) {
-        const int I = task % NS, J = task / NS;
-        const bool rI = rrot[I], rJ = rrot[J];
-        if (!rI && !rJ) continue;
-        int pI, qI, pJ, qJ;  // the circle pairing, recomputed instead of read from LDS
-        step_pair(cross, step, I, &pI, &qI);
-        step_pair(cross, step, J, &pJ, &qJ);
-        if (I == J) {  // rotated diagonal block: exact values
-          gr[pI][qI] = gr[qI][pI] = 0.0;
-          gr[pI][pI] = ra[I] - rtb[I];
-          gr[qI][qI] = rc[I] + rtb[I];
-          if (CPLX) gi[pI][qI] = gi[qI][pI] = gi[pI][pI] = gi[qI][qI] = 0.0;
-          continue;
+      {
+        // Each thread keeps one rotation I = t % NS in registers: it updates the 2 x 2 blocks
+        // (I, J) of G for J = t / NS + k NT / NS (rows by J_I^H, columns by J_J) and the rows
+        // t / NS + k NT / NS of the column pair of I in Q.
+        const int I = t % NS;
+        int pI, qI;  // the pairing, recomputed instead of read from LDS
+        step_pair<B>(cross, step, I, &pI, &qI);
+        const bool rI = rrot[I];
+        const double csI = rcs[I], snI = rsn[I], erI = rer[I], eiI = rei[I];
+        for (int J = t / NS; J < NS; J += NT / NS) {
+          const bool rJ = rrot[J];
+          if (!rI && !rJ) continue;
+          int pJ, qJ;
+          step_pair<B>(cross, step, J, &pJ, &qJ);
+          if (I == J) {  // rotated diagonal block: exact values
+            gr[pI][qI] = gr[qI][pI] = 0.0;
+            gr[pI][pI] = ra[I] - rtb[I];
+            gr[qI][qI] = rc[I] + rtb[I];
+            if (CPLX) gi[pI][qI] = gi[qI][pI] = gi[pI][pI] = gi[qI][qI] = 0.0;
+            continue;
+          }
+          // x = [[g(pI,pJ), g(pI,qJ)], [g(qI,pJ), g(qI,qJ)]]
+          double x00r = gr[pI][pJ], x01r = gr[pI][qJ], x10r = gr[qI][pJ], x11r = gr[qI][qJ];
+          double x00i = 0, x01i = 0, x10i = 0, x11i = 0;
+          if (CPLX) {
+            x00i = gi[pI][pJ];
+            x01i = gi[pI][qJ];
+            x10i = gi[qI][pJ];
+            x11i = gi[qI][qJ];
+          }
+          if (rJ) {  // columns: [x_p x_q] <- [x_p x_q] [[cs, sn], [-sn e, cs e]]
+            const double cs = rcs[J], sn = rsn[J], er = rer[J], ei = rei[J];
+            double eqr = er * x01r - ei * x01i, eqi = er * x01i + ei * x01r;
+            double npr = cs * x00r - sn * eqr, npi = cs * x00i - sn * eqi;
+            double nqr = sn * x00r + cs * eqr, nqi = sn * x00i + cs * eqi;
+            x00r = npr; x00i = npi; x01r = nqr; x01i = nqi;
+            eqr = er * x11r - ei * x11i;
+            eqi = er * x11i + ei * x11r;
+            npr = cs * x10r - sn * eqr;
+            npi = cs * x10i - sn * eqi;
+            nqr = sn * x10r + cs * eqr;
+            nqi = sn * x10i + cs * eqi;
+            x10r = npr; x10i = npi; x11r = nqr; x11i = nqi;
+          }
+          if (rI) {  // rows: [x_p; x_q] <- J^H [x_p; x_q]  (conj(e))
+            const double cs = csI, sn = snI, er = erI, ei = -eiI;
+            double eqr = er * x10r - ei * x10i, eqi = er * x10i + ei * x10r;
+            double npr = cs * x00r - sn * eqr, npi = cs * x00i - sn * eqi;
+            double nqr = sn * x00r + cs * eqr, nqi = sn * x00i + cs * eqi;
+            x00r = npr; x00i = npi; x10r = nqr; x10i = nqi;
+            eqr = er * x11r - ei * x11i;
+            eqi = er * x11i + ei * x11r;
+            npr = cs * x01r - sn * eqr;
+            npi = cs * x01i - sn * eqi;
+            nqr = sn * x01r + cs * eqr;
+            nqi = sn * x01i + cs * eqi;
+            x01r = npr; x01i = npi; x11r = nqr; x11i = nqi;
+          }
+          gr[pI][pJ] = x00r;
+          gr[pI][qJ] = x01r;
+          gr[qI][pJ] = x10r;
+          gr[qI][qJ] = x11r;
+          if (CPLX) {
+            gi[pI][pJ] = x00i;
+            gi[pI][qJ] = x01i;
+            gi[qI][pJ] = x10i;
+            gi[qI][qJ] = x11i;
+          }
         }
-        // x = [[g(pI,pJ), g(pI,qJ)], [g(qI,pJ), g(qI,qJ)]]
-        double x00r = gr[pI][pJ], x01r = gr[pI][qJ], x10r = gr[qI][pJ], x11r = gr[qI][qJ];
-        double x00i = 0, x01i = 0, x10i = 0, x11i = 0;
-        if (CPLX) {
-          x00i = gi[pI][pJ];
-          x01i = gi[pI][qJ];
-          x10i = gi[qI][pJ];
-          x11i = gi[qI][qJ];
-        }
-        if (rJ) {  // columns: [x_p x_q] <- [x_p x_q] [[cs, sn], [-sn e, cs e]]
-          const double cs = rcs[J], sn = rsn[J], er = rer[J], ei = rei[J];
-          double eqr = er * x01r - ei * x01i, eqi = er * x01i + ei * x01r;
-          double npr = cs * x00r - sn * eqr, npi = cs * x00i - sn * eqi;
-          double nqr = sn * x00r + cs * eqr, nqi = sn * x00i + cs * eqi;
-          x00r = npr; x00i = npi; x01r = nqr; x01i = nqi;
-          eqr = er * x11r - ei * x11i;
-          eqi = er * x11i + ei * x11r;
-          npr = cs * x10r - sn * eqr;
-          npi = cs * x10i - sn * eqi;
-          nqr = sn * x10r + cs * eqr;
-          nqi = sn * x10i + cs * eqi;
-          x10r = npr; x10i = npi; x11r = nqr; x11i = nqi;
-        }
-        if (rI) {  // rows: [x_p; x_q] <- J^H [x_p; x_q]  (conj(e))
-          const double cs = rcs[I], sn = rsn[I], er = rer[I], ei = -rei[I];
-          double eqr = er * x10r - ei * x10i, eqi = er * x10i + ei * x10r;
-          double npr = cs * x00r - sn * eqr, npi = cs * x00i - sn * eqi;
-          double nqr = sn * x00r + cs * eqr, nqi = sn * x00i + cs * eqi;
-          x00r = npr; x00i = npi; x10r = nqr; x10i = nqi;
-          eqr = er * x11r - ei * x11i;
-          eqi = er * x11i + ei * x11r;
-          npr = cs * x01r - sn * eqr;
-          npi = cs * x01i - sn * eqi;
-          nqr = sn * x01r + cs * eqr;
-          nqi = sn * x01i + cs * eqi;
-          x01r = npr; x01i = npi; x11r = nqr; x11i = nqi;
-        }
-        gr[pI][pJ] = x00r;
-        gr[pI][qJ] = x01r;
-        gr[qI][pJ] = x10r;
-        gr[qI][qJ] = x11r;
-        if (CPLX) {
-          gi[pI][pJ] = x00i;
-          gi[pI][qJ] = x01i;
-          gi[qI][pJ] = x10i;
-          gi[qI][qJ] = x11i;
-        }
-      }
-      // Q <- Q J, one row of a rotated column pair per task
-      for (int task = t; task < JP * NS; task += 256) {
-        const int r = task % JP, J = task / JP;
-        if (!rrot[J]) continue;
-        int p, q;
-        step_pair(cross, step, J, &p, &q);
-        const double cs = rcs[J], sn = rsn[J], er = rer[J], ei = rei[J];
-        const double pr_ = qr[r][p], qr_ = qr[r][q];
-        const double pi_ = CPLX ? qi[r][p] : 0.0, qi_ = CPLX ? qi[r][q] : 0.0;
-        const double eqr = er * qr_ - ei * qi_, eqi = er * qi_ + ei * qr_;
-        qr[r][p] = cs * pr_ - sn * eqr;
-        qr[r][q] = sn * pr_ + cs * eqr;
-        if (CPLX) {
-          qi[r][p] = cs * pi_ - sn * eqi;
-          qi[r][q] = sn * pi_ + cs * eqi;
+        if (rI) {  // Q <- Q J_I on this thread's rows
+          for (int r = t / NS; r < JP; r += NT / NS) {
+            const double pr_ = qr[r][pI], qr_ = qr[r][qI];
+            const double pi_ = CPLX ? qi[r][pI] : 0.0, qi_ = CPLX ? qi[r][qI] : 0.0;
+            const double eqr = erI * qr_ - eiI * qi_, eqi = erI * qi_ + eiI * qr_;
+            qr[r][pI] = csI * pr_ - snI * eqr;
+            qr[r][qI] = snI * pr_ + csI * eqr;
+            if (CPLX) {
+              qi[r][pI] = csI * pi_ - snI * eqi;
+              qi[r][qI] = snI * pi_ + csI * eqi;
+            }
+          }
         }
       }
       __syncthreads();
@@ -264,13 +274,14 @@ __device__ bool jacobi_lds(double (*gr)[JP + 1], double (*gi)[JP + 1], double (*
   return false;
 }
 
-// Diagonalise the 32 x 32 Hermitian sub-problem of each pair. Two-sided mode reads
+// Diagonalise the 2B x 2B Hermitian sub-problem of each pair. Two-sided mode reads
 // G[P,P] from the global matrix; one-sided mode sums the partial Grams. Writes the
-// accumulated unitary Q (and the diagonalised block D in two-sided mode), an
+// accumulated unitary Q (row-major, planar) and, in two-sided mode, the diagonalised
+// block D (column-major, planar), an
 // `active` flag (0 = pair already converged, nothing to apply) and the pair's
 // off-diagonal measure into conv (atomic max over the sweep).
-template <bool CPLX>
-__global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__ G, int64_t imG,
+template <bool CPLX, int B, int NT>
+__global__ __launch_bounds__(NT) void jac_eig_kernel(const double *__restrict__ G, int64_t imG,
                                                       int n, const double *__restrict__ partial,
                                                       int nsplit, bool two_sided,
                                                       const int *__restrict__ pairs, int cols,
@@ -279,12 +290,13 @@ __global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__
                                                       double *__restrict__ Dout,
                                                       int *__restrict__ active,
                                                       unsigned long long *__restrict__ conv) {
+  constexpr int JP = 2 * B;
   const int pair = blockIdx.x, t = threadIdx.x;
   __shared__ double gr[JP][JP + 1], gi[CPLX ? JP : 1][JP + 1], qr[JP][JP + 1], qi[CPLX ? JP : 1][JP + 1];
   __shared__ int colmap[JP];
-  if (t < JP) colmap[t] = pair_col(pairs + 2 * pair, t, cols);
+  if (t < JP) colmap[t] = pair_col<B>(pairs + 2 * pair, t, cols);
   __syncthreads();
-  for (int e = t; e < JP * JP; e += 256) {
+  for (int e = t; e < JP * JP; e += NT) {
     const int i = e % JP, j = e / JP;
     double vr = 0, vi = 0;
     if (two_sided) {
@@ -310,7 +322,7 @@ __global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__
   __syncthreads();
   // off-diagonal measure: max |g_ij| / sqrt(|g_ii g_jj|) over i != j above the noise floor
   double m = 0;
-  for (int e = t; e < JP * JP; e += 256) {
+  for (int e = t; e < JP * JP; e += NT) {
     const int i = e % JP, j = e / JP;
     if (i == j) continue;
     const double b = CPLX ? hypot(gr[i][j], gi[i][j]) : fabs(gr[i][j]);
@@ -318,22 +330,22 @@ __global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__
     const double s = sqrt(fabs(gr[i][i] * gr[j][j]));
     m = fmax(m, s > 0 ? b / s : 1.0);
   }
-  m = block_max(m);
+  m = block_max<NT>(m);
   if (t == 0) atomicMax(conv, (unsigned long long)__double_as_longlong(m));
   if (m <= tol) {
     if (t == 0) active[pair] = 0;
     return;
   }
-  (void)jacobi_lds<CPLX>(gr, gi, qr, qi, tol, abstol, max_inner, cross);
-  double *Q = Qout + int64_t(pair) * 2 * JP * JP;
-  for (int e = t; e < JP * JP; e += 256) {
-    const int i = e % JP, j = e / JP;
+  (void)jacobi_lds<CPLX, B, NT>(gr, gi, qr, qi, tol, abstol, max_inner, cross);
+  double *Q = Qout + int64_t(pair) * 2 * JP * JP;  // row-major: Q[i][j] at i * JP + j
+  for (int e = t; e < JP * JP; e += NT) {
+    const int i = e / JP, j = e % JP;
     Q[e] = qr[i][j];
     Q[JP * JP + e] = CPLX ? qi[i][j] : 0.0;
   }
   if (two_sided) {
     double *D = Dout + int64_t(pair) * 2 * JP * JP;
-    for (int e = t; e < JP * JP; e += 256) {
+    for (int e = t; e < JP * JP; e += NT) {
       const int i = e % JP, j = e / JP;
       D[e] = gr[i][j];
       D[JP * JP + e] = CPLX ? gi[i][j] : 0.0;
@@ -343,13 +355,14 @@ __global__ __launch_bounds__(256) void jac_eig_kernel(const double *__restrict__
 }
 
 // Y[:, P] <- Y[:, P] Q_P for Y in {X (rows_x rows), V (rows_v rows)}.
-template <bool CPLX>
+template <bool CPLX, int B>
 __global__ __launch_bounds__(256) void jac_colupd_kernel(double *__restrict__ X, int rows_x,
                                                          int64_t imX, double *__restrict__ V,
                                                          int rows_v, int64_t imV,
                                                          const int *__restrict__ pairs, int cols,
                                                          const double *__restrict__ Qin,
                                                          const int *__restrict__ active) {
+  constexpr int JP = 2 * B;
   const int pair = blockIdx.y;
   if (!active[pair]) return;
   __shared__ double qr[JP * JP], qi[CPLX ? JP * JP : 1];
@@ -360,7 +373,7 @@ __global__ __launch_bounds__(256) void jac_colupd_kernel(double *__restrict__ X,
     qr[e] = Q[e];
     if (CPLX) qi[e] = Q[JP * JP + e];
   }
-  if (t < JP) colmap[t] = pair_col(pairs + 2 * pair, t, cols);
+  if (t < JP) colmap[t] = pair_col<B>(pairs + 2 * pair, t, cols);
   __syncthreads();
   const int r = blockIdx.x * 256 + t;
   for (int which = 0; which < 2; ++which) {
@@ -381,10 +394,10 @@ __global__ __launch_bounds__(256) void jac_colupd_kernel(double *__restrict__ X,
       double sr = 0, si = 0;
 #pragma unroll
       for (int k = 0; k < JP; ++k) {
-        const double a = qr[k + JP * j];
+        const double a = qr[k * JP + j];
         sr += yr[k] * a;
         if (CPLX) {
-          const double b = qi[k + JP * j];
+          const double b = qi[k * JP + j];
           sr -= yi[k] * b;
           si += yr[k] * b + yi[k] * a;
         }
@@ -437,12 +450,12 @@ __global__ __launch_bounds__(256) void jac_twoside_kernel(double *__restrict__ G
     tr[i][j] = ok ? G[r + int64_t(c) * n] : 0.0;
     if (CPLX) ti[i][j] = ok ? G[imG + r + int64_t(c) * n] : 0.0;
     if (aP) {
-      pr_[i][j] = QP[e];
-      if (CPLX) pi_[i][j] = QP[JP * JP + e];
+      pr_[i][j] = QP[i * JP + j];
+      if (CPLX) pi_[i][j] = QP[JP * JP + i * JP + j];
     }
     if (aK) {
-      kr_[i][j] = QK[e];
-      if (CPLX) ki_[i][j] = QK[JP * JP + e];
+      kr_[i][j] = QK[i * JP + j];
+      if (CPLX) ki_[i][j] = QK[JP * JP + i * JP + j];
     }
   }
   __syncthreads();
@@ -513,16 +526,18 @@ typedef double jv4d __attribute__((ext_vector_type(4)));
 
 // Real two-sided mode on the matrix cores, exploiting the symmetry of G: one workgroup per
 // tile of the upper triangle of pair tiles (P <= K, blockIdx.x = K (K + 1) / 2 + P) forms
-// Q_P^T G[P, K] Q_K with v_mfma_f64_16x16x4f64 (one 16 x 16 quarter per wave, the product
-// staged through LDS) and writes it and its transpose G[K, P], so G stays bitwise symmetric
-// and half the tiles are read. The Q of an inactive pair is the identity (exact in FP64).
-// v_mfma_f64_16x16x4f64 lane (g = lane / 16, rr = lane % 16) supplies A[rr][k_g] and
-// B[k_g][rr]; accumulator register r holds C[g + 4 r][rr].
+// Q_P^T G[P, K] Q_K with v_mfma_f64_16x16x4f64 (each wave one quadrant of the 2B x 2B tile,
+// the intermediate product staged through LDS) and writes it and its transpose G[K, P], so G
+// stays bitwise symmetric and half the tiles are read. The Q of an inactive pair is the
+// identity (exact in FP64). v_mfma_f64_16x16x4f64 lane (g = lane / 16, rr = lane % 16)
+// supplies A[rr][k_g] and B[k_g][rr]; accumulator register r holds C[g + 4 r][rr].
+template <int B>
 __global__ __launch_bounds__(256) void jac_twoside_sym_kernel(double *__restrict__ G, int n,
                                                               const int *__restrict__ pairs,
                                                               const double *__restrict__ Qin,
                                                               const double *__restrict__ Din,
                                                               const int *__restrict__ active) {
+  constexpr int JP = 2 * B, H = JP / 2, FT = H / 16, KS = JP / 4;  // quadrant, MFMA tiles, k steps
   const int idx = blockIdx.x, t = threadIdx.x;
   int K = int((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
   while ((K + 1) * (K + 2) / 2 <= idx) ++K;
@@ -530,11 +545,11 @@ __global__ __launch_bounds__(256) void jac_twoside_sym_kernel(double *__restrict
   const int P = idx - K * (K + 1) / 2;
   const bool aP = active[P] != 0, aK = active[K] != 0;
   if (!aP && !aK) return;
-  __shared__ double T[JP][JP + 1], QP[JP][JP + 1], QK[JP][JP + 1];
+  __shared__ double T[JP][JP + 1];
   __shared__ int rowmap[JP], colmap[JP];
   if (t < JP) {
-    rowmap[t] = pair_col(pairs + 2 * P, t, n);
-    colmap[t] = pair_col(pairs + 2 * K, t, n);
+    rowmap[t] = pair_col<B>(pairs + 2 * P, t, n);
+    colmap[t] = pair_col<B>(pairs + 2 * K, t, n);
   }
   __syncthreads();
   if (P == K) {  // aP holds here
@@ -546,33 +561,77 @@ __global__ __launch_bounds__(256) void jac_twoside_sym_kernel(double *__restrict
     }
     return;
   }
+  // row-major rotations: Q[k][j] at k * JP + j
   const double *qp = Qin + int64_t(P) * 2 * JP * JP;
   const double *qk = Qin + int64_t(K) * 2 * JP * JP;
   for (int e = t; e < JP * JP; e += 256) {
     const int i = e % JP, j = e / JP;
     const int r = rowmap[i], c = colmap[j];
     T[i][j] = (r >= 0 && c >= 0) ? G[r + int64_t(c) * n] : 0.0;
-    QP[i][j] = aP ? qp[e] : (i == j ? 1.0 : 0.0);
-    QK[i][j] = aK ? qk[e] : (i == j ? 1.0 : 0.0);
+  }
+  const int lane = t & 63, w = t >> 6, g = lane >> 4, rr = lane & 15;
+  const int i0 = H * (w & 1), j0 = H * (w >> 1);
+  // B operands of T Q_K: Q_K[4 s + g][j0 + 16 y + rr] (the identity when K is inactive)
+  double fb[KS][FT];
+#pragma unroll
+  for (int s = 0; s < KS; ++s)
+#pragma unroll
+    for (int y = 0; y < FT; ++y) {
+      const int k = 4 * s + g, j = j0 + 16 * y + rr;
+      fb[s][y] = aK ? qk[k * JP + j] : (k == j ? 1.0 : 0.0);
+    }
+  __syncthreads();
+  jv4d acc[FT][FT];
+#pragma unroll
+  for (int x = 0; x < FT; ++x)
+#pragma unroll
+    for (int y = 0; y < FT; ++y) acc[x][y] = jv4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {  // U = T Q_K
+    double fa[FT];
+#pragma unroll
+    for (int x = 0; x < FT; ++x) fa[x] = T[i0 + 16 * x + rr][4 * s + g];
+#pragma unroll
+    for (int x = 0; x < FT; ++x)
+#pragma unroll
+      for (int y = 0; y < FT; ++y) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[x], fb[s][y], acc[x][y], 0, 0, 0);
+  }
+  // A operands of Q_P^T U: Q_P[4 s + g][i0 + 16 x + rr]
+#pragma unroll
+  for (int s = 0; s < KS; ++s)
+#pragma unroll
+    for (int x = 0; x < FT; ++x) {
+      const int k = 4 * s + g, i = i0 + 16 * x + rr;
+      fb[s][x] = aP ? qp[k * JP + i] : (k == i ? 1.0 : 0.0);
+    }
+  __syncthreads();
+#pragma unroll
+  for (int x = 0; x < FT; ++x)
+#pragma unroll
+    for (int y = 0; y < FT; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        T[i0 + 16 * x + g + 4 * r][j0 + 16 * y + rr] = acc[x][y][r];
+        acc[x][y][r] = 0.0;
+      }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {  // Q_P^T U
+    double fu[FT];
+#pragma unroll
+    for (int y = 0; y < FT; ++y) fu[y] = T[4 * s + g][j0 + 16 * y + rr];
+#pragma unroll
+    for (int x = 0; x < FT; ++x)
+#pragma unroll
+      for (int y = 0; y < FT; ++y) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[s][x], fu[y], acc[x][y], 0, 0, 0);
   }
   __syncthreads();
-  const int lane = t & 63, w = t >> 6, g = lane >> 4, rr = lane & 15;
-  const int i0 = 16 * (w & 1), j0 = 16 * (w >> 1);
-  jv4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int s = 0; s < JP / 4; ++s)  // U = T Q_K
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(T[i0 + rr][4 * s + g], QK[4 * s + g][j0 + rr], acc, 0, 0, 0);
-  __syncthreads();
+  for (int x = 0; x < FT; ++x)
 #pragma unroll
-  for (int r = 0; r < 4; ++r) T[i0 + g + 4 * r][j0 + rr] = acc[r];
-  __syncthreads();
-  acc = jv4d{0.0, 0.0, 0.0, 0.0};
+    for (int y = 0; y < FT; ++y)
 #pragma unroll
-  for (int s = 0; s < JP / 4; ++s)  // Q_P^T U
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(QP[4 * s + g][i0 + rr], T[4 * s + g][j0 + rr], acc, 0, 0, 0);
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < 4; ++r) T[i0 + g + 4 * r][j0 + rr] = acc[r];
+      for (int r = 0; r < 4; ++r) T[i0 + 16 * x + g + 4 * r][j0 + 16 * y + rr] = acc[x][y][r];
   __syncthreads();
   for (int e = t; e < JP * JP; e += 256) {
     {  // G[P, K], rows fastest
@@ -586,6 +645,62 @@ __global__ __launch_bounds__(256) void jac_twoside_sym_kernel(double *__restrict
       if (r >= 0 && c >= 0) G[c + int64_t(r) * n] = T[i][j];
     }
   }
+}
+
+// Real V[:, P] <- V[:, P] Q_P on the matrix cores, as V[:, P]^T <- Q_P^T V[:, P]^T: the
+// v_mfma_f64_16x16x4f64 B operand is then 16 consecutive rows of one column of V, read
+// straight from memory, and each accumulator register of a lane group holds 16 consecutive
+// rows of one output column. One workgroup per 128 rows and pair, wave w on rows 32 w..;
+// Q_P staged in LDS. Workgroups touch disjoint (rows, columns) blocks, so V updates in place.
+template <int B>
+__global__ __launch_bounds__(256) void jac_colupd_mfma_kernel(double *__restrict__ Y, int rows,
+                                                              const int *__restrict__ pairs, int cols,
+                                                              const double *__restrict__ Qin,
+                                                              const int *__restrict__ active) {
+  constexpr int JP = 2 * B, FJ = JP / 16, KS = JP / 4;
+  const int pair = blockIdx.y;
+  if (!active[pair]) return;
+  __shared__ double Qs[JP][JP + 1];
+  __shared__ int colmap[JP];
+  const int t = threadIdx.x;
+  const double *Q = Qin + int64_t(pair) * 2 * JP * JP;  // row-major Q[k][j]
+  for (int e = t; e < JP * JP; e += 256) Qs[e / JP][e % JP] = Q[e];
+  if (t < JP) colmap[t] = pair_col<B>(pairs + 2 * pair, t, cols);
+  __syncthreads();
+  const int lane = t & 63, w = t >> 6, g = lane >> 4, rr = lane & 15;
+  const int r0 = blockIdx.x * 128 + 32 * w;
+  double vb[KS][2];
+#pragma unroll
+  for (int s = 0; s < KS; ++s)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+      const int r = r0 + 16 * y + rr, c = colmap[4 * s + g];
+      vb[s][y] = (r < rows && c >= 0) ? Y[r + int64_t(c) * rows] : 0.0;
+    }
+  jv4d acc[FJ][2];
+#pragma unroll
+  for (int x = 0; x < FJ; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) acc[x][y] = jv4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+#pragma unroll
+    for (int x = 0; x < FJ; ++x) {
+      const double fa = Qs[4 * s + g][16 * x + rr];  // Q^T[j][k] = Q[k][j]
+#pragma unroll
+      for (int y = 0; y < 2; ++y) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa, vb[s][y], acc[x][y], 0, 0, 0);
+    }
+  }
+  // acc[x][y][r] = new V[r0 + 16 y + rr][column 16 x + g + 4 r of the pair]
+#pragma unroll
+  for (int x = 0; x < FJ; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = r0 + 16 * y + rr, c = colmap[16 * x + g + 4 * r];
+        if (i < rows && c >= 0) Y[i + int64_t(c) * rows] = acc[x][y][r];
+      }
 }
 
 // G <- (G + G^H) / 2: the MFMA product M M^H is not bitwise Hermitian (G[i,j] and G[j,i]
@@ -714,7 +829,7 @@ __global__ __launch_bounds__(256) void tiny_eig_kernel(const double *__restrict_
   double dm = t < JP ? fabs(gr[t][t]) : 0.0;
   dm = block_max(dm);
   const double abstol = 4.0 * 2.220446049250313e-16 * sqrt(double(p)) * dm;
-  const bool conv = jacobi_lds<CPLX>(gr, gi, qr, qi, tol, abstol, kMaxInnerCap);
+  const bool conv = jacobi_lds<CPLX, JB, 256>(gr, gi, qr, qi, tol, abstol, kMaxInnerCap);
   for (int e = t; e < p * p; e += 256) {
     const int i = e % p, j = e / p;
     V[i + int64_t(j) * p] = qr[i][j];
@@ -739,9 +854,10 @@ struct JacobiResult {
   double measure = 0;
 };
 
-static Schedule schedule_for(tnh_ctx *ctx, int cols) {
+// The circle schedule depends on the block count only, whatever the block width.
+static Schedule schedule_for(tnh_ctx *ctx, int cols, int width) {
   Schedule s;
-  s.nb = (cols + JB - 1) / JB;
+  s.nb = (cols + width - 1) / width;
   s.nbp = s.nb + (s.nb & 1);
   if (s.nbp < 2) s.nbp = 2;
   s.rounds = s.nbp - 1;
@@ -767,10 +883,19 @@ static Schedule schedule_for(tnh_ctx *ctx, int cols) {
 
 // two_sided: diagonalise Hermitian G (n x n) in place, V <- V * rotations.
 // one-sided: orthogonalise the columns of X (rows x n) in place, V (n x n) accumulates.
-static JacobiResult jacobi(tnh_ctx *ctx, bool two_sided, double *X, int rows, int n, bool cplx,
-                           double *V, double tol, double abstol, int max_sweeps) {
+// Columns are cut into blocks of width B (pairs of 2B columns); the real two-sided problem
+// uses B = 32, which halves the rounds per sweep and so the passes over G and V, the
+// others B = JB = 16.
+template <int B>
+static JacobiResult jacobi_blocks(tnh_ctx *ctx, bool two_sided, double *X, int rows, int n, bool cplx,
+                                  double *V, double tol, double abstol, int max_sweeps) {
+  constexpr int JPB = 2 * B;
+  constexpr int NT = B == JB ? 256 : 1024;  // threads of the pair eigensolver (1024 measured faster than 512 at B = 32)
+  static_assert(B == JB || B == 2 * JB, "block widths 16 and 32");
+  if constexpr (B != JB)
+    TNH_REQUIRE(two_sided && !cplx, TNH_ERR_ARG, "wide Jacobi blocks serve the real two-sided problem");
   JacobiResult res;
-  const Schedule s = schedule_for(ctx, n);
+  const Schedule s = schedule_for(ctx, n, B);
   const int64_t imX = int64_t(rows) * n, imV = int64_t(n) * n;
   int nsplit = 1;
   if (!two_sided) {
@@ -780,9 +905,9 @@ static JacobiResult jacobi(tnh_ctx *ctx, bool two_sided, double *X, int rows, in
   double *partial = nullptr;
   if (!two_sided)
     partial = static_cast<double *>(
-        ctx->workspace(601, sizeof(double) * size_t(s.npairs) * nsplit * 2 * JP * JP));
-  double *Q = static_cast<double *>(ctx->workspace(602, sizeof(double) * size_t(s.npairs) * 2 * JP * JP));
-  double *D = two_sided ? static_cast<double *>(ctx->workspace(603, sizeof(double) * size_t(s.npairs) * 2 * JP * JP))
+        ctx->workspace(601, sizeof(double) * size_t(s.npairs) * nsplit * 2 * JPB * JPB));
+  double *Q = static_cast<double *>(ctx->workspace(602, sizeof(double) * size_t(s.npairs) * 2 * JPB * JPB));
+  double *D = two_sided ? static_cast<double *>(ctx->workspace(603, sizeof(double) * size_t(s.npairs) * 2 * JPB * JPB))
                         : nullptr;
   int *active = static_cast<int *>(ctx->workspace(604, sizeof(int) * s.npairs));
   unsigned long long *conv = static_cast<unsigned long long *>(
@@ -799,31 +924,42 @@ static JacobiResult jacobi(tnh_ctx *ctx, bool two_sided, double *X, int rows, in
       // the first round of a sweep visits every block once: it also rotates the index pairs
       // inside the blocks; the other rounds only the pairs across the two blocks of a pair
       const bool cross = s.nb > 2 && r > 0;
-      if (!two_sided)
-        hipLaunchKernelGGL(jac_gram_kernel, dim3(s.npairs, nsplit), dim3(256), 0, ctx->stream, X,
-                           rows, imX, n, cplx, pr, nsplit, partial);
-      if (cplx)
-        hipLaunchKernelGGL(jac_eig_kernel<true>, dim3(s.npairs), dim3(256), 0, ctx->stream, X, imX,
-                           n, partial, nsplit, two_sided, pr, n, tol, abstol, max_inner, cross, Q, D,
-                           active, conv + sw);
-      else
-        hipLaunchKernelGGL(jac_eig_kernel<false>, dim3(s.npairs), dim3(256), 0, ctx->stream, X, imX,
-                           n, partial, nsplit, two_sided, pr, n, tol, abstol, max_inner, cross, Q, D,
+      if constexpr (B == JB) {
+        if (!two_sided)
+          hipLaunchKernelGGL(jac_gram_kernel, dim3(s.npairs, nsplit), dim3(256), 0, ctx->stream, X,
+                             rows, imX, n, cplx, pr, nsplit, partial);
+      }
+      if constexpr (B == JB) {
+        if (cplx)
+          hipLaunchKernelGGL((jac_eig_kernel<true, B, NT>), dim3(s.npairs), dim3(NT), 0, ctx->stream, X,
+                             imX, n, partial, nsplit, two_sided, pr, n, tol, abstol, max_inner, cross, Q, D,
+                             active, conv + sw);
+      }
+      if (!cplx)
+        hipLaunchKernelGGL((jac_eig_kernel<false, B, NT>), dim3(s.npairs), dim3(NT), 0, ctx->stream, X,
+                           imX, n, partial, nsplit, two_sided, pr, n, tol, abstol, max_inner, cross, Q, D,
                            active, conv + sw);
       // two-sided: G takes both rotations in one tiled pass, V its columns; one-sided: X and
       // V take the column rotations
       double *Xc = two_sided ? nullptr : X;
-      if (cplx)
-        hipLaunchKernelGGL(jac_colupd_kernel<true>, dim3((rmax + 255) / 256, s.npairs), dim3(256), 0,
+      if constexpr (B == JB) {
+        if (cplx)
+          hipLaunchKernelGGL((jac_colupd_kernel<true, B>), dim3((rmax + 255) / 256, s.npairs), dim3(256), 0,
+                             ctx->stream, Xc, rows, imX, V, n, imV, pr, n, Q, active);
+      }
+      if (!cplx && two_sided)
+        hipLaunchKernelGGL(jac_colupd_mfma_kernel<B>, dim3((n + 127) / 128, s.npairs), dim3(256), 0,
+                           ctx->stream, V, n, pr, n, Q, active);
+      else if (!cplx)
+        hipLaunchKernelGGL((jac_colupd_kernel<false, B>), dim3((rmax + 255) / 256, s.npairs), dim3(256), 0,
                            ctx->stream, Xc, rows, imX, V, n, imV, pr, n, Q, active);
-      else
-        hipLaunchKernelGGL(jac_colupd_kernel<false>, dim3((rmax + 255) / 256, s.npairs), dim3(256), 0,
-                           ctx->stream, Xc, rows, imX, V, n, imV, pr, n, Q, active);
-      if (two_sided && cplx)
-        hipLaunchKernelGGL(jac_twoside_kernel<true>, dim3(s.npairs, s.npairs), dim3(256), 0,
-                           ctx->stream, X, n, imX, pr, Q, D, active);
-      else if (two_sided)
-        hipLaunchKernelGGL(jac_twoside_sym_kernel, dim3(s.npairs * (s.npairs + 1) / 2), dim3(256), 0,
+      if constexpr (B == JB) {
+        if (two_sided && cplx)
+          hipLaunchKernelGGL(jac_twoside_kernel<true>, dim3(s.npairs, s.npairs), dim3(256), 0,
+                             ctx->stream, X, n, imX, pr, Q, D, active);
+      }
+      if (two_sided && !cplx)
+        hipLaunchKernelGGL(jac_twoside_sym_kernel<B>, dim3(s.npairs * (s.npairs + 1) / 2), dim3(256), 0,
                            ctx->stream, X, n, pr, Q, D, active);
     }
     TNH_HIP(hipGetLastError());
@@ -840,6 +976,13 @@ static JacobiResult jacobi(tnh_ctx *ctx, bool two_sided, double *X, int rows, in
     }
   }
   return res;
+}
+
+static JacobiResult jacobi(tnh_ctx *ctx, bool two_sided, double *X, int rows, int n, bool cplx,
+                           double *V, double tol, double abstol, int max_sweeps) {
+  if (two_sided && !cplx)
+    return jacobi_blocks<2 * JB>(ctx, two_sided, X, rows, n, cplx, V, tol, abstol, max_sweeps);
+  return jacobi_blocks<JB>(ctx, two_sided, X, rows, n, cplx, V, tol, abstol, max_sweeps);
 }
 
 // ------------------------------------------------------------- decomposition
