@@ -828,6 +828,18 @@ __global__ __launch_bounds__(256) void tiny_eig_kernel(const double *__restrict_
   __syncthreads();
   double dm = t < JP ? fabs(gr[t][t]) : 0.0;
   dm = block_max(dm);
+  // exact power-of-two scaling to max_i G_ii in [1, 2): the rotation test forms G_ii G_jj,
+  // which would overflow (or underflow) for Gram matrices of very large (or small) norm
+  const int ex = (dm > 0 && isfinite(dm)) ? ilogb(dm) : 0;
+  if (ex != 0) {
+    const double sc = ldexp(1.0, -ex);
+    for (int e = t; e < JP * JP; e += 256) {
+      gr[e % JP][e / JP] *= sc;
+      if (CPLX) gi[e % JP][e / JP] *= sc;
+    }
+    dm *= sc;
+    __syncthreads();
+  }
   const double abstol = 4.0 * 2.220446049250313e-16 * sqrt(double(p)) * dm;
   const bool conv = jacobi_lds<CPLX, JB, 256>(gr, gi, qr, qi, tol, abstol, kMaxInnerCap);
   for (int e = t; e < p * p; e += 256) {
@@ -835,7 +847,7 @@ __global__ __launch_bounds__(256) void tiny_eig_kernel(const double *__restrict_
     V[i + int64_t(j) * p] = qr[i][j];
     if (CPLX) V[imV + i + int64_t(j) * p] = qi[i][j];
   }
-  if (t < p) out[t] = gr[t][t];
+  if (t < p) out[t] = ldexp(gr[t][t], ex);
   if (t == 0) out[p] = conv ? 1.0 : 0.0;
 }
 
@@ -1103,6 +1115,13 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
       to_host(ctx, val.data(), dv, sizeof(double) * p);
       for (double v : val) dmax = std::max(dmax, std::fabs(v));
     }
+    // exact power-of-two scaling to max_i G_ii in [1, 2): the rotation test forms G_ii G_jj,
+    // which overflows for Gram matrices above ~1e154 (products of unnormalised tensors)
+    const int ex = (dmax > 0 && std::isfinite(dmax)) ? std::ilogb(dmax) : 0;
+    if (ex != 0) {
+      scale(ctx, G, TNH_F64, pp * planes, std::ldexp(1.0, -ex));
+      dmax = std::ldexp(dmax, -ex);
+    }
     hipLaunchKernelGGL(eye_kernel, dim3(unsigned((pp + 255) / 256)), dim3(256), 0, ctx->stream, V, p,
                        pp, cplx);
     const double abstol = 4.0 * 2.220446049250313e-16 * std::sqrt(double(p)) * dmax;
@@ -1115,7 +1134,7 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
       ok = true;
       for (auto &v : val) {
         if (!std::isfinite(v)) ok = false;
-        v = std::sqrt(std::max(v, 0.0));
+        v = std::sqrt(std::max(std::ldexp(v, ex), 0.0));
       }
     }
   }

@@ -29,12 +29,15 @@ def movecenter(psi: GMPS, idx: int, cutoff=0.0, maxdim=0, mindim=1, ctx: Context
     ctx = ctx or Context.default()
     dev = _upload_all(psi.tensors, ctx)
     arr = _handles(dev)
-    ctx.call("tnh_mps_movecenter", len(psi), arr, int(psi.center), int(idx), float(cutoff),
-             int(maxdim), int(mindim))
-    # the library replaced the handles in place
-    out = [Tensor(ctx, C.c_void_p(arr[i])) for i in range(len(psi))]
-    for t in dev:
-        t.h = None  # freed or reused by the library
+    try:
+        ctx.call("tnh_mps_movecenter", len(psi), arr, int(psi.center), int(idx), float(cutoff),
+                 int(maxdim), int(mindim))
+    finally:
+        # the library replaces the handles in place as it moves (also when it fails part-way,
+        # the array then holds the moved and the untouched sites): take ownership from it
+        out = [Tensor(ctx, C.c_void_p(arr[i])) for i in range(len(psi))]
+        for t in dev:
+            t.h = None  # freed or reused by the library
     psi.tensors = [t.numpy() for t in out]
     psi.center = idx
     return psi

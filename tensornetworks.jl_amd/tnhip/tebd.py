@@ -65,9 +65,11 @@ class DeviceMPS:
     def movecenter(self, idx, cutoff=0.0, maxdim=0, mindim=1):
         """movecenter!(psi, idx; cutoff, maxdim, mindim), gmps.jl:90-112."""
         arr = self._handles()
-        self.ctx.call("tnh_mps_movecenter", len(self), arr, int(self.center), int(idx), float(cutoff),
-                      int(maxdim), int(mindim))
-        self._adopt(arr)
+        try:
+            self.ctx.call("tnh_mps_movecenter", len(self), arr, int(self.center), int(idx), float(cutoff),
+                          int(maxdim), int(mindim))
+        finally:
+            self._adopt(arr)  # moved sites are replaced in place, also before a failure
         self.center = idx
 
     def norm(self):
@@ -181,12 +183,14 @@ def applygate(psi: DeviceMPS, site, gate_dev: Tensor, gate_np, direction, cutoff
         h2 = C.c_void_p(psi[site + 1].h.value)
         k = C.c_int()
         disc = C.c_double()
-        psi.ctx.call("tnh_applygate2", C.byref(h1), C.byref(h2), gate_dev.h, int(bool(direction)), float(cutoff),
-                     int(maxdim), int(mindim), C.byref(k), C.byref(disc))
-        for j, h in ((site, h1), (site + 1, h2)):
-            if psi.sites[j - 1].h.value != h.value:
-                psi.sites[j - 1].h = None
-                psi.sites[j - 1] = Tensor(psi.ctx, h)
+        try:
+            psi.ctx.call("tnh_applygate2", C.byref(h1), C.byref(h2), gate_dev.h, int(bool(direction)),
+                         float(cutoff), int(maxdim), int(mindim), C.byref(k), C.byref(disc))
+        finally:
+            for j, h in ((site, h1), (site + 1, h2)):  # replaced buffers, also before a failure
+                if psi.sites[j - 1].h.value != h.value:
+                    psi.sites[j - 1].h = None
+                    psi.sites[j - 1] = Tensor(psi.ctx, h)
         psi.center = site if direction else site + 1
         return disc.value
     if rng == 1:

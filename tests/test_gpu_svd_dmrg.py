@@ -57,6 +57,25 @@ def test_svd_full_rank(ctx, rng, fallback, cplx, shape):
     assert disc == 0.0
 
 
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("shape", [(20, 24), (100, 150)])
+@pytest.mark.parametrize("scale", [1e100, 1e-100])
+def test_svd_extreme_norm(ctx, rng, cplx, shape, scale):
+    """Gram entries near 1e200 (1e-200): the Jacobi rotation test multiplies two diagonal
+    entries, which would overflow (underflow) without the exact power-of-two rescaling of
+    G. Shapes cover the single-workgroup eigensolve (p = 20) and the block path (p = 100)."""
+    from tnhip import Tensor, ops as T
+
+    M = np.asfortranarray(randn(rng, shape, cplx) * scale)
+    U, S, Vt, disc, fb = T.svd_trunc(Tensor.from_numpy(M), 0.0, 0, 1)
+    assert not fb
+    u, s, vt = U.numpy(), S.numpy(), Vt.numpy()
+    ref = np.linalg.svd(M, compute_uv=False)
+    assert np.max(np.abs((s / ref[0]) ** 2 - (ref / ref[0]) ** 2)) <= 1e-13 * np.sqrt(max(shape))
+    assert np.linalg.norm((u * s) @ vt - M) <= 1e-12 * np.linalg.norm(M)
+    assert np.abs(u.conj().T @ u - np.eye(s.size)).max() <= 1e-12
+
+
 @pytest.mark.parametrize("fallback", [False, True])
 @pytest.mark.parametrize("cutoff,maxdim,mindim", [(1e-10, 0, 1), (1e-4, 0, 1), (0.0, 7, 1), (1e-6, 5, 1),
                                                   (1e-2, 0, 9), (1e-14, 0, 1)])
