@@ -268,6 +268,18 @@ int tnh_qjmc_run(tnh_ctx *ctx, const tnh_qjmc_problem *problem, int ntraj, const
                  int64_t nuniform, int nworkers, double *meas, int *njumps, int *jumps,
                  int *jump_steps, int maxjumps, int *maxbond);
 
+/* The same trajectories and outputs as tnh_qjmc_run (same algorithm and uniform order),
+ * advanced together on ctx's stream: all site tensors of all trajectories stay resident in
+ * fixed-capacity slots (bond dimension at most min(16, 32 / d)), and every launch applies
+ * one primitive (centre move, two-site gate split, one-site operator, normalisation) to
+ * every trajectory, one workgroup each, with the Gram eigensolve of each split in LDS.
+ * Replaces the per-trajectory loop over qjmc_simulation (qjmc.jl:28-167) for the small bond
+ * dimensions of dissipative chains. Returns TNH_ERR_UNSUPPORTED (state discarded) when a
+ * split would keep more than the slot capacity; tnh_qjmc_run has no such limit. */
+int tnh_qjmc_run_batched(tnh_ctx *ctx, const tnh_qjmc_problem *problem, int ntraj,
+                         const double *uniforms, int64_t nuniform, double *meas, int *njumps,
+                         int *jumps, int *jump_steps, int maxjumps, int *maxbond);
+
 /* ---------------------------------------------------- device-resident DMRG */
 
 typedef struct tnh_dmrg_params {

@@ -123,12 +123,16 @@ def _cplx_flat(m):
 
 def qjmc_trajectories(st: Sitetypes, init_names, H: OpList, jumpops: OpList, tmax, dt, uniforms,
                       observables: OpList | None = None, save=0.0, cutoff=1e-12, mindim=1, maxdim=0,
-                      order=2, nworkers=4, maxjumps=256, ctx=None):
+                      order=2, nworkers=4, maxjumps=256, engine="auto", ctx=None):
     """Many trajectories of qjmc_simulation (qjmc.jl:28-167, classical branch) run natively by
     tnh_qjmc_run: the step loop in C++, every tensor operation on the GPU, `nworkers` threads
     each with its own stream. All trajectories start from productMPS(st, init_names);
     trajectory t consumes uniforms[t] in the reference's rand() order. Jump operators must
-    be single-site (multi-site ones run through qjmc_simulation). Returns a dict with
+    be single-site (multi-site ones run through qjmc_simulation). engine: "batched" advances
+    all trajectories together (tnh_qjmc_run_batched: one launch per primitive for every
+    trajectory, bond dimension at most min(16, 32 / d)); "workers" runs them one by one on
+    `nworkers` streams (tnh_qjmc_run, no bond limit); "auto" tries the batched engine and
+    reruns on the workers when a bond dimension outgrows its slots. Returns a dict with
     `jumps` and `jumptimes` (lists per trajectory, 1-based indices, as the reference stores
     them), `measurements` (ntraj x nsave x nobs, coefficient * <O>), `times`, `maxbonddim`."""
     from ._lib import QjmcProblem
@@ -193,8 +197,24 @@ def qjmc_trajectories(st: Sitetypes, init_names, H: OpList, jumpops: OpList, tma
     jsteps = np.zeros((ntraj, maxjumps), dtype=np.int32)
     maxb = np.zeros(ntraj, dtype=np.int32)
     ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))  # noqa: E731
-    ctx.call("tnh_qjmc_run", C.byref(p), ntraj, dp(U), nu, int(nworkers), dp(meas), ip(njumps), ip(jumps),
-             ip(jsteps), int(maxjumps), ip(maxb))
+    if engine not in ("auto", "batched", "workers"):
+        raise ValueError("engine must be 'auto', 'batched' or 'workers'")
+    used = "workers"
+    if engine in ("auto", "batched"):
+        from ._lib import TnhError
+
+        try:
+            ctx.call("tnh_qjmc_run_batched", C.byref(p), ntraj, dp(U), nu, dp(meas), ip(njumps), ip(jumps),
+                     ip(jsteps), int(maxjumps), ip(maxb))
+            used = "batched"
+        except TnhError as e:
+            if engine == "batched" or "bond dimension above" not in str(e):
+                raise
+            meas[...] = 0
+            njumps[...] = 0
+    if used == "workers":
+        ctx.call("tnh_qjmc_run", C.byref(p), ntraj, dp(U), nu, int(nworkers), dp(meas), ip(njumps), ip(jumps),
+                 ip(jsteps), int(maxjumps), ip(maxb))
     coeffs = np.array(obs.coeffs, dtype=complex) if obs.coeffs else np.zeros(0)
     vals = meas[..., 0] + 1j * meas[..., 1]
     vals = vals[:, :, : len(osite)] * coeffs[None, None, :]
@@ -206,6 +226,7 @@ def qjmc_trajectories(st: Sitetypes, init_names, H: OpList, jumpops: OpList, tma
         "measurements": vals,
         "times": np.arange(nsave) * savesteps * dt,
         "maxbonddim": maxb.tolist(),
+        "engine": used,
     }
 
 

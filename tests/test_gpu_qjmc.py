@@ -64,9 +64,11 @@ def test_qjmc_trajectories_match_dense(ctx, classical):
         assert np.allclose(got, meas[m].real, atol=1e-8)
 
 
-def test_native_runner_matches_dense(ctx):
-    """tnh_qjmc_run (C++ step loop, two worker threads with their own streams) against the
-    dense oracle on the same uniforms: identical jump records, <sz_i> to 1e-8."""
+@pytest.mark.parametrize("engine", ["workers", "batched"])
+def test_native_runner_matches_dense(ctx, engine):
+    """tnh_qjmc_run (C++ step loop, two worker threads with their own streams) and
+    tnh_qjmc_run_batched (all trajectories per launch) against the dense oracle on the same
+    uniforms: identical jump records, <sz_i> to 1e-8."""
     from tnhip import spinhalf
     from tnhip.qjmc import qjmc_trajectories
     from oracle.qjmc import qjmc_dense
@@ -80,12 +82,56 @@ def test_native_runner_matches_dense(ctx):
                                      obs=[("z", i) for i in range(1, N + 1)])
     assert sum(len(j) for j in jr) >= 2
     out = qjmc_trajectories(spinhalf(), ["up"] * N, H, J, tmax, dt, U, observables=zlist(N), save=0.25,
-                            cutoff=0.0, nworkers=2)
+                            cutoff=0.0, nworkers=2, engine=engine)
+    assert out["engine"] == engine
     assert out["jumps"] == jr
     for m in range(U.shape[0]):
         assert np.allclose(out["jumptimes"][m], tr[m], atol=1e-12)
     assert np.allclose(out["times"], times)
     assert np.allclose(out["measurements"].real, meas.real, atol=1e-8)
+
+
+def test_batched_matches_workers_truncated(ctx):
+    """The dissipative TFIM with truncation (cutoff 1e-8, maxdim 256), 24 trajectories of
+    60 steps from all spins up: the batched engine against the per-trajectory workers on
+    the same uniforms. Both run the same splits (Gram eigensolve, truncation rule, one
+    Newton-Schulz step) in different summation orders, so jump records must agree exactly
+    and measurements to 1e-9."""
+    from tnhip import spinhalf
+    from tnhip.qjmc import qjmc_trajectories
+
+    N, g, gamma, dt, steps = 10, 0.5, 0.1, 0.01, 60
+    H, J = model(N, g, gamma)
+    U = np.random.default_rng(21).random((24, 4 * steps + 8))
+    kw = dict(observables=zlist(N), cutoff=1e-8, maxdim=256)
+    a = qjmc_trajectories(spinhalf(), ["up"] * N, H, J, steps * dt, dt, U, engine="batched", **kw)
+    b = qjmc_trajectories(spinhalf(), ["up"] * N, H, J, steps * dt, dt, U, engine="workers", nworkers=4, **kw)
+    assert sum(a["njumps"]) >= 3  # the case exercises jumps
+    assert a["jumps"] == b["jumps"]
+    assert a["jumptimes"] == b["jumptimes"]
+    assert a["maxbonddim"] == b["maxbonddim"]
+    assert np.abs(a["measurements"] - b["measurements"]).max() <= 1e-9
+
+
+def test_batched_capacity_overflow(ctx):
+    """Without truncation the bond dimension of a 10-site chain reaches 32, above the
+    batched slots (16 for spin-1/2): engine="batched" raises, engine="auto" reruns the
+    trajectories on the per-trajectory workers and returns their result."""
+    from tnhip import spinhalf, TnhError
+    from tnhip.qjmc import qjmc_trajectories
+
+    N, dt, steps = 10, 0.05, 12
+    H, J = model(N, 1.5, 1e-4)
+    U = np.random.default_rng(22).random((3, 4 * steps + 8))
+    kw = dict(observables=zlist(N), cutoff=0.0)
+    with pytest.raises(TnhError, match="bond dimension above"):
+        qjmc_trajectories(spinhalf(), ["up"] * N, H, J, steps * dt, dt, U, engine="batched", **kw)
+    a = qjmc_trajectories(spinhalf(), ["up"] * N, H, J, steps * dt, dt, U, engine="auto", **kw)
+    b = qjmc_trajectories(spinhalf(), ["up"] * N, H, J, steps * dt, dt, U, engine="workers", **kw)
+    assert a["engine"] == "workers"
+    assert max(a["maxbonddim"]) == 32
+    assert a["jumps"] == b["jumps"]
+    assert np.abs(a["measurements"] - b["measurements"]).max() <= 1e-12
 
 
 def test_native_runner_rejects_short_uniforms(ctx):
