@@ -70,7 +70,7 @@ __device__ bool jacobi_lds(double (*gr)[2 * B + 1], double (*gi)[2 * B + 1], dou
                            double (*qi)[2 * B + 1], double tol, double abstol, int sweeps,
                            bool cross = false) {
   constexpr int JP = 2 * B, NS = B;
-  static_assert(NT % NS == 0 && NT / NS <= NS, "thread layout of the step tasks");
+  static_assert(NT % NS == 0, "thread layout of the step tasks");  // threads beyond the tasks idle
   const int t = threadIdx.x;
   __shared__ int rrot[NS];
   __shared__ double rcs[NS], rsn[NS], rer[NS], rei[NS], ra[NS], rc[NS], rtb[NS];

@@ -75,29 +75,37 @@ __device__ int trunc_rank(const double *s, int r, double cutoff, int maxdim, int
   return k;
 }
 
+constexpr int QG = QM * (QM + 1);  // one padded Jacobi matrix plane
+
 struct Lds {
   double Mr[QMM], Mi[QMM];  // the matrix being split, column-major m x n
   double Ar[QMM], Ai[QMM];  // left factor (m x k)
   double Br[QMM], Bi[QMM];  // right factor (k x n)
-  double gr[QM][QM + 1], gi[QM][QM + 1], qr[QM][QM + 1], qi[QM][QM + 1];
-  double val[QM];
+  double gr[QG], gi[QG], qr[QG], qi[QG];  // Jacobi planes, row pitch 2B + 1 of the chosen width
+  double val[QM], sv[QM];  // singular values by index, and in descending order
+  double red[QT / 64];
   int perm[QM];
   int k;
-  int ok;
+  int flag;
 };
 
 // Truncated split of M (m x n, in L.Mr/Mi) as in split_matrix of the SVD module, side 0:
 // A = U_k (orthonormal columns, m x k), B = U_k^H M; side 1: B = V_k^H (orthonormal rows,
-// k x n), A = M V_k. Gram of the smaller-index side, power-of-two rescaling, cyclic Jacobi to
-// convergence, stable descending order, the truncation rule, one Newton-Schulz step on the
-// kept basis. Returns k (0 on failure, with *flag set).
-__device__ int split_lds(Lds &L, int m, int n, int side, double cutoff, int maxdim, int mindim,
-                         int CH, int *flag) {
+// k x n), A = M V_k. Gram of the basis side (p = m or n <= 2B), made exactly Hermitian,
+// power-of-two rescaling, cyclic Jacobi to convergence on the p x p problem padded to
+// 2B x 2B, stable descending order, the truncation rule, one Newton-Schulz step on the kept
+// basis. Returns k (0 on failure, with L.flag set).
+template <int B>
+__device__ int split_lds_w(Lds &L, int m, int n, int side, double cutoff, int maxdim, int mindim, int CH) {
+  constexpr int PB = 2 * B;
+  typedef double Row[PB + 1];
+  Row *gr = reinterpret_cast<Row *>(L.gr), *gi = reinterpret_cast<Row *>(L.gi);
+  Row *qr = reinterpret_cast<Row *>(L.qr), *qi = reinterpret_cast<Row *>(L.qi);
   const int t = threadIdx.x;
   const int p = side == 0 ? m : n;
-  // Hermitian Gram G = M M^H (side 0) or M^H M (side 1), padded with zeros to QM x QM
-  for (int e = t; e < QMM; e += QT) {
-    const int i = e % QM, j = e / QM;
+  // G = M M^H (side 0) or M^H M (side 1), zero-padded to PB x PB
+  for (int e = t; e < PB * PB; e += QT) {
+    const int i = e % PB, j = e / PB;
     double sr = 0, si = 0;
     if (i < p && j < p) {
       if (side == 0) {
@@ -116,30 +124,30 @@ __device__ int split_lds(Lds &L, int m, int n, int side, double cutoff, int maxd
         }
       }
     }
-    L.gr[i][j] = sr;
-    L.gi[i][j] = (i == j) ? 0.0 : si;
-    L.qr[i][j] = (i == j) ? 1.0 : 0.0;
-    L.qi[i][j] = 0.0;
+    gr[i][j] = sr;
+    gi[i][j] = (i == j) ? 0.0 : si;
+    qr[i][j] = (i == j) ? 1.0 : 0.0;
+    qi[i][j] = 0.0;
   }
   __syncthreads();
   // exactly Hermitian, (G + G^H) / 2, as the single-workgroup eigensolve of the SVD module
-  for (int e = t; e < QMM; e += QT) {
-    const int i = e % QM, j = e / QM;
+  for (int e = t; e < PB * PB; e += QT) {
+    const int i = e % PB, j = e / PB;
     if (i >= j) continue;
-    const double re = 0.5 * (L.gr[i][j] + L.gr[j][i]), im = 0.5 * (L.gi[i][j] - L.gi[j][i]);
-    L.gr[i][j] = L.gr[j][i] = re;
-    L.gi[i][j] = im;
-    L.gi[j][i] = -im;
+    const double re = 0.5 * (gr[i][j] + gr[j][i]), im = 0.5 * (gi[i][j] - gi[j][i]);
+    gr[i][j] = gr[j][i] = re;
+    gi[i][j] = im;
+    gi[j][i] = -im;
   }
   __syncthreads();
-  double dm = t < QM ? fabs(L.gr[t][t]) : 0.0;
+  double dm = t < PB ? fabs(gr[t][t]) : 0.0;
   dm = jac::block_max<QT>(dm);
   const int ex = (dm > 0 && isfinite(dm)) ? ilogb(dm) : 0;
   if (ex != 0) {
     const double sc = ldexp(1.0, -ex);
-    for (int e = t; e < QMM; e += QT) {
-      L.gr[e % QM][e / QM] *= sc;
-      L.gi[e % QM][e / QM] *= sc;
+    for (int e = t; e < PB * PB; e += QT) {
+      gr[e % PB][e / PB] *= sc;
+      gi[e % PB][e / PB] *= sc;
     }
     dm *= sc;
     __syncthreads();
@@ -147,8 +155,8 @@ __device__ int split_lds(Lds &L, int m, int n, int side, double cutoff, int maxd
   const double eps = 2.220446049250313e-16;
   const double tol = fmax(1e-14, 4.0 * eps * sqrt(double(p)));
   const double abstol = 4.0 * eps * sqrt(double(p)) * dm;
-  const bool conv = jac::jacobi_lds<true, JB, QT>(L.gr, L.gi, L.qr, L.qi, tol, abstol, kMaxInnerCap);
-  if (t < p) L.val[t] = sqrt(fmax(ldexp(L.gr[t][t], ex), 0.0));
+  const bool conv = jac::jacobi_lds<true, B, QT>(gr, gi, qr, qi, tol, abstol, kMaxInnerCap);
+  if (t < p) L.val[t] = sqrt(fmax(ldexp(gr[t][t], ex), 0.0));
   __syncthreads();
   if (t < p) {  // stable descending rank
     const double v = L.val[t];
@@ -159,30 +167,28 @@ __device__ int split_lds(Lds &L, int m, int n, int side, double cutoff, int maxd
   __syncthreads();
   if (t == 0) {
     int ok = conv ? 1 : 0;
-    double s[QM];
     for (int i = 0; i < p; ++i) {
-      s[i] = L.val[L.perm[i]];
-      if (!isfinite(s[i])) ok = 0;
+      L.sv[i] = L.val[L.perm[i]];
+      if (!isfinite(L.sv[i])) ok = 0;
     }
     double disc = 0;
-    int k = ok ? trunc_rank(s, min(m, n), cutoff, maxdim, mindim, &disc) : 0;
-    if (!conv) *flag |= F_NOCONV;
-    if (conv && !ok) *flag |= F_NUMERIC;
+    int k = ok ? trunc_rank(L.sv, min(m, n), cutoff, maxdim, mindim, &disc) : 0;
+    if (!conv) L.flag |= F_NOCONV;
+    if (conv && !ok) L.flag |= F_NUMERIC;
     if (ok && k > CH) {
-      *flag |= F_OVERFLOW;
+      L.flag |= F_OVERFLOW;
       ok = 0;
     }
     L.k = ok ? k : 0;
-    L.ok = ok;
   }
   __syncthreads();
   const int k = L.k;
   if (k == 0) return 0;
-  // basis U (p x k) = columns perm[0..k) of Q, kept in gr/gi (rows r, column j)
+  // basis U (p x k) = columns perm[0..k) of Q, kept in gr/gi (Q is only read, G no longer)
   for (int e = t; e < p * k; e += QT) {
     const int r = e % p, j = e / p, c = L.perm[j];
-    L.gr[r][j] = L.qr[r][c];  // Q (qr/qi) is only read here, G (gr/gi) no longer
-    L.gi[r][j] = L.qi[r][c];
+    gr[r][j] = qr[r][c];
+    gi[r][j] = qi[r][c];
   }
   __syncthreads();
   // Newton-Schulz: X = U^H U (k x k, in B), W = U X (p x k, in A), U <- 1.5 U - 0.5 W
@@ -190,7 +196,7 @@ __device__ int split_lds(Lds &L, int m, int n, int side, double cutoff, int maxd
     const int i = e % k, j = e / k;
     double sr = 0, si = 0;
     for (int r = 0; r < p; ++r) {  // conj(U[r,i]) U[r,j]
-      const double ar = L.gr[r][i], ai = L.gi[r][i], br = L.gr[r][j], bi = L.gi[r][j];
+      const double ar = gr[r][i], ai = gi[r][i], br = gr[r][j], bi = gi[r][j];
       sr += ar * br + ai * bi;
       si += ar * bi - ai * br;
     }
@@ -202,7 +208,7 @@ __device__ int split_lds(Lds &L, int m, int n, int side, double cutoff, int maxd
     const int r = e % p, j = e / p;
     double sr = 0, si = 0;
     for (int l = 0; l < k; ++l) {  // U[r,l] X[l,j]
-      const double ar = L.gr[r][l], ai = L.gi[r][l], br = L.Br[l + k * j], bi = L.Bi[l + k * j];
+      const double ar = gr[r][l], ai = gi[r][l], br = L.Br[l + k * j], bi = L.Bi[l + k * j];
       sr += ar * br - ai * bi;
       si += ar * bi + ai * br;
     }
@@ -212,21 +218,21 @@ __device__ int split_lds(Lds &L, int m, int n, int side, double cutoff, int maxd
   __syncthreads();
   for (int e = t; e < p * k; e += QT) {
     const int r = e % p, j = e / p;
-    L.gr[r][j] = 1.5 * L.gr[r][j] - 0.5 * L.Ar[e];
-    L.gi[r][j] = 1.5 * L.gi[r][j] - 0.5 * L.Ai[e];
+    gr[r][j] = 1.5 * gr[r][j] - 0.5 * L.Ar[e];
+    gi[r][j] = 1.5 * gi[r][j] - 0.5 * L.Ai[e];
   }
   __syncthreads();
   if (side == 0) {
     // A = U (m x k); B = U^H M (k x n)
     for (int e = t; e < m * k; e += QT) {
-      L.Ar[e] = L.gr[e % m][e / m];
-      L.Ai[e] = L.gi[e % m][e / m];
+      L.Ar[e] = gr[e % m][e / m];
+      L.Ai[e] = gi[e % m][e / m];
     }
     for (int e = t; e < k * n; e += QT) {
       const int j = e % k, c = e / k;
       double sr = 0, si = 0;
       for (int r = 0; r < m; ++r) {  // conj(U[r,j]) M[r,c]
-        const double ar = L.gr[r][j], ai = L.gi[r][j], br = L.Mr[r + m * c], bi = L.Mi[r + m * c];
+        const double ar = gr[r][j], ai = gi[r][j], br = L.Mr[r + m * c], bi = L.Mi[r + m * c];
         sr += ar * br + ai * bi;
         si += ar * bi - ai * br;
       }
@@ -237,14 +243,14 @@ __device__ int split_lds(Lds &L, int m, int n, int side, double cutoff, int maxd
     // B = V_k^H (k x n); A = M V_k (m x k)
     for (int e = t; e < k * n; e += QT) {
       const int j = e % k, c = e / k;
-      L.Br[e] = L.gr[c][j];
-      L.Bi[e] = -L.gi[c][j];
+      L.Br[e] = gr[c][j];
+      L.Bi[e] = -gi[c][j];
     }
     for (int e = t; e < m * k; e += QT) {
       const int r = e % m, j = e / m;
       double sr = 0, si = 0;
       for (int c = 0; c < n; ++c) {  // M[r,c] V[c,j]
-        const double ar = L.Mr[r + m * c], ai = L.Mi[r + m * c], br = L.gr[c][j], bi = L.gi[c][j];
+        const double ar = L.Mr[r + m * c], ai = L.Mi[r + m * c], br = gr[c][j], bi = gi[c][j];
         sr += ar * br - ai * bi;
         si += ar * bi + ai * br;
       }
@@ -254,6 +260,15 @@ __device__ int split_lds(Lds &L, int m, int n, int side, double cutoff, int maxd
   }
   __syncthreads();
   return k;
+}
+
+// the Jacobi width follows the basis dimension: padded sides 4, 8, 16 or 32
+__device__ int split_lds(Lds &L, int m, int n, int side, double cutoff, int maxdim, int mindim, int CH) {
+  const int p = side == 0 ? m : n;
+  if (p <= 4) return split_lds_w<2>(L, m, n, side, cutoff, maxdim, mindim, CH);
+  if (p <= 8) return split_lds_w<4>(L, m, n, side, cutoff, maxdim, mindim, CH);
+  if (p <= 16) return split_lds_w<8>(L, m, n, side, cutoff, maxdim, mindim, CH);
+  return split_lds_w<16>(L, m, n, side, cutoff, maxdim, mindim, CH);
 }
 
 __device__ inline void load_slot(const double *src, int64_t cap, int numel, double *re, double *im) {
@@ -270,25 +285,19 @@ __device__ inline void store_slot(double *dst, int64_t cap, int numel, const dou
   }
 }
 
-// ops[4 i .. 4 i + 3] = {trajectory, site, gate offset into the operator store, direction}:
-// theta = A_s A_{s+1}, theta'[a,o1,o2,b] = sum G[o1,i1,o2,i2] theta[a,i1,i2,b], split with
-// side = direction (gatelist.jl:137-171, no normalisation).
-__global__ __launch_bounds__(QT) void qb_gate_kernel(Batch b, const int *__restrict__ ops) {
-  __shared__ Lds L;
-  const int *o = ops + 4 * blockIdx.x;
-  const int tr = o[0], s = o[1], dir = o[3];
-  const double *G = b.op + 2 * int64_t(o[2]);
+// Two-site gate at (s, s + 1): theta = A_s A_{s+1}, theta'[a,o1,o2,b] = sum G[o1,i1,o2,i2]
+// theta[a,i1,i2,b], split with side = direction (gatelist.jl:137-171, no normalisation).
+__device__ void prim_gate(const Batch &b, Lds &L, int tr, int s, const double *G, int dir) {
   const int d = b.d, t = threadIdx.x;
   int *bd = b.bond + int64_t(tr) * (b.N + 1);
   const int Dl = bd[s - 1], Dm = bd[s], Dr = bd[s + 1];
-  if (b.flags[tr]) return;
   double *sa = slot(b, tr, s), *sb = slot(b, tr, s + 1);
   load_slot(sa, b.cap, Dl * d * Dm, L.Ar, L.Ai);
   load_slot(sb, b.cap, Dm * d * Dr, L.Br, L.Bi);
   __syncthreads();
   const int m = Dl * d, n = d * Dr;
-  // theta[(a,i1),(i2,b)] = sum_c A[(a,i1),c] B[c,(i2,b)], into gr/gi viewed flat
-  double *Tr = &L.gr[0][0], *Ti = &L.gi[0][0];
+  // theta[(a,i1),(i2,b)] = sum_c A[(a,i1),c] B[c,(i2,b)], into the Jacobi planes viewed flat
+  double *Tr = L.gr, *Ti = L.gi;
   for (int e = t; e < m * n; e += QT) {
     const int r = e % m, c = e / m;
     double sr = 0, si = 0;
@@ -318,46 +327,30 @@ __global__ __launch_bounds__(QT) void qb_gate_kernel(Batch b, const int *__restr
     L.Mi[e] = si;
   }
   __syncthreads();
-  __shared__ int flag;
-  if (t == 0) flag = 0;
-  __syncthreads();
-  const int k = split_lds(L, m, n, dir ? 1 : 0, b.cutoff, b.maxdim, b.mindim, b.CH, &flag);
-  if (k == 0) {
-    if (t == 0) b.flags[tr] |= flag;
-    return;
-  }
+  const int k = split_lds(L, m, n, dir ? 1 : 0, b.cutoff, b.maxdim, b.mindim, b.CH);
+  if (k == 0) return;
   store_slot(sa, b.cap, m * k, L.Ar, L.Ai);
   store_slot(sb, b.cap, k * n, L.Br, L.Bi);
   if (t == 0) bd[s] = k;
+  __syncthreads();
 }
 
-// ops[4 i ..] = {trajectory, site, unused, truncate}: P_RIGHT moves the centre from s to s + 1
-// (split A_s with side 0, absorb U^H A_s into A_{s+1}); P_LEFT from s to s - 1 (split A_s with
-// side 1, absorb A_s V_k into A_{s-1}). truncate = 0 is movecenter! without cutoff or maxdim.
-template <int DIR>
-__global__ __launch_bounds__(QT) void qb_move_kernel(Batch b, const int *__restrict__ ops) {
-  __shared__ Lds L;
-  const int *o = ops + 4 * blockIdx.x;
-  const int tr = o[0], s = o[1], trunc = o[3];
+// Centre move by one site: DIR 0 from s to s + 1 (split A_s with side 0, absorb U^H A_s into
+// A_{s+1}); DIR 1 from s to s - 1 (split A_s with side 1, absorb A_s V_k into A_{s-1}).
+// trunc = 0 is movecenter! without cutoff or maxdim.
+__device__ void prim_move(const Batch &b, Lds &L, int tr, int s, int dir, int trunc) {
   const int d = b.d, t = threadIdx.x;
   int *bd = b.bond + int64_t(tr) * (b.N + 1);
-  if (b.flags[tr]) return;
   const int Dl = bd[s - 1], Dr = bd[s];
   double *sa = slot(b, tr, s);
-  const int m = DIR == 0 ? Dl * d : Dl, n = DIR == 0 ? Dr : d * Dr;
+  const int m = dir == 0 ? Dl * d : Dl, n = dir == 0 ? Dr : d * Dr;
   load_slot(sa, b.cap, Dl * d * Dr, L.Mr, L.Mi);
   __syncthreads();
-  __shared__ int flag;
-  if (t == 0) flag = 0;
-  __syncthreads();
-  const int k = split_lds(L, m, n, DIR, trunc ? b.cutoff : 0.0, trunc ? b.maxdim : 0,
-                          trunc ? b.mindim : 1, b.CH, &flag);
-  if (k == 0) {
-    if (t == 0) b.flags[tr] |= flag;
-    return;
-  }
-  double *Pr = &L.gr[0][0], *Pi = &L.gi[0][0];
-  if (DIR == 0) {
+  const int k = split_lds(L, m, n, dir, trunc ? b.cutoff : 0.0, trunc ? b.maxdim : 0, trunc ? b.mindim : 1,
+                          b.CH);
+  if (k == 0) return;
+  double *Pr = L.gr, *Pi = L.gi;
+  if (dir == 0) {
     // A_{s+1} <- B (k x Dr) A_{s+1} (Dr x d Dr2)
     const int Dr2 = bd[s + 1];
     double *sn = slot(b, tr, s + 1);
@@ -402,22 +395,17 @@ __global__ __launch_bounds__(QT) void qb_move_kernel(Batch b, const int *__restr
     store_slot(sp, b.cap, mm * k, Pr, Pi);
     if (t == 0) bd[s - 1] = k;
   }
+  __syncthreads();
 }
 
-// ops[4 i ..] = {trajectory, site, operator offset, unused}: A[a,o,b] <- sum_i O[o,i] A[a,i,b]
-// (applyop!, mps.jl:141-151, as tnh_apply_site_op).
-__global__ __launch_bounds__(QT) void qb_siteop_kernel(Batch b, const int *__restrict__ ops) {
-  __shared__ double Xr[QMM], Xi[QMM];
-  const int *o = ops + 4 * blockIdx.x;
-  const int tr = o[0], s = o[1];
-  const double *O = b.op + 2 * int64_t(o[2]);
+// A[a,o,b] <- sum_i O[o,i] A[a,i,b] (applyop!, mps.jl:141-151, as tnh_apply_site_op).
+__device__ void prim_siteop(const Batch &b, Lds &L, int tr, int s, const double *O) {
   const int d = b.d, t = threadIdx.x;
-  if (b.flags[tr]) return;
   const int *bd = b.bond + int64_t(tr) * (b.N + 1);
   const int Dl = bd[s - 1], Dr = bd[s];
   double *sa = slot(b, tr, s);
   const int ne = Dl * d * Dr;
-  load_slot(sa, b.cap, ne, Xr, Xi);
+  load_slot(sa, b.cap, ne, L.Mr, L.Mi);
   __syncthreads();
   for (int e = t; e < ne; e += QT) {
     const int a = e % Dl, oo = (e / Dl) % d, bb = e / (Dl * d);
@@ -425,39 +413,78 @@ __global__ __launch_bounds__(QT) void qb_siteop_kernel(Batch b, const int *__res
     for (int i = 0; i < d; ++i) {
       const double gr_ = O[2 * (oo + d * i)], gi_ = O[2 * (oo + d * i) + 1];
       const int x = a + Dl * (i + d * bb);
-      sr += gr_ * Xr[x] - gi_ * Xi[x];
-      si += gr_ * Xi[x] + gi_ * Xr[x];
+      sr += gr_ * L.Mr[x] - gi_ * L.Mi[x];
+      si += gr_ * L.Mi[x] + gi_ * L.Mr[x];
     }
     sa[e] = sr;
     sa[b.cap + e] = si;
   }
+  __syncthreads();
 }
 
-// ops[4 i ..] = {trajectory, centre site}: A_c <- A_c / ||A_c|| (normalize!, gmps.jl:46-51).
-__global__ __launch_bounds__(QT) void qb_normalize_kernel(Batch b, const int *__restrict__ ops) {
-  const int *o = ops + 4 * blockIdx.x;
-  const int tr = o[0], s = o[1], d = b.d, t = threadIdx.x;
-  if (b.flags[tr]) return;
+// A_c <- A_c / ||A_c|| (normalize!, gmps.jl:46-51). Returns false on a zero or non-finite norm.
+__device__ bool prim_normalize(const Batch &b, Lds &L, int tr, int s) {
+  const int d = b.d, t = threadIdx.x;
   const int *bd = b.bond + int64_t(tr) * (b.N + 1);
   const int ne = bd[s - 1] * d * bd[s];
   double *sa = slot(b, tr, s);
   double acc = 0;
   for (int e = t; e < ne; e += QT) acc += sa[e] * sa[e] + sa[b.cap + e] * sa[b.cap + e];
-  __shared__ double red[QT / 64];
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  if ((t & 63) == 0) red[t >> 6] = acc;
+  if ((t & 63) == 0) L.red[t >> 6] = acc;
   __syncthreads();
   double tot = 0;
-  for (int w = 0; w < QT / 64; ++w) tot += red[w];
+  for (int w = 0; w < QT / 64; ++w) tot += L.red[w];
+  __syncthreads();
   const double nr = sqrt(tot);
-  if (!(nr > 0) || !isfinite(nr)) {
-    if (t == 0) b.flags[tr] |= F_NUMERIC;
-    return;
-  }
+  if (!(nr > 0) || !isfinite(nr)) return false;
   const double inv = 1.0 / nr;
   for (int e = t; e < ne; e += QT) {
     sa[e] *= inv;
     sa[b.cap + e] *= inv;
+  }
+  __syncthreads();
+  return true;
+}
+
+// One workgroup per listed trajectory runs that trajectory's whole primitive list
+// items[start[w] .. start[w + 1]) in order ({kind, site, arg, flag} each): trajectories are
+// independent, so a phase of the step is a single launch. A failed split or normalisation
+// records its flag and ends that trajectory's list.
+__global__ __launch_bounds__(QT) void qb_run_kernel(Batch b, const int *__restrict__ trajs,
+                                                     const int *__restrict__ start,
+                                                     const int *__restrict__ items) {
+  __shared__ Lds L;
+  const int w = blockIdx.x, tr = trajs[w], t = threadIdx.x;
+  if (t == 0) L.flag = b.flags[tr];
+  __syncthreads();
+  if (L.flag) return;
+  for (int i = start[w]; i < start[w + 1]; ++i) {
+    const int *it = items + 4 * i;
+    const int kind = it[0], s = it[1], arg = it[2], fl = it[3];
+    bool ok = true;
+    switch (kind) {
+      case P_RIGHT:
+        prim_move(b, L, tr, s, 0, fl);
+        break;
+      case P_LEFT:
+        prim_move(b, L, tr, s, 1, fl);
+        break;
+      case P_GATE:
+        prim_gate(b, L, tr, s, b.op + 2 * int64_t(arg), fl);
+        break;
+      case P_SITEOP:
+        prim_siteop(b, L, tr, s, b.op + 2 * int64_t(arg));
+        break;
+      default:
+        ok = prim_normalize(b, L, tr, s);
+    }
+    if (t == 0 && !ok) L.flag |= F_NUMERIC;
+    __syncthreads();
+    if (L.flag) {
+      if (t == 0) b.flags[tr] = L.flag;
+      return;
+    }
   }
 }
 
@@ -697,62 +724,35 @@ struct Engine {
     }
   }
 
-  // run every queue to empty: a round takes the front primitive of each trajectory and
-  // launches one kernel per kind present; all rounds go up in one upload
+  // run every queue to empty in one launch: each trajectory with work gets a workgroup that
+  // walks its own list; the lists of all trajectories go up in one upload
   void run_queues() {
-    std::vector<int> host;
-    struct L {
-      int kind, off, count;
-    };
-    std::vector<L> launches;
-    for (;;) {
-      int before = int(host.size());
-      (void)before;
-      bool any = false;
-      for (int kind = 0; kind < 5; ++kind) {
-        const int off = int(host.size()) / 4;
-        int cnt = 0;
-        for (int t = 0; t < T; ++t) {
-          auto &x = tq[t];
-          if (x.q.empty() || x.q.front().kind != kind) continue;
-          const Item it = x.q.front();
-          host.insert(host.end(), {t, it.site, it.arg, it.flag});
-          ++cnt;
-        }
-        if (cnt) launches.push_back({kind, off, cnt});
-        any |= cnt > 0;
-      }
-      if (!any) break;
-      for (auto &x : tq)
-        if (!x.q.empty()) x.q.pop_front();
+    std::vector<int> host, trajs, start;
+    for (int t = 0; t < T; ++t) {
+      auto &x = tq[t];
+      if (x.q.empty()) continue;
+      trajs.push_back(t);
+      start.push_back(int(host.size() / 4));
+      for (const Item &it : x.q) host.insert(host.end(), {it.kind, it.site, it.arg, it.flag});
+      x.q.clear();
     }
-    if (launches.empty()) return;
-    const size_t bytes = host.size() * sizeof(int);
+    const int nw = int(trajs.size());
+    if (nw == 0) return;
+    start.push_back(int(host.size() / 4));
+    // one buffer: trajs | start | items
+    std::vector<int> buf;
+    buf.reserve(trajs.size() + start.size() + host.size());
+    buf.insert(buf.end(), trajs.begin(), trajs.end());
+    buf.insert(buf.end(), start.begin(), start.end());
+    buf.insert(buf.end(), host.begin(), host.end());
+    const size_t bytes = buf.size() * sizeof(int);
     if (bytes > ops_cap) {  // the caching allocator orders reuse after queued work
       if (ops_dev) ctx->release(ops_dev, ops_cap);
       ops_dev = ctx->alloc(std::max(bytes, ops_cap * 2), &ops_cap);
     }
-    to_device(ctx, ops_dev, host.data(), bytes);
+    to_device(ctx, ops_dev, buf.data(), bytes);
     const int *od = static_cast<const int *>(ops_dev);
-    for (const auto &l : launches) {
-      const int *o = od + 4 * l.off;
-      switch (l.kind) {
-        case P_RIGHT:
-          hipLaunchKernelGGL(qb_move_kernel<0>, dim3(l.count), dim3(QT), 0, ctx->stream, b, o);
-          break;
-        case P_LEFT:
-          hipLaunchKernelGGL(qb_move_kernel<1>, dim3(l.count), dim3(QT), 0, ctx->stream, b, o);
-          break;
-        case P_GATE:
-          hipLaunchKernelGGL(qb_gate_kernel, dim3(l.count), dim3(QT), 0, ctx->stream, b, o);
-          break;
-        case P_SITEOP:
-          hipLaunchKernelGGL(qb_siteop_kernel, dim3(l.count), dim3(QT), 0, ctx->stream, b, o);
-          break;
-        default:
-          hipLaunchKernelGGL(qb_normalize_kernel, dim3(l.count), dim3(QT), 0, ctx->stream, b, o);
-      }
-    }
+    hipLaunchKernelGGL(qb_run_kernel, dim3(nw), dim3(QT), 0, ctx->stream, b, od, od + nw, od + 2 * nw + 1);
     TNH_HIP(hipGetLastError());
   }
 };
