@@ -232,13 +232,15 @@ def gradient_tfim_n64(chi=512, block=8):
             "block": block, "max_abs_diff_vs_inner": float(np.abs(g - hf).max())}
 
 
-def qjmc_tfim_n40(steps=20, ntraj=64, nworkers=8, chi_max=256):
+def qjmc_tfim_n40(ntraj=512, steps=200, sample_traj=16, sample_steps=20, nworkers=8, chi_max=256):
     """Quantum-jump trajectories of the dissipative TFIM, N = 40, g = 0.5, jump operators
     s-_i at rate 0.1, dt = 0.01, second-order Trotter, cutoff 1e-8, chi_max 256, from all
-    spins up, on fixed uniforms (seed 0): `ntraj` trajectories of `steps` steps through the
-    native runner (nworkers threads, one stream each), <sz_i> saved every step; and one
-    trajectory through the Python host loop for comparison. Reports trajectory steps per
-    second (a bounded sample of the 512 x 200 workload: early times, small bond dims)."""
+    spins up, on fixed uniforms (seed 0), <sz_i> saved every step. The full workload
+    (`ntraj` x `steps`) runs on the batched engine (all trajectories per launch); a bounded
+    sample (the first `sample_traj` trajectories, `sample_steps` steps) also runs on the
+    per-trajectory workers (`nworkers` streams), whose jump records must equal the batched
+    engine's on the same uniforms, and one trajectory through the Python host loop.
+    Reports trajectory steps per second for each."""
     from tnhip import productMPS, spinhalf
     from tnhip.models import OpList
     from tnhip.qjmc import qjmc_simulation, qjmc_trajectories
@@ -258,29 +260,40 @@ def qjmc_tfim_n40(steps=20, ntraj=64, nworkers=8, chi_max=256):
         zs.add("z", i)
     st = spinhalf()
     U = np.random.default_rng(0).random((ntraj, 4 * steps + 16))
+    kw = dict(observables=zs, cutoff=1e-8, maxdim=chi_max)
+    # batched engine: warm-up (code paths, allocations), then the full workload
+    qjmc_trajectories(st, ["up"] * N, H, Jo, 2 * dt, dt, U[:8], engine="batched", **kw)
+    t0 = time.perf_counter()
+    full = qjmc_trajectories(st, ["up"] * N, H, Jo, steps * dt, dt, U, engine="batched", **kw)
+    el_full = time.perf_counter() - t0
+    # bounded sample on both engines
+    Us = U[:sample_traj]
+    a = qjmc_trajectories(st, ["up"] * N, H, Jo, sample_steps * dt, dt, Us, engine="batched", **kw)
+    t0 = time.perf_counter()
+    w = qjmc_trajectories(st, ["up"] * N, H, Jo, sample_steps * dt, dt, Us, engine="workers",
+                          nworkers=nworkers, **kw)
+    el_w = time.perf_counter() - t0
     # host loop, one trajectory
     psi = DeviceMPS.from_gmps(productMPS(st, ["up"] * N))
     u = iter(U[0])
     t0 = time.perf_counter()
-    jumps, _ = qjmc_simulation(st, psi, H, Jo, steps * dt, dt, [], cutoff=1e-8, maxdim=chi_max, verbose=False,
-                               rand=lambda: float(next(u)))
+    jumps, _ = qjmc_simulation(st, psi, H, Jo, sample_steps * dt, dt, [], cutoff=1e-8, maxdim=chi_max,
+                               verbose=False, rand=lambda: float(next(u)))
     psi.ctx.synchronize()
     host_el = time.perf_counter() - t0
-    # native runner, ntraj trajectories
-    qjmc_trajectories(st, ["up"] * N, H, Jo, 2 * dt, dt, U[:nworkers], observables=zs, cutoff=1e-8,
-                      maxdim=chi_max, nworkers=nworkers)  # warm-up: worker contexts, code paths
-    t0 = time.perf_counter()
-    out = qjmc_trajectories(st, ["up"] * N, H, Jo, steps * dt, dt, U, observables=zs, cutoff=1e-8,
-                            maxdim=chi_max, nworkers=nworkers)
-    el = time.perf_counter() - t0
-    rate = ntraj * steps / el
-    return {"native": {"seconds": el, "trajectories": ntraj, "steps": steps, "workers": nworkers,
-                       "trajectory_steps_per_s": rate, "jumps": int(sum(out["njumps"])),
-                       "maxbonddim": int(max(out["maxbonddim"])),
-                       "sz_avg_final": float(out["measurements"][:, -1, :].real.mean())},
-            "host_loop": {"seconds": host_el, "steps": steps, "trajectory_steps_per_s": steps / host_el,
-                          "jumps": len(jumps)},
-            "projected_512x200_hours": 512 * 200 / rate / 3600}
+    sz = full["measurements"].real.mean(axis=(0, 2))
+    return {"batched": {"seconds": el_full, "trajectories": ntraj, "steps": steps,
+                        "trajectory_steps_per_s": ntraj * steps / el_full, "jumps": int(sum(full["njumps"])),
+                        "maxbonddim": int(max(full["maxbonddim"])), "sz_avg_t0": float(sz[0]),
+                        "sz_avg_final": float(sz[-1])},
+            "workers_sample": {"seconds": el_w, "trajectories": sample_traj, "steps": sample_steps,
+                               "workers": nworkers, "trajectory_steps_per_s": sample_traj * sample_steps / el_w,
+                               "jump_records_equal_batched": w["jumps"] == a["jumps"],
+                               "max_abs_meas_diff_vs_batched":
+                                   float(np.abs(w["measurements"] - a["measurements"]).max())},
+            "host_loop": {"seconds": host_el, "steps": sample_steps,
+                          "trajectory_steps_per_s": sample_steps / host_el, "jumps": len(jumps),
+                          "jump_records_equal_batched": [j for j in jumps] == a["jumps"][0]}}
 
 
 def bond_step(ctx, ops, Tensor, dL, dR, dW, dx, prec, chi, torch):
