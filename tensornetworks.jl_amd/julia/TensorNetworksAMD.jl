@@ -248,6 +248,57 @@ end
 #       return real(E), Ē -> (NoTangent(), Ē .* real.(g), NoTangent(), NoTangent())
 #   end
 
-export Context, GPUProjMPS, dmrg_gpu, mpo_energy_grad
+# ------------------------------- many QJMC trajectories (tnh_qjmc_run(_batched))
+const TNH_ERR_UNSUPPORTED = Cint(-7)
+
+# Layout of tnh_qjmc_problem (include/tnhip.h); the pointers refer to arrays the caller
+# keeps alive (GC.@preserve) for the duration of the call.
+struct QjmcProblem
+    N::Cint; d::Cint
+    nlayers::Cint
+    layer_count::Ptr{Cint}; gate_site::Ptr{Cint}; gate_range::Ptr{Cint}
+    gate_offset::Ptr{Int64}; gate_data::Ptr{Float64}
+    njump::Cint
+    jump_site::Ptr{Cint}; jump_coeff::Ptr{Float64}; jump_op::Ptr{Float64}
+    nobs::Cint
+    obs_site::Ptr{Cint}; obs_op::Ptr{Float64}
+    init_state::Ptr{Float64}
+    dt::Cdouble
+    nsteps::Cint; savesteps::Cint
+    cutoff::Cdouble
+    maxdim::Cint; mindim::Cint
+end
+
+"""
+    qjmc_trajectories!(prob, U, meas, njumps, jumps, jsteps, maxb; batched=true)
+
+Many trajectories of qjmc_simulation (src/algorithms/mps/qjmc.jl:28-167, classical branch):
+U[:, t] holds trajectory t's uniforms in the reference's rand() order. `batched` advances all
+trajectories together (bond dimension <= min(16, 32 / d)) and falls back to the
+per-trajectory runner when a bond outgrows that.
+"""
+function qjmc_trajectories!(prob::Ref{QjmcProblem}, U::Matrix{Float64}, meas::Array{Float64},
+                            njumps::Vector{Cint}, jumps::Matrix{Cint}, jsteps::Matrix{Cint},
+                            maxb::Vector{Cint}; batched::Bool = true, nworkers::Int = 4,
+                            ctx::Context = default_context())
+    nu, ntraj = size(U)
+    maxjumps = size(jumps, 1)
+    if batched
+        rc = ccall((:tnh_qjmc_run_batched, libtnhip), Cint,
+                   (Ptr{Cvoid}, Ptr{QjmcProblem}, Cint, Ptr{Float64}, Int64, Ptr{Float64},
+                    Ptr{Cint}, Ptr{Cint}, Ptr{Cint}, Cint, Ptr{Cint}),
+                   ctx.h, prob, ntraj, U, nu, meas, njumps, jumps, jsteps, maxjumps, maxb)
+        rc == 0 && return nothing
+        rc == TNH_ERR_UNSUPPORTED || check(rc, ctx.h, "tnh_qjmc_run_batched")
+    end
+    check(ccall((:tnh_qjmc_run, libtnhip), Cint,
+                (Ptr{Cvoid}, Ptr{QjmcProblem}, Cint, Ptr{Float64}, Int64, Cint, Ptr{Float64},
+                 Ptr{Cint}, Ptr{Cint}, Ptr{Cint}, Cint, Ptr{Cint}),
+                ctx.h, prob, ntraj, U, nu, nworkers, meas, njumps, jumps, jsteps, maxjumps, maxb),
+          ctx.h, "tnh_qjmc_run")
+    return nothing
+end
+
+export Context, GPUProjMPS, dmrg_gpu, mpo_energy_grad, QjmcProblem, qjmc_trajectories!
 
 end # module
