@@ -266,13 +266,19 @@ def qjmc_tfim_n40(ntraj=512, steps=200, sample_traj=16, sample_steps=20, nworker
     t0 = time.perf_counter()
     full = qjmc_trajectories(st, ["up"] * N, H, Jo, steps * dt, dt, U, engine="batched", **kw)
     el_full = time.perf_counter() - t0
-    # bounded sample on both engines
-    Us = U[:sample_traj]
+    # bounded sample on both engines (sample_traj = 0: batched engine only, one host thread)
+    Us = U[:max(sample_traj, 1)]
     a = qjmc_trajectories(st, ["up"] * N, H, Jo, sample_steps * dt, dt, Us, engine="batched", **kw)
-    t0 = time.perf_counter()
-    w = qjmc_trajectories(st, ["up"] * N, H, Jo, sample_steps * dt, dt, Us, engine="workers",
-                          nworkers=nworkers, **kw)
-    el_w = time.perf_counter() - t0
+    res_w = None
+    if sample_traj > 0:
+        t0 = time.perf_counter()
+        w = qjmc_trajectories(st, ["up"] * N, H, Jo, sample_steps * dt, dt, Us, engine="workers",
+                              nworkers=nworkers, **kw)
+        el_w = time.perf_counter() - t0
+        res_w = {"seconds": el_w, "trajectories": sample_traj, "steps": sample_steps, "workers": nworkers,
+                 "trajectory_steps_per_s": sample_traj * sample_steps / el_w,
+                 "jump_records_equal_batched": w["jumps"] == a["jumps"],
+                 "max_abs_meas_diff_vs_batched": float(np.abs(w["measurements"] - a["measurements"]).max())}
     # host loop, one trajectory
     psi = DeviceMPS.from_gmps(productMPS(st, ["up"] * N))
     u = iter(U[0])
@@ -286,11 +292,7 @@ def qjmc_tfim_n40(ntraj=512, steps=200, sample_traj=16, sample_steps=20, nworker
                         "trajectory_steps_per_s": ntraj * steps / el_full, "jumps": int(sum(full["njumps"])),
                         "maxbonddim": int(max(full["maxbonddim"])), "sz_avg_t0": float(sz[0]),
                         "sz_avg_final": float(sz[-1])},
-            "workers_sample": {"seconds": el_w, "trajectories": sample_traj, "steps": sample_steps,
-                               "workers": nworkers, "trajectory_steps_per_s": sample_traj * sample_steps / el_w,
-                               "jump_records_equal_batched": w["jumps"] == a["jumps"],
-                               "max_abs_meas_diff_vs_batched":
-                                   float(np.abs(w["measurements"] - a["measurements"]).max())},
+            "workers_sample": res_w,
             "host_loop": {"seconds": host_el, "steps": sample_steps,
                           "trajectory_steps_per_s": sample_steps / host_el, "jumps": len(jumps),
                           "jump_records_equal_batched": [j for j in jumps] == a["jumps"][0]}}
@@ -370,6 +372,8 @@ def main(argv=None):
     ap.add_argument("--dmrg-small-only", action="store_true",
                     help="run only the N=32 Heisenberg DMRG leg (GPU, and the CPU port unless --no-cpu)")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--qjmc-only", action="store_true",
+                    help="run only the QJMC leg on the batched engine and the host loop (one host thread)")
     ap.add_argument("--bond-only", action="store_true",
                     help="run only the chi=2048 DMRG bond step (FP64 and FP32 matvecs)")
     ap.add_argument("--svd-only", action="store_true",
@@ -408,6 +412,9 @@ def main(argv=None):
         return 0
     if a.dmrg_small_only:
         print(json.dumps({"dmrg_heisenberg_N32": dmrg_heisenberg_n32(cpu=not a.no_cpu)}))
+        return 0
+    if a.qjmc_only:
+        print(json.dumps({"qjmc_tfim_N40": qjmc_tfim_n40(sample_traj=0)}))
         return 0
     if a.legs_only:
         print(json.dumps({"gradient_tfim_N64": gradient_tfim_n64(), "qjmc_tfim_N40": qjmc_tfim_n40()}))
