@@ -613,18 +613,21 @@ __global__ void gather_cols_adjoint_kernel(const double *src, int rows, int64_t 
   }
 }
 
-// The whole eigenproblem of a Hermitian p x p Gram matrix (p <= JP) in one workgroup: G is
-// made exactly Hermitian in LDS, the noise floor 4 eps sqrt(p) max_i G_ii is taken from its
-// diagonal, the cyclic Jacobi runs until a sweep rotates nothing, and V (p x p, planar), the
-// eigenvalues out[0..p) and the converged flag out[p] (1 or 0) are written.
-template <bool CPLX>
-__global__ __launch_bounds__(256) void tiny_eig_kernel(const double *__restrict__ G, int p, int64_t imG,
-                                                       double tol, double *__restrict__ V, int64_t imV,
-                                                       double *__restrict__ out) {
-  __shared__ double gr[JP][JP + 1], gi[CPLX ? JP : 1][JP + 1], qr[JP][JP + 1], qi[CPLX ? JP : 1][JP + 1];
+// The whole eigenproblem of a Hermitian p x p Gram matrix (p <= 2B) in one workgroup of NT
+// threads: G is made exactly Hermitian in LDS and padded with zeros to 2B x 2B, the noise
+// floor 4 eps sqrt(p) max_i G_ii is taken from its diagonal, the cyclic Jacobi runs until a
+// sweep rotates nothing, and V (p x p, planar), the eigenvalues out[0..p) and the converged
+// flag out[p] (1 or 0) are written. The host picks the smallest width that holds p (a sweep
+// is 2B - 1 steps).
+template <bool CPLX, int B, int NT>
+__global__ __launch_bounds__(NT) void tiny_eig_kernel(const double *__restrict__ G, int p, int64_t imG,
+                                                      double tol, double *__restrict__ V, int64_t imV,
+                                                      double *__restrict__ out) {
+  constexpr int PB = 2 * B;
+  __shared__ double gr[PB][PB + 1], gi[CPLX ? PB : 1][PB + 1], qr[PB][PB + 1], qi[CPLX ? PB : 1][PB + 1];
   const int t = threadIdx.x;
-  for (int e = t; e < JP * JP; e += 256) {
-    const int i = e % JP, j = e / JP;
+  for (int e = t; e < PB * PB; e += NT) {
+    const int i = e % PB, j = e / PB;
     double vr = 0, vi = 0;
     if (i < p && j < p) {
       vr = 0.5 * (G[i + int64_t(j) * p] + G[j + int64_t(i) * p]);
@@ -638,29 +641,42 @@ __global__ __launch_bounds__(256) void tiny_eig_kernel(const double *__restrict_
     }
   }
   __syncthreads();
-  double dm = t < JP ? fabs(gr[t][t]) : 0.0;
-  dm = block_max(dm);
+  double dm = t < PB ? fabs(gr[t][t]) : 0.0;
+  dm = block_max<NT>(dm);
   // exact power-of-two scaling to max_i G_ii in [1, 2): the rotation test forms G_ii G_jj,
   // which would overflow (or underflow) for Gram matrices of very large (or small) norm
   const int ex = (dm > 0 && isfinite(dm)) ? ilogb(dm) : 0;
   if (ex != 0) {
     const double sc = ldexp(1.0, -ex);
-    for (int e = t; e < JP * JP; e += 256) {
-      gr[e % JP][e / JP] *= sc;
-      if (CPLX) gi[e % JP][e / JP] *= sc;
+    for (int e = t; e < PB * PB; e += NT) {
+      gr[e % PB][e / PB] *= sc;
+      if (CPLX) gi[e % PB][e / PB] *= sc;
     }
     dm *= sc;
     __syncthreads();
   }
   const double abstol = 4.0 * 2.220446049250313e-16 * sqrt(double(p)) * dm;
-  const bool conv = jacobi_lds<CPLX, JB, 256>(gr, gi, qr, qi, tol, abstol, kMaxInnerCap);
-  for (int e = t; e < p * p; e += 256) {
+  const bool conv = jacobi_lds<CPLX, B, NT>(gr, gi, qr, qi, tol, abstol, kMaxInnerCap);
+  for (int e = t; e < p * p; e += NT) {
     const int i = e % p, j = e / p;
     V[i + int64_t(j) * p] = qr[i][j];
     if (CPLX) V[imV + i + int64_t(j) * p] = qi[i][j];
   }
   if (t < p) out[t] = ldexp(gr[t][t], ex);
   if (t == 0) out[p] = conv ? 1.0 : 0.0;
+}
+
+template <bool CPLX>
+void launch_tiny_eig(hipStream_t st, const double *G, int p, int64_t imG, double tol, double *V,
+                     int64_t imV, double *out) {
+  if (p <= 4)
+    hipLaunchKernelGGL((tiny_eig_kernel<CPLX, 2, 64>), dim3(1), dim3(64), 0, st, G, p, imG, tol, V, imV, out);
+  else if (p <= 8)
+    hipLaunchKernelGGL((tiny_eig_kernel<CPLX, 4, 64>), dim3(1), dim3(64), 0, st, G, p, imG, tol, V, imV, out);
+  else if (p <= 16)
+    hipLaunchKernelGGL((tiny_eig_kernel<CPLX, 8, 64>), dim3(1), dim3(64), 0, st, G, p, imG, tol, V, imV, out);
+  else
+    hipLaunchKernelGGL((tiny_eig_kernel<CPLX, 16, 256>), dim3(1), dim3(256), 0, st, G, p, imG, tol, V, imV, out);
 }
 
 struct Schedule {
@@ -896,11 +912,9 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
       // the whole problem fits one workgroup: a single launch and a single download
       double *dout = static_cast<double *>(ctx->workspace(616, sizeof(double) * (p + 1)));
       if (cplx)
-        hipLaunchKernelGGL(tiny_eig_kernel<true>, dim3(1), dim3(256), 0, ctx->stream, G, p, pp, tol, V,
-                           pp, dout);
+        launch_tiny_eig<true>(ctx->stream, G, p, pp, tol, V, pp, dout);
       else
-        hipLaunchKernelGGL(tiny_eig_kernel<false>, dim3(1), dim3(256), 0, ctx->stream, G, p, pp, tol, V,
-                           pp, dout);
+        launch_tiny_eig<false>(ctx->stream, G, p, pp, tol, V, pp, dout);
       TNH_HIP(hipGetLastError());
       std::vector<double> h(p + 1);
       to_host(ctx, h.data(), dout, sizeof(double) * (p + 1));
