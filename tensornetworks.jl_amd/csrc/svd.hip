@@ -675,8 +675,10 @@ void launch_tiny_eig(hipStream_t st, const double *G, int p, int64_t imG, double
     hipLaunchKernelGGL((tiny_eig_kernel<CPLX, 4, 64>), dim3(1), dim3(64), 0, st, G, p, imG, tol, V, imV, out);
   else if (p <= 16)
     hipLaunchKernelGGL((tiny_eig_kernel<CPLX, 8, 64>), dim3(1), dim3(64), 0, st, G, p, imG, tol, V, imV, out);
-  else
+  else if (p <= 32)
     hipLaunchKernelGGL((tiny_eig_kernel<CPLX, 16, 256>), dim3(1), dim3(256), 0, st, G, p, imG, tol, V, imV, out);
+  else  // p <= 64: 2 x 64 x 65 doubles of LDS per plane pair (133 KB complex)
+    hipLaunchKernelGGL((tiny_eig_kernel<CPLX, 32, 1024>), dim3(1), dim3(1024), 0, st, G, p, imG, tol, V, imV, out);
 }
 
 struct Schedule {
@@ -908,7 +910,7 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
       A.conj = true;
     }
     cgemm(ctx, g, A, B, G, cplx ? G + pp : nullptr, TNH_F64, TNH_PREC_F64);
-    if (p <= JP) {
+    if (p <= 2 * JP) {
       // the whole problem fits one workgroup: a single launch and a single download
       double *dout = static_cast<double *>(ctx->workspace(616, sizeof(double) * (p + 1)));
       if (cplx)
