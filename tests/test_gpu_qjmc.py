@@ -134,6 +134,49 @@ def test_batched_capacity_overflow(ctx):
     assert np.abs(a["measurements"] - b["measurements"]).max() <= 1e-12
 
 
+def test_batched_matches_workers_spin_one(ctx):
+    """d = 3 (spin-1: slot capacity 10, Gram matrices up to 30 x 30): spin-1 XXZ-like chain
+    with a transverse field and S- decay, truncated to maxdim 8. The batched engine against
+    the per-trajectory workers (the library's general operators) on the same uniforms:
+    identical jump records and bond dimensions, measurements to 1e-9. The dense oracle is
+    spin-1/2 only; the spin-1/2 engines are pinned to it above."""
+    from tnhip import OpList
+    from tnhip.models import Sitetypes
+    from tnhip.qjmc import qjmc_trajectories
+
+    s2 = 2 ** -0.5
+    st = Sitetypes(3)
+    st.add_state("up", [1, 0, 0])
+    st.add_op("id", np.eye(3), "id")
+    st.add_op("sz", np.diag([1.0, 0.0, -1.0]), "sz")
+    st.add_op("sx", s2 * np.array([[0, 1, 0], [1, 0, 1], [0, 1, 0]]), "sx")
+    st.add_op("s+", 2 ** 0.5 * np.array([[0, 1, 0], [0, 0, 1], [0, 0, 0]]), "s-")
+    st.add_op("s-", 2 ** 0.5 * np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]]), "s+")
+    N, dt, steps = 6, 0.02, 40
+    H = OpList(N)
+    for i in range(1, N):
+        H.add(["sz", "sz"], [i, i + 1], 1.0)
+        H.add(["sx", "sx"], [i, i + 1], 0.5)
+    for i in range(1, N + 1):
+        H.add("sx", i, 0.7)
+    J = OpList(N)
+    for i in range(1, N + 1):
+        J.add("s-", i, 0.4)
+    obs = OpList(N)
+    for i in range(1, N + 1):
+        obs.add("sz", i)
+    U = np.random.default_rng(31).random((12, 4 * steps + 8))
+    kw = dict(observables=obs, cutoff=1e-10, maxdim=8)
+    a = qjmc_trajectories(st, ["up"] * N, H, J, steps * dt, dt, U, engine="batched", **kw)
+    b = qjmc_trajectories(st, ["up"] * N, H, J, steps * dt, dt, U, engine="workers", nworkers=3, **kw)
+    assert a["engine"] == "batched"
+    assert sum(a["njumps"]) >= 3
+    assert max(a["maxbonddim"]) >= 4  # entangled, truncated states
+    assert a["jumps"] == b["jumps"]
+    assert a["maxbonddim"] == b["maxbonddim"]
+    assert np.abs(a["measurements"] - b["measurements"]).max() <= 1e-9
+
+
 def test_native_runner_rejects_short_uniforms(ctx):
     from tnhip import spinhalf, TnhError
     from tnhip.qjmc import qjmc_trajectories
