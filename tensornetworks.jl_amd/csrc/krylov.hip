@@ -83,10 +83,18 @@ __global__ void mdot_final_kernel(const double *__restrict__ part, int nblk, int
   }
 }
 
-// w -= sum_i c_i v_i
-__global__ __launch_bounds__(256) void vsub_kernel(double *__restrict__ w,
-                                                   const double *__restrict__ V, int64_t vstride,
-                                                   int nv, Coef c, int64_t n, bool cplx) {
+// w -= sum_i c_i v_i with the coefficients read from device memory (c[2 i], c[2 i + 1] = re,
+// im, as mdot_final_kernel writes them), so the Gram-Schmidt pass needs no host round trip
+__global__ __launch_bounds__(256) void vsub_dev_kernel(double *__restrict__ w,
+                                                       const double *__restrict__ V, int64_t vstride,
+                                                       int nv, const double *__restrict__ c, int64_t n,
+                                                       bool cplx) {
+  __shared__ double cr[kMaxVec], ci[kMaxVec];
+  if (threadIdx.x < nv) {
+    cr[threadIdx.x] = c[2 * threadIdx.x];
+    ci[threadIdx.x] = cplx ? c[2 * threadIdx.x + 1] : 0.0;
+  }
+  __syncthreads();
   for (int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; k < n;
        k += int64_t(gridDim.x) * blockDim.x) {
     double ar = w[k], ai = cplx ? w[n + k] : 0.0;
@@ -94,15 +102,27 @@ __global__ __launch_bounds__(256) void vsub_kernel(double *__restrict__ w,
       const double vr = V[i * vstride + k];
       if (cplx) {
         const double vi = V[i * vstride + n + k];
-        ar -= c.re[i] * vr - c.im[i] * vi;
-        ai -= c.re[i] * vi + c.im[i] * vr;
+        ar -= cr[i] * vr - ci[i] * vi;
+        ai -= cr[i] * vi + ci[i] * vr;
       } else {
-        ar -= c.re[i] * vr;
+        ar -= cr[i] * vr;
       }
     }
     w[k] = ar;
     if (cplx) w[n + k] = ai;
   }
+}
+
+// dst = src / sqrt(nsq[0]) over m values (dst = 0 when nsq[0] is not positive and finite:
+// a breakdown the host detects afterwards, whose vector is then never used)
+__global__ __launch_bounds__(256) void normalize_copy_kernel(double *__restrict__ dst,
+                                                             const double *__restrict__ src,
+                                                             const double *__restrict__ nsq, int64_t m) {
+  const double q = nsq[0];
+  const double inv = (q > 0 && isfinite(q)) ? 1.0 / sqrt(q) : 0.0;
+  for (int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; k < m;
+       k += int64_t(gridDim.x) * blockDim.x)
+    dst[k] = src[k] * inv;
 }
 
 // out = scale * sum_i y_i v_i
@@ -240,6 +260,10 @@ double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, in
   double *Wv = V + vstride * (kdim + 1);
   double *part = static_cast<double *>(ctx->workspace(701, sizeof(double) * 2 * kDotBlocks * kMaxVec));
   double *dres = static_cast<double *>(ctx->workspace(702, sizeof(double) * 2 * kMaxVec));
+  // per Krylov step j: the two Gram-Schmidt coefficient passes (2 x 2 kMaxVec values) and
+  // ||w||^2 (2 values), kept on the device until the cycle ends
+  constexpr int kStep = 4 * kMaxVec + 2;
+  double *hdev = static_cast<double *>(ctx->workspace(703, sizeof(double) * kStep * kMaxVec));
   double *hres = ctx->host_scalars;
   const int g = grid1(ctx, n);
 
@@ -248,15 +272,6 @@ double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, in
     b.ptr = p;
     b.owned = false;
     return b;
-  };
-  // <v_i, w> for i < nv, to the host
-  auto dots = [&](int nv, const double *w) {
-    hipLaunchKernelGGL(mdot_kernel, dim3(kDotBlocks, nv), dim3(256), 0, ctx->stream, V, vstride, w,
-                       n, cplx, part);
-    hipLaunchKernelGGL(mdot_final_kernel, dim3(nv), dim3(64), 0, ctx->stream, part, kDotBlocks,
-                       nv, dres);
-    TNH_HIP(hipGetLastError());
-    to_host(ctx, hres, dres, sizeof(double) * 2 * nv);
   };
   auto norm_of = [&](const double *w) {
     // ||w||^2 via the same kernel with V replaced by w itself
@@ -286,23 +301,42 @@ double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, in
   for (int it = 0; it < maxiter; ++it) {
     m = kdim;
     bool breakdown = false;
+    // The cycle runs on the device without host round trips: both Gram-Schmidt passes take
+    // their coefficients from device memory and V[j+1] = w / ||w|| is formed there. The host
+    // then replays the projected matrix and the breakdown test of each step in order, and
+    // steps after a breakdown (whose vectors are never used) are not counted.
     for (int j = start; j < kdim; ++j) {
       tnh_buf vj = view(V + j * vstride), wb = view(Wv);
       matvec(&vj, &wb);
-      ++nmv;
+      double *hj = hdev + int64_t(j) * kStep;
       for (int pass = 0; pass < 2; ++pass) {
-        dots(j + 1, Wv);
-        Coef c{};
-        for (int i = 0; i <= j; ++i) {
-          c.re[i] = hres[2 * i];
-          c.im[i] = cplx ? hres[2 * i + 1] : 0.0;
-          h(i, j) += cd(c.re[i], c.im[i]);
-        }
-        hipLaunchKernelGGL(vsub_kernel, dim3(g), dim3(256), 0, ctx->stream, Wv, V, vstride, j + 1, c,
+        double *cj = hj + pass * 2 * kMaxVec;
+        hipLaunchKernelGGL(mdot_kernel, dim3(kDotBlocks, j + 1), dim3(256), 0, ctx->stream, V, vstride,
+                           Wv, n, cplx, part);
+        hipLaunchKernelGGL(mdot_final_kernel, dim3(j + 1), dim3(64), 0, ctx->stream, part, kDotBlocks,
+                           j + 1, cj);
+        hipLaunchKernelGGL(vsub_dev_kernel, dim3(g), dim3(256), 0, ctx->stream, Wv, V, vstride, j + 1, cj,
                            n, cplx);
-        TNH_HIP(hipGetLastError());
       }
-      const double beta = norm_of(Wv);
+      double *nj = hj + 4 * kMaxVec;
+      hipLaunchKernelGGL(mdot_kernel, dim3(kDotBlocks, 1), dim3(256), 0, ctx->stream, Wv, vstride, Wv, n,
+                         cplx, part);
+      hipLaunchKernelGGL(mdot_final_kernel, dim3(1), dim3(64), 0, ctx->stream, part, kDotBlocks, 1, nj);
+      hipLaunchKernelGGL(normalize_copy_kernel, dim3(g), dim3(256), 0, ctx->stream, V + (j + 1) * vstride,
+                         Wv, nj, vstride);
+      TNH_HIP(hipGetLastError());
+    }
+    std::vector<double> hh(size_t(kStep) * (kdim - start));
+    to_host(ctx, hh.data(), hdev + int64_t(start) * kStep, sizeof(double) * hh.size());
+    for (int j = start; j < kdim; ++j) {
+      const double *hj = hh.data() + size_t(j - start) * kStep;
+      ++nmv;
+      for (int pass = 0; pass < 2; ++pass)
+        for (int i = 0; i <= j; ++i) {
+          const double *cj = hj + pass * 2 * kMaxVec;
+          h(i, j) += cd(cj[2 * i], cplx ? cj[2 * i + 1] : 0.0);
+        }
+      const double beta = std::sqrt(std::max(0.0, hj[4 * kMaxVec]));
       TNH_REQUIRE(std::isfinite(beta), TNH_ERR_NUMERIC, "eigsolve: non-finite Krylov vector");
       h(j + 1, j) = beta;
       double scale_h = 1.0;
@@ -313,8 +347,6 @@ double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, in
         breakdown = true;
         break;
       }
-      copy_device(ctx, V + (j + 1) * vstride, Wv, sizeof(double) * vstride);
-      scale(ctx, V + (j + 1) * vstride, TNH_F64, vstride, 1.0 / beta);
     }
     std::vector<cd> Hm(size_t(m) * m);
     for (int j = 0; j < m; ++j)
@@ -331,11 +363,13 @@ double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, in
     }
     hipLaunchKernelGGL(lincomb_kernel, dim3(g), dim3(256), 0, ctx->stream, Wv, V, vstride, m, yc, n,
                        cplx);
+    hipLaunchKernelGGL(mdot_kernel, dim3(kDotBlocks, 1), dim3(256), 0, ctx->stream, Wv, vstride, Wv, n,
+                       cplx, part);
+    hipLaunchKernelGGL(mdot_final_kernel, dim3(1), dim3(64), 0, ctx->stream, part, kDotBlocks, 1, dres);
     TNH_HIP(hipGetLastError());
-    const double rn = norm_of(Wv);
     copy_device(ctx, V + vstride, V + int64_t(m) * vstride, sizeof(double) * vstride);
-    copy_device(ctx, V, Wv, sizeof(double) * vstride);
-    scale(ctx, V, TNH_F64, vstride, 1.0 / rn);
+    hipLaunchKernelGGL(normalize_copy_kernel, dim3(g), dim3(256), 0, ctx->stream, V, Wv, dres, vstride);
+    TNH_HIP(hipGetLastError());
     std::fill(H.begin(), H.end(), cd(0));
     h(0, 0) = theta_e;
     h(1, 0) = b;
