@@ -86,17 +86,27 @@ __device__ bool jacobi_lds(double (*gr)[2 * B + 1], double (*gi)[2 * B + 1], dou
         step_pair<B>(cross, step, t, &p, &q);
         const double a = gr[p][p], c = gr[q][q];
         const double br = gr[p][q], bi = CPLX ? gi[p][q] : 0.0;
-        const double bb = CPLX ? hypot(br, bi) : fabs(br);
-        const bool rot = bb > abstol && bb > tol * sqrt(fabs(a * c));
-        double cs = 1, sn = 0, tt = 0, er = 1, ei = 0;
+        // the thresholds |b| > abstol and |b| > tol sqrt(|a c|), compared squared (callers
+        // scale G to max_i G_ii ~ 1, so no square overflows)
+        const double b2 = CPLX ? br * br + bi * bi : br * br;
+        const bool rot = b2 > abstol * abstol && b2 > tol * tol * fabs(a * c);
+        double cs = 1, sn = 0, tt = 0, er = 1, ei = 0, bb = 0;
         if (rot) {
-          const double ib = 1.0 / bb;
-          const double tau = 0.5 * (c - a) * ib;
-          tt = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1 + tau * tau));
+          // t = sign(c - a) / (|tau| + sqrt(1 + tau^2)) with tau = (c - a) / (2 |b|), in the
+          // form 2 |b| sign(c - a) / (|c - a| + sqrt((c - a)^2 + 4 |b|^2)): one square root
+          // and one division fewer on the step's critical path
+          bb = CPLX ? sqrt(b2) : fabs(br);
+          const double dd = c - a;
+          tt = (dd >= 0 ? 2.0 : -2.0) * bb / (fabs(dd) + sqrt(dd * dd + 4.0 * b2));
           cs = rsqrt(1 + tt * tt);
           sn = tt * cs;
-          er = br * ib;  // e^{-i phi} = conj(b) / |b|
-          ei = -bi * ib;
+          if (CPLX) {  // e^{-i phi} = conj(b) / |b|
+            const double ib = 1.0 / bb;
+            er = br * ib;
+            ei = -bi * ib;
+          } else {
+            er = br >= 0 ? 1.0 : -1.0;
+          }
           anyrot[sw] = 1;
         }
         rrot[t] = rot;
