@@ -18,9 +18,10 @@
 // LDS layouts follow the operand's global layout, so nothing is transposed in flight:
 //   k-contiguous operand  -> [row][k], row stride BK + KPAD: a fragment is one 16-byte
 //                            LDS read (ds_read_b128);
-//   row-contiguous operand -> [k][row], row stride ROWS + 16 elements: the 16 lanes of a
-//                            lane group read 16 consecutive rows, and the +16 padding puts
-//                            the 4 lane groups on disjoint bank ranges (FP32).
+//   row-contiguous operand -> [k][row], row stride ROWS + 4 (FP32) or ROWS + 16 (FP64)
+//                            elements: the 16 lanes of a lane group read 16 consecutive
+//                            rows, and the FP32 padding puts the 4 lane groups on disjoint
+//                            bank ranges.
 // (bf16 planes always use [row][k].)
 // Global loads are 16-byte vectors when the host has checked alignment and divisibility
 // (Params::vecA / vecB), else bounds-checked scalars. Tiles are prefetched into registers
@@ -112,7 +113,11 @@ struct Tile {
   static constexpr int BK = ModeTraits<MODE>::BK;
   static constexpr bool LDS_KC = KCG;
   static constexpr int LDK = BK + ModeTraits<MODE>::KPAD;  // [row][k] stride
-  static constexpr int LDR = ROWS + 16;                    // [k][row] stride
+  // [k][row] stride. FP32: ROWS + 4, so the lane groups g = 0..3 of a fragment read (rows
+  // k = 4g + kk) start 16 banks apart and each half-wave's 32 four-byte reads hit 32 distinct
+  // banks (ROWS + 16 put all four groups on the same 16 banks). FP64: 16 lanes of 8 bytes
+  // already fill the banks, so the stride only keeps 16-byte alignment.
+  static constexpr int LDR = ROWS + (sizeof(S) == 4 ? 4 : 16);
   static constexpr int ELEMS = LDS_KC ? ROWS * LDK : BK * LDR;
   static constexpr int VW = 16 / sizeof(T);  // elements per 16-byte vector
   static constexpr int NE = ROWS * BK / 256;  // elements per thread
