@@ -515,6 +515,205 @@ __global__ __launch_bounds__(256) void jac_colupd_mfma_kernel(double *__restrict
       }
 }
 
+// Complex (planar) counterparts of the two MFMA kernels above, for Hermitian G: each complex
+// product is four real MFMA products (a subtraction passes the negated operand). The Q
+// operands are read from L2 inside the k loop; G tiles of complex problems are few (their
+// p rarely exceeds a few thousand), so these launches are latency- rather than
+// throughput-bound. The mirrored tile is the conjugate transpose.
+template <int B>
+__global__ __launch_bounds__(256) void jac_twoside_herm_kernel(double *__restrict__ G, int n, int64_t imG,
+                                                               const int *__restrict__ pairs,
+                                                               const double *__restrict__ Qin,
+                                                               const double *__restrict__ Din,
+                                                               const int *__restrict__ active) {
+  constexpr int JP = 2 * B, H = JP / 2, FT = H / 16, KS = JP / 4, JJ = JP * JP;
+  const int idx = blockIdx.x, t = threadIdx.x;
+  int K = int((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+  while ((K + 1) * (K + 2) / 2 <= idx) ++K;
+  while (K * (K + 1) / 2 > idx) --K;
+  const int P = idx - K * (K + 1) / 2;
+  const bool aP = active[P] != 0, aK = active[K] != 0;
+  if (!aP && !aK) return;
+  __shared__ double Tr[JP][JP + 1], Ti[JP][JP + 1];
+  __shared__ int rowmap[JP], colmap[JP];
+  if (t < JP) {
+    rowmap[t] = pair_col<B>(pairs + 2 * P, t, n);
+    colmap[t] = pair_col<B>(pairs + 2 * K, t, n);
+  }
+  __syncthreads();
+  if (P == K) {  // aP holds here
+    const double *D = Din + int64_t(P) * 2 * JJ;
+    for (int e = t; e < JJ; e += 256) {
+      const int i = e % JP, j = e / JP;
+      const int r = rowmap[i], c = colmap[j];
+      if (r >= 0 && c >= 0) {
+        G[r + int64_t(c) * n] = D[e];
+        G[imG + r + int64_t(c) * n] = D[JJ + e];
+      }
+    }
+    return;
+  }
+  const double *qp = Qin + int64_t(P) * 2 * JJ;  // row-major re plane, then im plane
+  const double *qk = Qin + int64_t(K) * 2 * JJ;
+  for (int e = t; e < JJ; e += 256) {
+    const int i = e % JP, j = e / JP;
+    const int r = rowmap[i], c = colmap[j];
+    const bool ok = r >= 0 && c >= 0;
+    Tr[i][j] = ok ? G[r + int64_t(c) * n] : 0.0;
+    Ti[i][j] = ok ? G[imG + r + int64_t(c) * n] : 0.0;
+  }
+  __syncthreads();
+  const int lane = t & 63, w = t >> 6, g = lane >> 4, rr = lane & 15;
+  const int i0 = H * (w & 1), j0 = H * (w >> 1);
+  auto qre = [&](const double *q, bool act, int k, int j) { return act ? q[k * JP + j] : (k == j ? 1.0 : 0.0); };
+  auto qim = [&](const double *q, bool act, int k, int j) { return act ? q[JJ + k * JP + j] : 0.0; };
+  jv4d ar[FT][FT], ai[FT][FT];
+#pragma unroll
+  for (int x = 0; x < FT; ++x)
+#pragma unroll
+    for (int y = 0; y < FT; ++y) ar[x][y] = ai[x][y] = jv4d{0.0, 0.0, 0.0, 0.0};
+  for (int s = 0; s < KS; ++s) {  // U = T Q_K
+    const int k = 4 * s + g;
+    double fr[FT], fi[FT], br[FT], bi[FT];
+#pragma unroll
+    for (int x = 0; x < FT; ++x) {
+      fr[x] = Tr[i0 + 16 * x + rr][k];
+      fi[x] = Ti[i0 + 16 * x + rr][k];
+      br[x] = qre(qk, aK, k, j0 + 16 * x + rr);
+      bi[x] = qim(qk, aK, k, j0 + 16 * x + rr);
+    }
+#pragma unroll
+    for (int x = 0; x < FT; ++x)
+#pragma unroll
+      for (int y = 0; y < FT; ++y) {
+        ar[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr[x], br[y], ar[x][y], 0, 0, 0);
+        ar[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(-fi[x], bi[y], ar[x][y], 0, 0, 0);
+        ai[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr[x], bi[y], ai[x][y], 0, 0, 0);
+        ai[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(fi[x], br[y], ai[x][y], 0, 0, 0);
+      }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int x = 0; x < FT; ++x)
+#pragma unroll
+    for (int y = 0; y < FT; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        Tr[i0 + 16 * x + g + 4 * r][j0 + 16 * y + rr] = ar[x][y][r];
+        Ti[i0 + 16 * x + g + 4 * r][j0 + 16 * y + rr] = ai[x][y][r];
+        ar[x][y][r] = 0.0;
+        ai[x][y][r] = 0.0;
+      }
+  __syncthreads();
+  for (int s = 0; s < KS; ++s) {  // Q_P^H U
+    const int k = 4 * s + g;
+    double pr[FT], pi[FT], ur[FT], ui[FT];
+#pragma unroll
+    for (int x = 0; x < FT; ++x) {
+      pr[x] = qre(qp, aP, k, i0 + 16 * x + rr);
+      pi[x] = qim(qp, aP, k, i0 + 16 * x + rr);
+      ur[x] = Tr[k][j0 + 16 * x + rr];
+      ui[x] = Ti[k][j0 + 16 * x + rr];
+    }
+#pragma unroll
+    for (int x = 0; x < FT; ++x)
+#pragma unroll
+      for (int y = 0; y < FT; ++y) {  // conj(Q_P[k][i]) U[k][j]
+        ar[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr[x], ur[y], ar[x][y], 0, 0, 0);
+        ar[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(pi[x], ui[y], ar[x][y], 0, 0, 0);
+        ai[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr[x], ui[y], ai[x][y], 0, 0, 0);
+        ai[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(-pi[x], ur[y], ai[x][y], 0, 0, 0);
+      }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int x = 0; x < FT; ++x)
+#pragma unroll
+    for (int y = 0; y < FT; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        Tr[i0 + 16 * x + g + 4 * r][j0 + 16 * y + rr] = ar[x][y][r];
+        Ti[i0 + 16 * x + g + 4 * r][j0 + 16 * y + rr] = ai[x][y][r];
+      }
+  __syncthreads();
+  for (int e = t; e < JJ; e += 256) {
+    {  // G[P, K], rows fastest
+      const int i = e % JP, j = e / JP;
+      const int r = rowmap[i], c = colmap[j];
+      if (r >= 0 && c >= 0) {
+        G[r + int64_t(c) * n] = Tr[i][j];
+        G[imG + r + int64_t(c) * n] = Ti[i][j];
+      }
+    }
+    {  // G[K, P] = G[P, K]^H
+      const int j = e % JP, i = e / JP;
+      const int r = rowmap[i], c = colmap[j];
+      if (r >= 0 && c >= 0) {
+        G[c + int64_t(r) * n] = Tr[i][j];
+        G[imG + c + int64_t(r) * n] = -Ti[i][j];
+      }
+    }
+  }
+}
+
+// Complex V[:, P] <- V[:, P] Q_P, as V^T <- Q^T V^T on the matrix cores (see the real
+// kernel): workgroups of 64 rows (16 per wave) and one pair, Q_P staged in LDS.
+template <int B>
+__global__ __launch_bounds__(256) void jac_colupd_herm_kernel(double *__restrict__ Y, int rows, int64_t imY,
+                                                              const int *__restrict__ pairs, int cols,
+                                                              const double *__restrict__ Qin,
+                                                              const int *__restrict__ active) {
+  constexpr int JP = 2 * B, FJ = JP / 16, KS = JP / 4, JJ = JP * JP;
+  const int pair = blockIdx.y;
+  if (!active[pair]) return;
+  __shared__ double Qr[JP][JP + 1], Qi[JP][JP + 1];
+  __shared__ int colmap[JP];
+  const int t = threadIdx.x;
+  const double *Q = Qin + int64_t(pair) * 2 * JJ;
+  for (int e = t; e < JJ; e += 256) {
+    Qr[e / JP][e % JP] = Q[e];
+    Qi[e / JP][e % JP] = Q[JJ + e];
+  }
+  if (t < JP) colmap[t] = pair_col<B>(pairs + 2 * pair, t, cols);
+  __syncthreads();
+  const int lane = t & 63, w = t >> 6, g = lane >> 4, rr = lane & 15;
+  const int r0 = blockIdx.x * 64 + 16 * w;
+  const int row = r0 + rr;
+  double vr[KS], vi[KS];
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    const int c = colmap[4 * s + g];
+    const bool ok = row < rows && c >= 0;
+    vr[s] = ok ? Y[row + int64_t(c) * rows] : 0.0;
+    vi[s] = ok ? Y[imY + row + int64_t(c) * rows] : 0.0;
+  }
+  jv4d ar[FJ], ai[FJ];
+#pragma unroll
+  for (int x = 0; x < FJ; ++x) ar[x] = ai[x] = jv4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+#pragma unroll
+    for (int x = 0; x < FJ; ++x) {
+      const double qr_ = Qr[4 * s + g][16 * x + rr], qi_ = Qi[4 * s + g][16 * x + rr];
+      ar[x] = __builtin_amdgcn_mfma_f64_16x16x4f64(qr_, vr[s], ar[x], 0, 0, 0);
+      ar[x] = __builtin_amdgcn_mfma_f64_16x16x4f64(-qi_, vi[s], ar[x], 0, 0, 0);
+      ai[x] = __builtin_amdgcn_mfma_f64_16x16x4f64(qr_, vi[s], ai[x], 0, 0, 0);
+      ai[x] = __builtin_amdgcn_mfma_f64_16x16x4f64(qi_, vr[s], ai[x], 0, 0, 0);
+    }
+  }
+  // ar[x][r] = new V[row][column 16 x + g + 4 r of the pair]
+#pragma unroll
+  for (int x = 0; x < FJ; ++x)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int c = colmap[16 * x + g + 4 * r];
+      if (row < rows && c >= 0) {
+        Y[row + int64_t(c) * rows] = ar[x][r];
+        Y[imY + row + int64_t(c) * rows] = ai[x][r];
+      }
+    }
+}
+
 // G <- (G + G^H) / 2: the MFMA product M M^H is not bitwise Hermitian (G[i,j] and G[j,i]
 // come from different tiles), and the Jacobi rotations assume it is.
 __global__ void hermitize_kernel(double *G, int n, int64_t imG, bool cplx) {
@@ -725,9 +924,9 @@ static Schedule schedule_for(tnh_ctx *ctx, int cols, int width) {
 
 // two_sided: diagonalise Hermitian G (n x n) in place, V <- V * rotations.
 // one-sided: orthogonalise the columns of X (rows x n) in place, V (n x n) accumulates.
-// Columns are cut into blocks of width B (pairs of 2B columns); the real two-sided problem
-// uses B = 32, which halves the rounds per sweep and so the passes over G and V, the
-// others B = JB = 16.
+// Columns are cut into blocks of width B (pairs of 2B columns); the two-sided problem (real
+// or Hermitian) uses B = 32, which halves the rounds per sweep and so the passes over G and
+// V, the one-sided fallback B = JB = 16.
 template <int B>
 static JacobiResult jacobi_blocks(tnh_ctx *ctx, bool two_sided, double *X, int rows, int n, bool cplx,
                                   double *V, double tol, double abstol, int max_sweeps) {
@@ -735,7 +934,7 @@ static JacobiResult jacobi_blocks(tnh_ctx *ctx, bool two_sided, double *X, int r
   constexpr int NT = B == JB ? 256 : 1024;  // threads of the pair eigensolver (1024 measured faster than 512 at B = 32)
   static_assert(B == JB || B == 2 * JB, "block widths 16 and 32");
   if constexpr (B != JB)
-    TNH_REQUIRE(two_sided && !cplx, TNH_ERR_ARG, "wide Jacobi blocks serve the real two-sided problem");
+    TNH_REQUIRE(two_sided, TNH_ERR_ARG, "wide Jacobi blocks serve the two-sided problem");
   JacobiResult res;
   const Schedule s = schedule_for(ctx, n, B);
   const int64_t imX = int64_t(rows) * n, imV = int64_t(n) * n;
@@ -771,13 +970,11 @@ static JacobiResult jacobi_blocks(tnh_ctx *ctx, bool two_sided, double *X, int r
           hipLaunchKernelGGL(jac_gram_kernel, dim3(s.npairs, nsplit), dim3(256), 0, ctx->stream, X,
                              rows, imX, n, cplx, pr, nsplit, partial);
       }
-      if constexpr (B == JB) {
-        if (cplx)
-          hipLaunchKernelGGL((jac_eig_kernel<true, B, NT>), dim3(s.npairs), dim3(NT), 0, ctx->stream, X,
-                             imX, n, partial, nsplit, two_sided, pr, n, tol, abstol, max_inner, cross, Q, D,
-                             active, conv + sw);
-      }
-      if (!cplx)
+      if (cplx)
+        hipLaunchKernelGGL((jac_eig_kernel<true, B, NT>), dim3(s.npairs), dim3(NT), 0, ctx->stream, X,
+                           imX, n, partial, nsplit, two_sided, pr, n, tol, abstol, max_inner, cross, Q, D,
+                           active, conv + sw);
+      else
         hipLaunchKernelGGL((jac_eig_kernel<false, B, NT>), dim3(s.npairs), dim3(NT), 0, ctx->stream, X,
                            imX, n, partial, nsplit, two_sided, pr, n, tol, abstol, max_inner, cross, Q, D,
                            active, conv + sw);
@@ -788,6 +985,10 @@ static JacobiResult jacobi_blocks(tnh_ctx *ctx, bool two_sided, double *X, int r
         if (cplx)
           hipLaunchKernelGGL((jac_colupd_kernel<true, B>), dim3((rmax + 255) / 256, s.npairs), dim3(256), 0,
                              ctx->stream, Xc, rows, imX, V, n, imV, pr, n, Q, active);
+      } else {
+        if (cplx)
+          hipLaunchKernelGGL(jac_colupd_herm_kernel<B>, dim3((n + 63) / 64, s.npairs), dim3(256), 0,
+                             ctx->stream, V, n, imV, pr, n, Q, active);
       }
       if (!cplx && two_sided)
         hipLaunchKernelGGL(jac_colupd_mfma_kernel<B>, dim3((n + 127) / 128, s.npairs), dim3(256), 0,
@@ -798,6 +999,10 @@ static JacobiResult jacobi_blocks(tnh_ctx *ctx, bool two_sided, double *X, int r
       if constexpr (B == JB) {
         if (two_sided && cplx)
           hipLaunchKernelGGL(jac_twoside_kernel<true>, dim3(s.npairs, s.npairs), dim3(256), 0,
+                             ctx->stream, X, n, imX, pr, Q, D, active);
+      } else {
+        if (two_sided && cplx)
+          hipLaunchKernelGGL(jac_twoside_herm_kernel<B>, dim3(s.npairs * (s.npairs + 1) / 2), dim3(256), 0,
                              ctx->stream, X, n, imX, pr, Q, D, active);
       }
       if (two_sided && !cplx)
@@ -822,7 +1027,7 @@ static JacobiResult jacobi_blocks(tnh_ctx *ctx, bool two_sided, double *X, int r
 
 static JacobiResult jacobi(tnh_ctx *ctx, bool two_sided, double *X, int rows, int n, bool cplx,
                            double *V, double tol, double abstol, int max_sweeps) {
-  if (two_sided && !cplx)
+  if (two_sided)
     return jacobi_blocks<2 * JB>(ctx, two_sided, X, rows, n, cplx, V, tol, abstol, max_sweeps);
   return jacobi_blocks<JB>(ctx, two_sided, X, rows, n, cplx, V, tol, abstol, max_sweeps);
 }
