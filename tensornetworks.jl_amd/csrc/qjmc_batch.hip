@@ -76,11 +76,15 @@ __device__ int trunc_rank(const double *s, int r, double cutoff, int maxdim, int
 }
 
 constexpr int QG = QM * (QM + 1);  // one padded Jacobi matrix plane
+// Factors and site tensors have one side <= CH <= 16 and the other <= 32 (d CH <= 32), so
+// 512 entries hold them; with the 32 x 32 split matrix and the Jacobi planes the workgroup
+// needs ~70 KB of LDS, two workgroups per CU.
+constexpr int QAB = QMM / 2;
 
 struct Lds {
   double Mr[QMM], Mi[QMM];  // the matrix being split, column-major m x n
-  double Ar[QMM], Ai[QMM];  // left factor (m x k)
-  double Br[QMM], Bi[QMM];  // right factor (k x n)
+  double Ar[QAB], Ai[QAB];  // left factor (m x k), a site tensor, Newton-Schulz W (p x k)
+  double Br[QAB], Bi[QAB];  // right factor (k x n), a site tensor, Newton-Schulz X (k x k)
   double gr[QG], gi[QG], qr[QG], qi[QG];  // Jacobi planes, row pitch 2B + 1 of the chosen width
   double val[QM], sv[QM];  // singular values by index, and in descending order
   double red[QT / 64];
