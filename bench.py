@@ -388,6 +388,47 @@ def run_matvecs(ctx, T, ops, dL, dR, dW, dx, out, prec, steps, warmup, torch):
     return el, g, mix
 
 
+def matvec_chi_sweep(ctx, ops, Tensor, torch, chis=(256, 512, 1024), seed=0, D=5, d=2, steps=20, warmup=3):
+    """The matvec at the smaller bond dimensions of the synthetic workload, per precision,
+    with the autotuned tiles and with the fixed size rule's tiles (same inputs)."""
+    out = {}
+    W = heisenberg_bulk_w()
+    dW = Tensor.from_numpy(W, ctx=ctx)
+    for chi in chis:
+        Lt, Rt, tht = make_inputs(chi, D, seed, torch)
+        Lt32, Rt32 = Lt.float(), Rt.float()
+        o = torch.empty(chi * d * d * chi, device="cuda", dtype=torch.float64)
+        dx = Tensor.wrap(tht.data_ptr(), (chi, d, d, chi), np.float64, owner=tht, ctx=ctx)
+        do = Tensor.wrap(o.data_ptr(), (chi, d, d, chi), np.float64, owner=o, ctx=ctx)
+        F = matvec_flops(chi, D, d)
+        row = {"gflop_per_matvec": F / 1e9}
+        for prec in ("f64", "f32", "bf16x3"):
+            src = (Lt, Rt) if prec == "f64" else (Lt32, Rt32)
+            dt = np.float64 if prec == "f64" else np.float32
+            dL = Tensor.wrap(src[0].data_ptr(), (chi, D, chi), dt, owner=src[0], ctx=ctx)
+            dR = Tensor.wrap(src[1].data_ptr(), (chi, D, chi), dt, owner=src[1], ctx=ctx)
+            r = {}
+            for mode in ("rule", "autotuned"):
+                ctx.gemm_autotune(mode == "autotuned")
+                el, g, _ = run_matvecs(ctx, None, ops, dL, dR, dW, dx, do, prec, steps, warmup, torch)
+                r[mode] = {"ms_per_matvec": 1e3 * el / steps, "tflops": F * steps / el / 1e12,
+                           "gemm_tflops": (g["flops"] / g["ms"] / 1e9) if g["ms"] > 0 else None}
+            ctx.gemm_autotune(True)
+            r["tflops"] = r["autotuned"]["tflops"]
+            r["frac_of_peak"] = r["tflops"] / PEAK[prec]
+            r["autotune_speedup_vs_rule"] = r["rule"]["ms_per_matvec"] / r["autotuned"]["ms_per_matvec"]
+            row[prec] = r
+        out[str(chi)] = row
+        del dx, do
+    return out
+
+
+def tile_choices(ctx):
+    """The autotuner's table in short form: mode, transposes, M x N x K -> chosen tile."""
+    return ["%s %s %dx%dx%d -> %s (rule %s)" % (e["mode"], e["trans"], e["M"], e["N"], e["K"], e["tile"], e["rule"])
+            for e in ctx.gemm_tuning()]
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -415,6 +456,8 @@ def main(argv=None):
                     help="run only the chi=2048 DMRG bond step (FP64 and FP32 matvecs)")
     ap.add_argument("--svd-only", action="store_true",
                     help="run only the truncated SVD of the 2chi x 2chi theta matrix (1 warm-up + 1 timed)")
+    ap.add_argument("--sweep-only", action="store_true",
+                    help="run only the matvec at chi = 256, 512, 1024 (autotuned vs size-rule tiles)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB FP64 device copy (reformat kernel), the "
                          "known-size reference for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -458,6 +501,10 @@ def main(argv=None):
         return 0
     if a.legs_only:
         print(json.dumps({"gradient_tfim_N64": gradient_tfim_n64(), "qjmc_tfim_N40": qjmc_tfim_n40()}))
+        return 0
+    if a.sweep_only:
+        sw = matvec_chi_sweep(ctx, ops, Tensor, torch, seed=a.seed)
+        print(json.dumps({"matvec_chi_sweep": sw, "gemm_tiles": tile_choices(ctx)}))
         return 0
     Lt, Rt, tht = make_inputs(chi, D, a.seed, torch)
     if a.svd_only:
@@ -525,6 +572,8 @@ def main(argv=None):
         if "rel_err_vs_f64" in r:
             extra[prec]["rel_err_vs_f64"] = r["rel_err_vs_f64"]
     extra["fp32_speedup_vs_fp64"] = res["f32"]["tflops"] / res["f64"]["tflops"]
+    extra["gemm_tiles_chi%d" % chi] = tile_choices(ctx)
+    extra["matvec_chi_sweep"] = matvec_chi_sweep(ctx, ops, Tensor, torch, seed=a.seed)
 
     if not a.no_svd:
         # truncated SVD of the 2chi x 2chi theta matrix, kept rank chi (FP64 Gram + Jacobi)

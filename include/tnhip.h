@@ -83,6 +83,22 @@ int tnh_profile_reset(tnh_ctx *ctx);
 int tnh_profile_read(tnh_ctx *ctx, const char *family, int64_t *launches, double *total_ms,
                      double *flops, double *bytes);
 
+/* GEMM tile selection (no counterpart in the reference, whose contractions call BLAS
+ * through TensorOperations). mode 1 (the default): the first product of each size class
+ * (arithmetic mode, operand types, transposes, vector paths, and M, N, K rounded to four
+ * classes per octave) with at least 2^28 flops, both sides >= 96 and no batching times
+ * every candidate tile (one untimed and two timed launches each, on a scratch output; the
+ * stream is synchronised once) and the fastest is reused for the class; smaller or
+ * batched products, and products issued while the stream is being captured, take the
+ * size rule. mode 0: always the size rule. mode 2: forget the tuned classes. Tiles
+ * differ only in the output block a workgroup computes, never in the summation order,
+ * so results do not depend on the choice. */
+int tnh_gemm_autotune(tnh_ctx *ctx, int mode);
+/* The tuned classes as text, one line each: mode, transposes, the timed product's
+ * M N K, the chosen and the rule's tile, and the milliseconds of every candidate.
+ * Writes at most len bytes (NUL-terminated); *needed receives the full size. */
+int tnh_gemm_tuning_report(tnh_ctx *ctx, char *buf, int64_t len, int64_t *needed);
+
 /* ------------------------------------------------------------------ buffers */
 
 /* Allocate an uninitialised device tensor. ndim <= 8. */

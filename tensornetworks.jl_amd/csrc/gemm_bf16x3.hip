@@ -82,14 +82,19 @@ void split_bf16(hipStream_t s, const void *X, int dtype, bool rfast, int64_t ld,
 }
 
 // p.A / p.B point at split planes ([row][k], row stride p.lda = p.ldb = Kp, p.K = Kp).
-void launch_bf16p(hipStream_t s, const Params &p, int batch, int tC, bool big) {
+// tile: 0 = 64 x 64, 1 = 128 x 128 (kTileShapes in gemm.hip)
+void launch_bf16p(hipStream_t s, const Params &p, int batch, int tC, int tile) {
   const bool c64 = tC == TNH_F64;
-  if (big) {
-    if (c64) launch_p<double, 128, 128>(s, p, batch);
-    else launch_p<float, 128, 128>(s, p, batch);
-  } else {
-    if (c64) launch_p<double, 64, 64>(s, p, batch);
-    else launch_p<float, 64, 64>(s, p, batch);
+  switch (tile) {
+    case 0:
+      if (c64) launch_p<double, 64, 64>(s, p, batch);
+      else launch_p<float, 64, 64>(s, p, batch);
+      return;
+    case 1:
+      if (c64) launch_p<double, 128, 128>(s, p, batch);
+      else launch_p<float, 128, 128>(s, p, batch);
+      return;
+    default: throw Error(TNH_ERR_ARG, "gemm: unknown split-BF16 tile " + std::to_string(tile));
   }
 }
 

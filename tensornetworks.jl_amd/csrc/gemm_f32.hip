@@ -38,9 +38,14 @@ static void launch_types(hipStream_t s, const Params &p, int batch, bool tra, bo
 #undef TNH_CASE
 }
 
-void launch_f32(hipStream_t s, const Params &p, int batch, bool tra, bool trb, int tA, int tB, int tC, bool big) {
-  if (big) launch_types<128, 128>(s, p, batch, tra, trb, tA, tB, tC);
-  else launch_types<64, 64>(s, p, batch, tra, trb, tA, tB, tC);
+// tile: 0 = 64 x 64, 1 = 128 x 128, 2 = 128 x 64 (kTileShapes in gemm.hip)
+void launch_f32(hipStream_t s, const Params &p, int batch, bool tra, bool trb, int tA, int tB, int tC, int tile) {
+  switch (tile) {
+    case 0: return launch_types<64, 64>(s, p, batch, tra, trb, tA, tB, tC);
+    case 1: return launch_types<128, 128>(s, p, batch, tra, trb, tA, tB, tC);
+    case 2: return launch_types<128, 64>(s, p, batch, tra, trb, tA, tB, tC);
+    default: throw Error(TNH_ERR_ARG, "gemm: unknown FP32 tile " + std::to_string(tile));
+  }
 }
 
 }  // namespace gk

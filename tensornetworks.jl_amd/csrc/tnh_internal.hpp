@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <cstring>
 #include <exception>
@@ -121,6 +122,18 @@ struct tnh_ctx {
   bool up_pending = false;
   // Device copies of the block-Jacobi round-robin schedules, keyed by column count.
   std::map<int, std::pair<void *, size_t>> schedules;
+
+  // GEMM tile autotuner. A size class is (mode, operand types, transposes, vector paths,
+  // M, N, K rounded to four classes per octave); its tile is chosen on the first product
+  // of that class by timing every candidate tile on a scratch output, and reused after.
+  bool gemm_autotune = true;
+  struct TileChoice {
+    int tile = 0;
+    int rule = 0;
+    float ms[4] = {0, 0, 0, 0};  // best of two timed runs per candidate
+    int64_t M = 0, N = 0, K = 0;  // the product that was timed
+  };
+  std::map<std::array<int64_t, 12>, TileChoice> gemm_tiles;
 
   ~tnh_ctx();
 };

@@ -88,6 +88,8 @@ SIGNATURES = {
     "tnh_profile_enable": (C.c_int, [ctx_p, C.c_int]),
     "tnh_profile_reset": (C.c_int, [ctx_p]),
     "tnh_profile_read": (C.c_int, [ctx_p, C.c_char_p, i64p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "tnh_gemm_autotune": (C.c_int, [ctx_p, C.c_int]),
+    "tnh_gemm_tuning_report": (C.c_int, [ctx_p, C.c_char_p, C.c_int64, i64p]),
     "tnh_buf_create": (C.c_int, [ctx_p, C.c_int, C.c_int, i64p, C.POINTER(buf_p)]),
     "tnh_buf_upload": (C.c_int, [ctx_p, C.c_void_p, C.c_int, C.c_int, C.c_int, i64p, C.POINTER(buf_p)]),
     "tnh_buf_download": (C.c_int, [ctx_p, buf_p, C.c_void_p, C.c_int]),

@@ -84,6 +84,28 @@ class Context:
         self.call("tnh_profile_read", family.encode(), C.byref(n), C.byref(ms), C.byref(fl), C.byref(by))
         return {"launches": n.value, "ms": ms.value, "flops": fl.value, "bytes": by.value}
 
+    # GEMM tile selection: True = autotune per size class (default), False = size rule
+    def gemm_autotune(self, on: bool = True):
+        self.call("tnh_gemm_autotune", 1 if on else 0)
+
+    def gemm_autotune_reset(self):
+        self.call("tnh_gemm_autotune", 2)
+
+    def gemm_tuning(self) -> list[dict]:
+        """The tuned size classes: mode, transposes, the timed M N K, chosen and rule tile,
+        milliseconds per candidate tile."""
+        need = C.c_int64()
+        self.call("tnh_gemm_tuning_report", None, 0, C.byref(need))
+        buf = C.create_string_buffer(int(need.value))
+        self.call("tnh_gemm_tuning_report", buf, int(need.value), C.byref(need))
+        out = []
+        for line in buf.value.decode().splitlines():
+            f = line.split()
+            kv = dict(x.split("=", 1) for x in f[2:])
+            out.append({"mode": f[0], "trans": f[1], "M": int(kv["M"]), "N": int(kv["N"]), "K": int(kv["K"]),
+                        "tile": kv["tile"], "rule": kv["rule"], "ms": [float(x) for x in kv["ms"].split(",")]})
+        return out
+
     def close(self):
         if self.h:
             lib.tnh_ctx_destroy(self.h)
