@@ -121,8 +121,14 @@ void tnh_ctx::prof_resolve() {
   prof_pending.clear();
 }
 
+hipStream_t tnh_ctx::aux_stream() {
+  if (!aux) TNH_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
+  return aux;
+}
+
 tnh_ctx::~tnh_ctx() {
   if (stream) (void)hipStreamSynchronize(stream);
+  if (aux) (void)hipStreamSynchronize(aux);
   for (auto &kv : prof_pending)
     for (auto &r : kv.second) {
       (void)hipEventDestroy(r.a);
@@ -138,6 +144,7 @@ tnh_ctx::~tnh_ctx() {
   if (pin_down) (void)hipHostFree(pin_down);
   if (pin_up) (void)hipHostFree(pin_up);
   if (up_done) (void)hipEventDestroy(up_done);
+  if (aux) (void)hipStreamDestroy(aux);
   if (stream) (void)hipStreamDestroy(stream);
 }
 

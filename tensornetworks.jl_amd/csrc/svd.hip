@@ -947,10 +947,19 @@ static JacobiResult jacobi_blocks(tnh_ctx *ctx, bool two_sided, double *X, int r
   if (!two_sided)
     partial = static_cast<double *>(
         ctx->workspace(601, sizeof(double) * size_t(s.npairs) * nsplit * 2 * JPB * JPB));
-  double *Q = static_cast<double *>(ctx->workspace(602, sizeof(double) * size_t(s.npairs) * 2 * JPB * JPB));
+  // Two-sided problems overlap the column update of V (it needs only the round's rotations)
+  // with the next round's pair solver, a latency-bound launch of one workgroup per pair:
+  // V is updated on the helper stream once G's tile update has run, the rotations Q and the activity flags are double-buffered by round parity, and
+  // the pair solver of round g + 2 waits for the V update of round g before it overwrites
+  // that round's buffer. The helper stream starts after, and is joined back into, the
+  // context's stream.
+  const bool overlap = two_sided;
+  const int nbuf = overlap ? 2 : 1;
+  const size_t qn = size_t(s.npairs) * 2 * JPB * JPB;
+  double *Q = static_cast<double *>(ctx->workspace(602, sizeof(double) * qn * nbuf));
   double *D = two_sided ? static_cast<double *>(ctx->workspace(603, sizeof(double) * size_t(s.npairs) * 2 * JPB * JPB))
                         : nullptr;
-  int *active = static_cast<int *>(ctx->workspace(604, sizeof(int) * s.npairs));
+  int *active = static_cast<int *>(ctx->workspace(604, sizeof(int) * s.npairs * nbuf));
   unsigned long long *conv = static_cast<unsigned long long *>(
       ctx->workspace(605, sizeof(unsigned long long) * (max_sweeps + 1)));
   TNH_HIP(hipMemsetAsync(conv, 0, sizeof(unsigned long long) * (max_sweeps + 1), ctx->stream));
@@ -959,69 +968,117 @@ static JacobiResult jacobi_blocks(tnh_ctx *ctx, bool two_sided, double *X, int r
   // Otherwise one inner sweep per pair and round: measured fastest overall (the outer
   // sweeps revisit every pair, so fully converged sub-problems buy little).
   const int max_inner = (s.nb <= 2) ? kMaxInnerCap : 1;
-  for (int sw = 0; sw < max_sweeps; ++sw) {
-    for (int r = 0; r < s.rounds; ++r) {
-      const int *pr = s.dev + size_t(r) * s.npairs * 2;
-      // the first round of a sweep visits every block once: it also rotates the index pairs
-      // inside the blocks; the other rounds only the pairs across the two blocks of a pair
-      const bool cross = s.nb > 2 && r > 0;
-      if constexpr (B == JB) {
-        if (!two_sided)
-          hipLaunchKernelGGL(jac_gram_kernel, dim3(s.npairs, nsplit), dim3(256), 0, ctx->stream, X,
-                             rows, imX, n, cplx, pr, nsplit, partial);
-      }
-      if (cplx)
-        hipLaunchKernelGGL((jac_eig_kernel<true, B, NT>), dim3(s.npairs), dim3(NT), 0, ctx->stream, X,
-                           imX, n, partial, nsplit, two_sided, pr, n, tol, abstol, max_inner, cross, Q, D,
-                           active, conv + sw);
-      else
-        hipLaunchKernelGGL((jac_eig_kernel<false, B, NT>), dim3(s.npairs), dim3(NT), 0, ctx->stream, X,
-                           imX, n, partial, nsplit, two_sided, pr, n, tol, abstol, max_inner, cross, Q, D,
-                           active, conv + sw);
-      // two-sided: G takes both rotations in one tiled pass, V its columns; one-sided: X and
-      // V take the column rotations
-      double *Xc = two_sided ? nullptr : X;
-      if constexpr (B == JB) {
-        if (cplx)
-          hipLaunchKernelGGL((jac_colupd_kernel<true, B>), dim3((rmax + 255) / 256, s.npairs), dim3(256), 0,
-                             ctx->stream, Xc, rows, imX, V, n, imV, pr, n, Q, active);
-      } else {
-        if (cplx)
-          hipLaunchKernelGGL(jac_colupd_herm_kernel<B>, dim3((n + 63) / 64, s.npairs), dim3(256), 0,
-                             ctx->stream, V, n, imV, pr, n, Q, active);
-      }
-      if (!cplx && two_sided)
-        hipLaunchKernelGGL(jac_colupd_mfma_kernel<B>, dim3((n + 127) / 128, s.npairs), dim3(256), 0,
-                           ctx->stream, V, n, pr, n, Q, active);
-      else if (!cplx)
-        hipLaunchKernelGGL((jac_colupd_kernel<false, B>), dim3((rmax + 255) / 256, s.npairs), dim3(256), 0,
-                           ctx->stream, Xc, rows, imX, V, n, imV, pr, n, Q, active);
-      if constexpr (B == JB) {
-        if (two_sided && cplx)
-          hipLaunchKernelGGL(jac_twoside_kernel<true>, dim3(s.npairs, s.npairs), dim3(256), 0,
-                             ctx->stream, X, n, imX, pr, Q, D, active);
-      } else {
-        if (two_sided && cplx)
-          hipLaunchKernelGGL(jac_twoside_herm_kernel<B>, dim3(s.npairs * (s.npairs + 1) / 2), dim3(256), 0,
-                             ctx->stream, X, n, imX, pr, Q, D, active);
-      }
-      if (two_sided && !cplx)
-        hipLaunchKernelGGL(jac_twoside_sym_kernel<B>, dim3(s.npairs * (s.npairs + 1) / 2), dim3(256), 0,
-                           ctx->stream, X, n, pr, Q, D, active);
+  hipStream_t vs = ctx->stream;
+  hipEvent_t evq[2] = {nullptr, nullptr}, evv[2] = {nullptr, nullptr}, evj = nullptr;
+  if (overlap) {
+    vs = ctx->aux_stream();
+    for (int i = 0; i < 2; ++i) {
+      evq[i] = ctx->take_event();
+      evv[i] = ctx->take_event();
     }
-    TNH_HIP(hipGetLastError());
-    unsigned long long bits = 0;
-    to_host(ctx, &bits, conv + sw, sizeof(bits));
-    double m;
-    std::memcpy(&m, &bits, sizeof(m));
-    res.sweeps = sw + 1;
-    res.measure = m;
-    if (!std::isfinite(m)) return res;
-    if (m <= tol) {
-      res.converged = true;
-      return res;
-    }
+    evj = ctx->take_event();
+    TNH_HIP(hipEventRecord(evj, ctx->stream));
+    TNH_HIP(hipStreamWaitEvent(vs, evj, 0));
   }
+  auto sweeps = [&]() {
+    for (int sw = 0; sw < max_sweeps; ++sw) {
+      for (int r = 0; r < s.rounds; ++r) {
+        const int *pr = s.dev + size_t(r) * s.npairs * 2;
+        const int64_t gr = int64_t(sw) * s.rounds + r;
+        const int b = overlap ? int(gr & 1) : 0;
+        double *Qb = Q + qn * b;
+        int *act = active + s.npairs * b;
+        if (overlap && gr >= 2) TNH_HIP(hipStreamWaitEvent(ctx->stream, evv[b], 0));
+        // the first round of a sweep visits every block once: it also rotates the index pairs
+        // inside the blocks; the other rounds only the pairs across the two blocks of a pair
+        const bool cross = s.nb > 2 && r > 0;
+        if constexpr (B == JB) {
+          if (!two_sided)
+            hipLaunchKernelGGL(jac_gram_kernel, dim3(s.npairs, nsplit), dim3(256), 0, ctx->stream, X,
+                               rows, imX, n, cplx, pr, nsplit, partial);
+        }
+        if (cplx)
+          hipLaunchKernelGGL((jac_eig_kernel<true, B, NT>), dim3(s.npairs), dim3(NT), 0, ctx->stream, X,
+                             imX, n, partial, nsplit, two_sided, pr, n, tol, abstol, max_inner, cross, Qb, D,
+                             act, conv + sw);
+        else
+          hipLaunchKernelGGL((jac_eig_kernel<false, B, NT>), dim3(s.npairs), dim3(NT), 0, ctx->stream, X,
+                             imX, n, partial, nsplit, two_sided, pr, n, tol, abstol, max_inner, cross, Qb, D,
+                             act, conv + sw);
+        if constexpr (B == JB) {
+          if (two_sided && cplx)
+            hipLaunchKernelGGL(jac_twoside_kernel<true>, dim3(s.npairs, s.npairs), dim3(256), 0,
+                               ctx->stream, X, n, imX, pr, Qb, D, act);
+        } else {
+          if (two_sided && cplx)
+            hipLaunchKernelGGL(jac_twoside_herm_kernel<B>, dim3(s.npairs * (s.npairs + 1) / 2), dim3(256), 0,
+                               ctx->stream, X, n, imX, pr, Qb, D, act);
+        }
+        if (two_sided && !cplx)
+          hipLaunchKernelGGL(jac_twoside_sym_kernel<B>, dim3(s.npairs * (s.npairs + 1) / 2), dim3(256), 0,
+                             ctx->stream, X, n, pr, Qb, D, act);
+        if (overlap) {
+          TNH_HIP(hipEventRecord(evq[b], ctx->stream));
+          TNH_HIP(hipStreamWaitEvent(vs, evq[b], 0));
+        }
+        // two-sided: V takes the column rotations on the helper stream after the tile update,
+        // alongside the next round's pair solver (which fills a quarter of the CUs); one-sided:
+        // X and V take the column rotations
+        double *Xc = two_sided ? nullptr : X;
+        if constexpr (B == JB) {
+          if (cplx)
+            hipLaunchKernelGGL((jac_colupd_kernel<true, B>), dim3((rmax + 255) / 256, s.npairs), dim3(256), 0,
+                               vs, Xc, rows, imX, V, n, imV, pr, n, Qb, act);
+        } else {
+          if (cplx)
+            hipLaunchKernelGGL(jac_colupd_herm_kernel<B>, dim3((n + 63) / 64, s.npairs), dim3(256), 0, vs, V,
+                               n, imV, pr, n, Qb, act);
+        }
+        if (!cplx && two_sided)
+          hipLaunchKernelGGL(jac_colupd_mfma_kernel<B>, dim3((n + 127) / 128, s.npairs), dim3(256), 0, vs, V, n,
+                             pr, n, Qb, act);
+        else if (!cplx)
+          hipLaunchKernelGGL((jac_colupd_kernel<false, B>), dim3((rmax + 255) / 256, s.npairs), dim3(256), 0,
+                             vs, Xc, rows, imX, V, n, imV, pr, n, Qb, act);
+        if (overlap) TNH_HIP(hipEventRecord(evv[b], vs));
+      }
+      TNH_HIP(hipGetLastError());
+      unsigned long long bits = 0;
+      to_host(ctx, &bits, conv + sw, sizeof(bits));
+      double m;
+      std::memcpy(&m, &bits, sizeof(m));
+      res.sweeps = sw + 1;
+      res.measure = m;
+      if (!std::isfinite(m)) return;
+      if (m <= tol) {
+        res.converged = true;
+        return;
+      }
+    }
+  };
+  // the context's stream waits for the last V update; events go back to the pool
+  auto join = [&]() {
+    if (!overlap) return;
+    hipError_t e1 = hipEventRecord(evj, vs);
+    hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent(ctx->stream, evj, 0) : e1;
+    for (int i = 0; i < 2; ++i) {
+      ctx->event_pool.push_back(evq[i]);
+      ctx->event_pool.push_back(evv[i]);
+    }
+    ctx->event_pool.push_back(evj);
+    TNH_HIP(e2);
+  };
+  try {
+    sweeps();
+  } catch (...) {
+    if (overlap) (void)hipStreamSynchronize(vs);
+    try {
+      join();
+    } catch (...) {
+    }
+    throw;
+  }
+  join();
   return res;
 }
 

@@ -69,6 +69,11 @@ struct tnh_buf {
 struct tnh_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // Helper stream (default priority) for work that overlaps the main stream inside one
+  // call; every call that uses it joins it back into `stream` before returning, so the
+  // context stays ordered on `stream` for its callers.
+  hipStream_t aux = nullptr;
+  hipStream_t aux_stream();
   std::string err;
   int cu_count = 256;
 
