@@ -456,6 +456,7 @@ def main(argv=None):
                     help="run only the chi=2048 DMRG bond step (FP64 and FP32 matvecs)")
     ap.add_argument("--svd-only", action="store_true",
                     help="run only the truncated SVD of the 2chi x 2chi theta matrix (1 warm-up + 1 timed)")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the matvec sweep over chi = 256, 512, 1024")
     ap.add_argument("--sweep-only", action="store_true",
                     help="run only the matvec at chi = 256, 512, 1024 (autotuned vs size-rule tiles)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -573,7 +574,8 @@ def main(argv=None):
             extra[prec]["rel_err_vs_f64"] = r["rel_err_vs_f64"]
     extra["fp32_speedup_vs_fp64"] = res["f32"]["tflops"] / res["f64"]["tflops"]
     extra["gemm_tiles_chi%d" % chi] = tile_choices(ctx)
-    extra["matvec_chi_sweep"] = matvec_chi_sweep(ctx, ops, Tensor, torch, seed=a.seed)
+    if not a.no_sweep:
+        extra["matvec_chi_sweep"] = matvec_chi_sweep(ctx, ops, Tensor, torch, seed=a.seed)
 
     if not a.no_svd:
         # truncated SVD of the 2chi x 2chi theta matrix, kept rank chi (FP64 Gram + Jacobi)
