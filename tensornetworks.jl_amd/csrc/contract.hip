@@ -415,6 +415,18 @@ void heff_apply(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *W1, const tnh_buf
   char *T2 = static_cast<char *>(ctx->workspace(812, n2 * rs * (t2c ? 2 : 1)));
 
   // (1) T1[(a,w),(s1,s2,b)] = L[(a,w),a'] x[a',(s1,s2,b)]
+  // FP32 arithmetic on an FP64 theta: one rounding pass to an FP32 copy first (a streaming
+  // kernel at HBM rate) lets the GEMM stage 16-byte FP32 vectors without converting inside
+  // its k loop; skipped for small theta, where the extra launch would cost more.
+  COperand xop = cop(x);
+  if (prec == TNH_PREC_F32 && is_double(x->dtype) && x->numel >= (int64_t(1) << 20)) {
+    const int tx = is_complex(x->dtype) ? complex_of(TNH_F32) : TNH_F32;
+    char *xs = static_cast<char *>(ctx->workspace(813, size_t(x->numel) * 4 * (is_complex(x->dtype) ? 2 : 1)));
+    reformat(ctx, x->ptr, x->dtype, false, xs, tx, false, x->numel);
+    xop.re = xs;
+    xop.im = is_complex(x->dtype) ? xs + size_t(x->numel) * 4 : nullptr;
+    xop.dtype = TNH_F32;
+  }
   Gemm g1;
   g1.M = int(cl * D1);
   g1.N = int(d * d * cr_);
@@ -422,7 +434,7 @@ void heff_apply(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *W1, const tnh_buf
   g1.lda = cl * D1;
   g1.ldb = cl;
   g1.ldc = cl * D1;
-  cgemm(ctx, g1, cop(L), cop(x), T1, t1c ? T1 + n1 * rs : nullptr, real_of(tT1), prec);
+  cgemm(ctx, g1, cop(L), xop, T1, t1c ? T1 + n1 * rs : nullptr, real_of(tT1), prec);
 
   // (2) T2[a,(o1,o2,w2),b] = T1[a,(w,s1,s2),b] Wc[(w,s1,s2),(o1,o2,w2)]
   Small s;

@@ -6,11 +6,6 @@
 
 #include "tnh_internal.hpp"
 
-namespace tnh {
-void reformat(tnh_ctx *ctx, const void *src, int tsrc, bool src_interleaved, void *dst,
-              int tdst, bool dst_interleaved, int64_t n);
-}
-
 // ------------------------------------------------------------------ allocator
 
 void *tnh_ctx::alloc(size_t bytes, size_t *cap_out) {

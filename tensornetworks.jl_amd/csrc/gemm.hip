@@ -131,7 +131,12 @@ void gemm(tnh_ctx *ctx, const Gemm &g, int prec) {
     // alignment class as C): one untimed launch, then two timed ones, best kept
     const size_t cbytes = (size_t(g.ldc) * (g.N - 1) + g.M) * real_size(g.tC);
     size_t cap = 0;
-    void *scratch = ctx->alloc(cbytes, &cap);
+    void *scratch = nullptr;
+    try {
+      scratch = ctx->alloc(cbytes, &cap);
+    } catch (const Error &) {
+      return rule;  // no memory for a scratch output: the size rule, re-tried on a later call
+    }
     tnh_ctx::TileChoice tc;
     tc.rule = rule;
     tc.M = q.M;

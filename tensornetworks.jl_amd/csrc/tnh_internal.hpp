@@ -204,6 +204,9 @@ void small_contract(tnh_ctx *ctx, const Small &s);
 void convert(tnh_ctx *ctx, const void *src, int tsrc, void *dst, int tdst, int64_t n);
 void scale(tnh_ctx *ctx, void *x, int t, int64_t n, double a);
 void fill(tnh_ctx *ctx, void *x, int t, int64_t n, double v);
+// Convert n elements between dtypes and complex layouts (planar or interleaved).
+void reformat(tnh_ctx *ctx, const void *src, int tsrc, bool src_interleaved, void *dst,
+              int tdst, bool dst_interleaved, int64_t n);
 void axpby(tnh_ctx *ctx, int64_t n, double a, const double *x, double b, double *y);
 // Device-to-device copy by a kernel on the context's stream.
 void copy_device(tnh_ctx *ctx, void *dst, const void *src, size_t bytes);
