@@ -88,7 +88,8 @@ int tnh_profile_read(tnh_ctx *ctx, const char *family, int64_t *launches, double
  * (arithmetic mode, operand types, transposes, vector paths, and M, N, K rounded to four
  * classes per octave) with at least 2^28 flops, both sides >= 96 and no batching times
  * every candidate tile (one untimed and two timed launches each, on a scratch output; the
- * stream is synchronised once) and the fastest is reused for the class; smaller or
+ * stream is synchronised once); the fastest is kept for the class if it beats the size
+ * rule's tile by at least 3%, else the rule's tile (near-ties stay stable); smaller or
  * batched products, and products issued while the stream is being captured, take the
  * size rule. mode 0: always the size rule. mode 2: forget the tuned classes. Tiles
  * differ only in the output block a workgroup computes, never in the summation order,

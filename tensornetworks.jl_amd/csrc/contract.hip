@@ -87,6 +87,83 @@ __global__ __launch_bounds__(256) void small_kernel(SmallParams p) {
   }
 }
 
+// Vector form of small_kernel for a real input and I, J <= 32: each thread takes VA
+// consecutive a (one 16-byte load per input column), so a wave moves 1 KB per load or
+// store instruction and every mixing coefficient read from LDS serves VA multiply-adds.
+// The host checks that na, the offsets, the batch strides and the imaginary-plane offset
+// are multiples of VA and that both base pointers are 16-byte aligned.
+constexpr int kVecIJ = 32;
+template <typename TI, typename TO, bool CW, int VA>
+__global__ __launch_bounds__(256) void small_vec_kernel(SmallParams p) {
+  using R = typename std::conditional<std::is_same<TI, double>::value ||
+                                          std::is_same<TO, double>::value,
+                                      double, float>::type;
+  typedef TI VI __attribute__((ext_vector_type(VA)));
+  typedef TO VO __attribute__((ext_vector_type(VA)));
+  extern __shared__ double wsm[];
+  R *W = reinterpret_cast<R *>(wsm);
+  const int IJ = p.I * p.J;
+  for (int e = threadIdx.x; e < IJ * (CW ? 2 : 1); e += blockDim.x) W[e] = R(p.Wc[e]);
+  __syncthreads();
+
+  const int nav = p.na / VA;
+  const int64_t idx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (idx >= p.total / VA) return;
+  const int a = int(idx % nav) * VA;
+  const int64_t b = idx / nav;
+  const TI *in = static_cast<const TI *>(p.in) + a + b * p.sbI;
+  TO *out = static_cast<TO *>(p.out) + a + b * p.sbJ;
+
+  R v[kVecIJ][VA];
+#pragma unroll
+  for (int i = 0; i < kVecIJ; ++i) {
+    if (i < p.I) {
+      const VI x = *reinterpret_cast<const VI *>(in + p.offI[i]);
+#pragma unroll
+      for (int u = 0; u < VA; ++u) v[i][u] = R(x[u]);
+    }
+  }
+  for (int j = 0; j < p.J; ++j) {
+    const R *wr = W + j * p.I;
+    const R *wi = W + IJ + j * p.I;
+    R sr[VA], si[VA];
+#pragma unroll
+    for (int u = 0; u < VA; ++u) sr[u] = si[u] = R(0);
+#pragma unroll
+    for (int i = 0; i < kVecIJ; ++i) {
+      if (i < p.I) {
+        const R c = wr[i];
+#pragma unroll
+        for (int u = 0; u < VA; ++u) sr[u] += v[i][u] * c;
+        if constexpr (CW) {
+          const R ci = wi[i];
+#pragma unroll
+          for (int u = 0; u < VA; ++u) si[u] += v[i][u] * ci;
+        }
+      }
+    }
+    TO *o = out + p.offJ[j];
+    if (p.beta != 0.0) {
+      const VO x = *reinterpret_cast<const VO *>(o);
+#pragma unroll
+      for (int u = 0; u < VA; ++u) sr[u] += R(p.beta) * R(x[u]);
+      if constexpr (CW) {
+        const VO y = *reinterpret_cast<const VO *>(o + p.out_im);
+#pragma unroll
+        for (int u = 0; u < VA; ++u) si[u] += R(p.beta) * R(y[u]);
+      }
+    }
+    VO rv, iv;
+#pragma unroll
+    for (int u = 0; u < VA; ++u) {
+      rv[u] = TO(sr[u]);
+      iv[u] = TO(si[u]);
+    }
+    *reinterpret_cast<VO *>(o) = rv;
+    if constexpr (CW) *reinterpret_cast<VO *>(o + p.out_im) = iv;
+  }
+}
+
 // Wc[(w,s1,s2),(o1,o2,w2)] = sum_w1 W1[w,o1,s1,w1] W2[w1,o2,s2,w2] (planar complex out).
 template <typename T>
 __global__ void wc_pair_kernel(const T *W1, int64_t w1_im, const T *W2, int64_t w2_im, int D1,
@@ -209,6 +286,32 @@ void small_contract(tnh_ctx *ctx, const Small &s) {
   const size_t shm = size_t(s.I) * s.J * (cw ? 2 : 1) * ((i64 || o64) ? 8 : 4);
   dim3 grid(unsigned((p.total + 255) / 256));
   hipEvent_t ev = ctx->prof_begin();
+  // vector form: real input, I, J <= 32, everything a multiple of the 16-byte vector width
+  const int va = (i64 || o64) ? 2 : 4;  // FP64 arithmetic: two values per thread (register budget)
+  bool vec = !cin && s.I <= kVecIJ && s.J <= kVecIJ && s.na % va == 0 && s.sbI % va == 0 &&
+             s.sbJ % va == 0 && (!cw || s.out_im % va == 0) &&
+             (reinterpret_cast<uintptr_t>(s.in) & 15u) == 0 && (reinterpret_cast<uintptr_t>(s.out) & 15u) == 0;
+  for (int i = 0; vec && i < s.I; ++i) vec = s.offI[i] % va == 0;
+  for (int j = 0; vec && j < s.J; ++j) vec = s.offJ[j] % va == 0;
+  if (vec) {
+    dim3 gv(unsigned((p.total / va + 255) / 256));
+#define TNH_SMALLV(TI, TO, VA)                                                                 \
+  do {                                                                                          \
+    if (cw)                                                                                     \
+      hipLaunchKernelGGL((small_vec_kernel<TI, TO, true, VA>), gv, dim3(256), shm, ctx->stream, p); \
+    else                                                                                        \
+      hipLaunchKernelGGL((small_vec_kernel<TI, TO, false, VA>), gv, dim3(256), shm, ctx->stream, p); \
+  } while (0)
+    if (i64 && o64)
+      TNH_SMALLV(double, double, 2);
+    else if (i64)
+      TNH_SMALLV(double, float, 2);
+    else if (o64)
+      TNH_SMALLV(float, double, 2);
+    else
+      TNH_SMALLV(float, float, 4);
+#undef TNH_SMALLV
+  } else {
 #define TNH_SMALL(TI, TO)                                                                     \
   do {                                                                                          \
     if (cin && cw)                                                                              \
@@ -220,15 +323,16 @@ void small_contract(tnh_ctx *ctx, const Small &s) {
     else                                                                                        \
       hipLaunchKernelGGL((small_kernel<TI, TO, false, false>), grid, dim3(256), shm, ctx->stream, p); \
   } while (0)
-  if (i64 && o64)
-    TNH_SMALL(double, double);
-  else if (i64)
-    TNH_SMALL(double, float);
-  else if (o64)
-    TNH_SMALL(float, double);
-  else
-    TNH_SMALL(float, float);
+    if (i64 && o64)
+      TNH_SMALL(double, double);
+    else if (i64)
+      TNH_SMALL(double, float);
+    else if (o64)
+      TNH_SMALL(float, double);
+    else
+      TNH_SMALL(float, float);
 #undef TNH_SMALL
+  }
   TNH_HIP(hipGetLastError());
   if (ev) {
     const double planes_in = cin ? 2 : 1, planes_out = (cin || cw) ? 2 : 1;

@@ -32,6 +32,7 @@ constexpr int kNumTiles[3] = {3, 3, 2};
 // Products below this many flops (or with a side under 96, or batched) keep the size rule:
 // their launch latency outweighs the tile choice and timing them costs more than it saves.
 constexpr double kTuneMinFlops = 268435456.0;  // 2^28
+constexpr float kTuneMargin = 0.03f;  // a tile must beat the rule's by 3% to replace it
 
 // Four classes per octave: the exponent and the two bits below the leading one.
 int64_t size_class(int64_t x) {
@@ -148,7 +149,9 @@ void gemm(tnh_ctx *ctx, const Gemm &g, int prec) {
       s.C = scratch;
       s.beta = 0.0;
       float best = 0.0f;
-      // the rule's tile first, so that a tie keeps it
+      // the rule's tile first; another tile replaces it only when at least kTuneMargin
+      // faster, so that timing noise between near-equal tiles cannot flip the choice
+      float rule_ms = 0.0f;
       for (int i = 0; i < kNumTiles[mode]; ++i) {
         const int tile = i == 0 ? rule : (i == rule ? 0 : i);
         launch(mode, s, tile);
@@ -164,7 +167,10 @@ void gemm(tnh_ctx *ctx, const Gemm &g, int prec) {
         TNH_HIP(hipEventElapsedTime(&t2, ev1, ev2));
         const float t = std::min(t1, t2);
         tc.ms[tile] = t;
-        if (i == 0 || t < best) {
+        if (i == 0) {
+          rule_ms = best = t;
+          tc.tile = tile;
+        } else if (t < best && t < (1.0f - kTuneMargin) * rule_ms) {
           best = t;
           tc.tile = tile;
         }

@@ -1,7 +1,8 @@
-"""GEMM tile autotuner (tnh_gemm_autotune): each tuned size class keeps its fastest timed
-candidate tile, the choice is reused, small or batched products keep the size rule, and
-results are bitwise identical to the size rule's tile (tiles differ only in the output
-block a workgroup computes, not in the order of the sum over k)."""
+"""GEMM tile autotuner (tnh_gemm_autotune): each tuned size class keeps the size rule's
+tile unless a timed candidate is at least 3% faster, the choice is reused, small or batched
+products keep the size rule, and results are bitwise identical to the size rule's tile
+(tiles differ only in the output block a workgroup computes, not in the order of the sum
+over k)."""
 import numpy as np
 import pytest
 
@@ -44,8 +45,13 @@ def test_autotuned_tile_matches_rule_bitwise(ctx, rng, prec, ops):
         assert len(e["ms"]) == NCAND[prec] and all(t > 0 for t in e["ms"])
         shapes = {"f64": ["64x64", "128x64", "128x128"], "f32": ["64x64", "128x128", "128x64"],
                   "bf16x3": ["64x64", "128x128"]}[prec]
-        # the chosen tile is the fastest candidate (ties keep the rule's tile)
-        assert e["ms"][shapes.index(e["tile"])] == min(e["ms"])
+        # the rule's tile is kept unless the fastest candidate beats it by at least 3%
+        # (the report rounds to 1e-4 ms)
+        ti, ri = shapes.index(e["tile"]), shapes.index(e["rule"])
+        if ti == ri:
+            assert min(e["ms"]) >= 0.97 * e["ms"][ri] - 2e-4
+        else:
+            assert e["ms"][ti] == min(e["ms"]) and e["ms"][ti] <= 0.97 * e["ms"][ri] + 2e-4
         again = run_gemm(ctx, prec, ops, A, B, C0, M, N, K)
         assert len(ctx.gemm_tuning()) == 1  # reused, not re-timed
         assert np.array_equal(tuned, rule)
