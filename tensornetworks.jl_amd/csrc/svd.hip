@@ -927,6 +927,11 @@ static Schedule schedule_for(tnh_ctx *ctx, int cols, int width) {
 // Columns are cut into blocks of width B (pairs of 2B columns); the two-sided problem (real
 // or Hermitian) uses B = 32, which halves the rounds per sweep and so the passes over G and
 // V, the one-sided fallback B = JB = 16.
+// Two-sided problems up to this size use 16-wide blocks: twice the rounds of 32-wide ones,
+// but a round is then latency- rather than throughput-bound and its 32 x 32 pair solver
+// takes a fraction of the 64 x 64 one's time.
+constexpr int kNarrowMax = 512;
+
 template <int B>
 static JacobiResult jacobi_blocks(tnh_ctx *ctx, bool two_sided, double *X, int rows, int n, bool cplx,
                                   double *V, double tol, double abstol, int max_sweeps) {
@@ -1084,7 +1089,7 @@ static JacobiResult jacobi_blocks(tnh_ctx *ctx, bool two_sided, double *X, int r
 
 static JacobiResult jacobi(tnh_ctx *ctx, bool two_sided, double *X, int rows, int n, bool cplx,
                            double *V, double tol, double abstol, int max_sweeps) {
-  if (two_sided)
+  if (two_sided && n > kNarrowMax)
     return jacobi_blocks<2 * JB>(ctx, two_sided, X, rows, n, cplx, V, tol, abstol, max_sweeps);
   return jacobi_blocks<JB>(ctx, two_sided, X, rows, n, cplx, V, tol, abstol, max_sweeps);
 }
