@@ -39,6 +39,22 @@ end
 const DEFAULT = Ref{Union{Nothing,Context}}(nothing)
 default_context() = something(DEFAULT[], (DEFAULT[] = Context(0)))
 
+# GEMM tile selection of a context: autotuned per size class (default) or the size rule.
+gemm_autotune!(ctx::Context, on::Bool) =
+    check(ccall((:tnh_gemm_autotune, libtnhip), Cint, (Ptr{Cvoid}, Cint), ctx.h, on ? 1 : 0),
+          ctx.h, "tnh_gemm_autotune")
+
+function gemm_tuning_report(ctx::Context)
+    need = Ref{Int64}(0)
+    check(ccall((:tnh_gemm_tuning_report, libtnhip), Cint, (Ptr{Cvoid}, Ptr{UInt8}, Int64, Ptr{Int64}),
+                ctx.h, C_NULL, 0, need), ctx.h, "tnh_gemm_tuning_report")
+    buf = Vector{UInt8}(undef, need[])
+    GC.@preserve buf check(ccall((:tnh_gemm_tuning_report, libtnhip), Cint,
+                                 (Ptr{Cvoid}, Ptr{UInt8}, Int64, Ptr{Int64}), ctx.h, buf, need[], need),
+                           ctx.h, "tnh_gemm_tuning_report")
+    return unsafe_string(pointer(buf))
+end
+
 # --------------------------------------------------------------------- buffers
 mutable struct DeviceTensor
     ctx::Context
