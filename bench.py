@@ -161,13 +161,14 @@ def dmrg_heisenberg_n32(cpu=True):
     return out
 
 
-def dmrg_tfim_n100(chi, sweeps, refine_tol=None, precisions=("f64", "f32")):
+def dmrg_tfim_n100(chi, sweeps, refine_tol=None, precisions=("f64", "f32"), cpu=False):
     """Critical TFIM N = 100 (g = 1), random chi = 16 start (seed 0), cutoff 1e-12: an
     all-FP64 run and lower-precision runs (FP32-MFMA, split-BF16x3) with FP64 refinement
     passes, the same number of passes each. The lower precision switches to FP64 when the
     relative energy change of a pass falls below refine_tol (None: the dmrg default for the
     precision; 0: only for the last two passes). Reports per-pass times and energies and
-    the energy errors per site."""
+    the energy errors per site; with cpu, also the FP64 CPU port from the same initial
+    tensors and its energy difference to the FP64 GPU run."""
     from tnhip import MPO, randomMPS, spinhalf
     from tnhip.mps import dmrg
 
@@ -175,8 +176,11 @@ def dmrg_tfim_n100(chi, sweeps, refine_tol=None, precisions=("f64", "f32")):
     e_exact = 1.0 - 1.0 / np.sin(np.pi / (4 * N + 2))  # open critical TFIM closed form
     M = MPO(spinhalf(), tfim_oplist(N))
     res = {}
+    start = None
     for prec in precisions:
         psi = randomMPS(2, N, 16, rng=np.random.default_rng(0))
+        if start is None:
+            start = [t.copy() for t in psi.tensors], psi.center
         hist = []
         t0 = time.perf_counter()
         psi, e = dmrg(psi, M, maxsweeps=sweeps, minsweeps=sweeps, tol=0.0, cutoff=1e-12, maxdim=chi,
@@ -196,6 +200,19 @@ def dmrg_tfim_n100(chi, sweeps, refine_tol=None, precisions=("f64", "f32")):
         if lo:
             res[f"{prec}_pass_speedup_vs_f64"] = sum(hi) / sum(lo)
         res[f"{prec}_run_speedup_vs_f64"] = res["f64"]["seconds"] / res[prec]["seconds"]
+    if cpu:
+        from oracle.dmrg import dmrg as dmrg_cpu
+        from oracle.mps import MPS
+
+        p = MPS([t.copy() for t in start[0]], start[1])
+        t0 = time.perf_counter()
+        _, ec = dmrg_cpu(p, [np.asarray(w) for w in M.tensors], minsweeps=sweeps, maxsweeps=sweeps, tol=0.0,
+                         cutoff=1e-12, maxdim=chi)
+        res["cpu_port"] = {"seconds": time.perf_counter() - t0, "energy": ec,
+                           "abs_dE_per_site_vs_exact": abs(ec - e_exact) / N}
+        if "f64" in res:
+            res["abs_dE_per_site_f64_vs_cpu"] = abs(res["f64"]["energy"] - ec) / N
+            res["f64_speedup_vs_cpu_port"] = res["cpu_port"]["seconds"] / res["f64"]["seconds"]
     res["config"] = {"N": N, "g": 1.0, "chi_max": chi, "passes": sweeps, "cutoff": 1e-12, "krylovdim": 3,
                      "refine_tol": refine_tol, "E0_exact": e_exact}
     return res
@@ -489,7 +506,7 @@ def main(argv=None):
     ctx = tnhip.Context.default()
     if a.dmrg_only:
         print(json.dumps({"dmrg_tfim_N100": dmrg_tfim_n100(a.dmrg_chi, a.dmrg_passes, a.refine_tol,
-                                                           ("f64", "f32", "bf16x3"))}))
+                                                           ("f64", "f32", "bf16x3"), cpu=not a.no_cpu)}))
         return 0
     if a.dmrg_small_only:
         print(json.dumps({"dmrg_heisenberg_N32": dmrg_heisenberg_n32(cpu=not a.no_cpu)}))
@@ -597,7 +614,8 @@ def main(argv=None):
         bond_steps(ctx, ops, Tensor, dL64, dR64, dL32, dR32, dW, dx, chi, torch)
         extra["dmrg_bond_step_chi%d" % chi] = bond_steps(ctx, ops, Tensor, dL64, dR64, dL32, dR32, dW, dx,
                                                           chi, torch)
-        extra["dmrg_tfim_N100"] = dmrg_tfim_n100(a.dmrg_chi, a.dmrg_passes, a.refine_tol, ("f64", "f32", "bf16x3"))
+        extra["dmrg_tfim_N100"] = dmrg_tfim_n100(a.dmrg_chi, a.dmrg_passes, a.refine_tol, ("f64", "f32", "bf16x3"),
+                                                 cpu=not a.no_cpu)
         extra["dmrg_heisenberg_N32"] = dmrg_heisenberg_n32(cpu=not a.no_cpu)
     if not a.no_extra_legs:
         extra["gradient_tfim_N64"] = gradient_tfim_n64()
