@@ -122,6 +122,25 @@ def test_heff_complex_vs_oracle(ctx, rng):
     assert rel(out.numpy(), ref) <= 1e-13
 
 
+@pytest.mark.parametrize("cplx", [False, True])
+def test_heff_f32_large_theta_vs_oracle(ctx, rng, cplx):
+    """FP32 matvec on a theta of >= 2^20 elements, where theta is first rounded to an FP32
+    copy (chi = 512: 4 chi^2 = 2^20), real and complex, against the FP64 oracle."""
+    from tnhip import Tensor, ops as T
+    from oracle.projmps import product_optimal
+
+    chi = 512
+    W = heisenberg_w()
+    L, R = random_env(rng, chi, 5, cplx), random_env(rng, chi, 5, cplx)
+    x = randn(rng, (chi, 2, 2, chi), cplx)
+    ref = product_optimal(L, W, W, R, x, coeff=0.7)
+    cdt = np.complex64 if cplx else np.float32
+    dL, dR = Tensor.from_numpy(L, cdt), Tensor.from_numpy(R, cdt)
+    out = T.heff_apply(dL, Tensor.from_numpy(W), Tensor.from_numpy(W), dR, Tensor.from_numpy(x), coeff=0.7,
+                       precision="f32")
+    assert rel(out.numpy(), ref) <= tol_for("f32", 5 * chi) * 4
+
+
 def test_heff_edge_blocks(ctx, rng):
     """L = R = None stands for the ones(1,1,1) block at the chain ends."""
     from tnhip import Tensor, ops as T
