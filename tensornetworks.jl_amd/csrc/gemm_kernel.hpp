@@ -354,6 +354,10 @@ __global__ __launch_bounds__(256) void gemm_kernel(Params p) {
       }
     }
 
+    // FP32-output modes run the matrix-core section at raised wave priority: the CU's
+    // other workgroup then issues its loads and LDS stores around it instead of delaying
+    // the MFMA stream (+1.9% for FP32; FP64 64 x 64 tiles lost 11% with it).
+    if constexpr (MODE != MODE_F64) __builtin_amdgcn_s_setprio(1);
     if constexpr (MODE == MODE_BF16P) {
       constexpr int LD = TileA::LDK;
       v8us ah[FM], al[FM], bh[FN], bl[FN];
@@ -403,6 +407,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(Params p) {
             }
       }
     }
+    if constexpr (MODE != MODE_F64) __builtin_amdgcn_s_setprio(0);
   }
 
   // ---- epilogue: accumulator register r of lane (g, rr) holds C[row][rr] of each
