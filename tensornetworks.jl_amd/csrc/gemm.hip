@@ -121,13 +121,13 @@ void gemm(tnh_ctx *ctx, const Gemm &g, int prec) {
     if (!ctx->gemm_autotune || g.batch != 1 || g.M < 96 || g.N < 96 ||
         2.0 * double(q.M) * q.N * q.K < kTuneMinFlops)
       return rule;
-    hipStreamCaptureStatus cap_status = hipStreamCaptureStatusNone;
-    TNH_HIP(hipStreamIsCapturing(ctx->stream, &cap_status));
-    if (cap_status != hipStreamCaptureStatusNone) return rule;  // no synchronisation while captured
     const std::array<int64_t, 12> key = {mode, g.transA, g.transB, g.tA, g.tB, g.tC, q.vecA, q.vecB,
                                          q.vecC, size_class(q.M), size_class(q.N), size_class(q.K)};
     auto it = ctx->gemm_tiles.find(key);
     if (it != ctx->gemm_tiles.end()) return it->second.tile;
+    hipStreamCaptureStatus cap_status = hipStreamCaptureStatusNone;
+    TNH_HIP(hipStreamIsCapturing(ctx->stream, &cap_status));
+    if (cap_status != hipStreamCaptureStatusNone) return rule;  // no synchronisation while captured
     // time every candidate writing a scratch C (beta = 0, same leading dimension and
     // alignment class as C): one untimed launch, then two timed ones, best kept
     const size_t cbytes = (size_t(g.ldc) * (g.N - 1) + g.M) * real_size(g.tC);
@@ -143,8 +143,16 @@ void gemm(tnh_ctx *ctx, const Gemm &g, int prec) {
     tc.M = q.M;
     tc.N = q.N;
     tc.K = q.K;
-    hipEvent_t ev0 = ctx->take_event(), ev1 = ctx->take_event(), ev2 = ctx->take_event();
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    auto give_back = [&]() {
+      for (hipEvent_t e : {ev0, ev1, ev2})
+        if (e) ctx->event_pool.push_back(e);
+      ctx->release(scratch, cap);
+    };
     try {
+      ev0 = ctx->take_event();
+      ev1 = ctx->take_event();
+      ev2 = ctx->take_event();
       gk::Params s = q;
       s.C = scratch;
       s.beta = 0.0;
@@ -176,16 +184,10 @@ void gemm(tnh_ctx *ctx, const Gemm &g, int prec) {
         }
       }
     } catch (...) {
-      ctx->event_pool.push_back(ev0);
-      ctx->event_pool.push_back(ev1);
-      ctx->event_pool.push_back(ev2);
-      ctx->release(scratch, cap);
+      give_back();
       throw;
     }
-    ctx->event_pool.push_back(ev0);
-    ctx->event_pool.push_back(ev1);
-    ctx->event_pool.push_back(ev2);
-    ctx->release(scratch, cap);
+    give_back();
     ctx->gemm_tiles[key] = tc;
     return tc.tile;
   };
