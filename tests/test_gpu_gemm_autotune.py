@@ -105,3 +105,31 @@ def test_matvec_autotuned_equals_rule(ctx, rng, prec):
     finally:
         ctx.gemm_autotune(True)
         ctx.gemm_autotune_reset()
+
+
+def test_tuning_report_truncation_and_bad_mode(ctx, rng):
+    """tnh_gemm_tuning_report writes at most len bytes (NUL-terminated) and returns the full
+    size; tnh_gemm_autotune rejects modes other than 0, 1, 2."""
+    import ctypes as C
+
+    import tnhip
+    from tnhip._lib import lib
+
+    try:
+        ctx.gemm_autotune(True)
+        ctx.gemm_autotune_reset()
+        M, N, K = 1000, 900, 600
+        run_gemm(ctx, "f32", "NN", randn(rng, (M, K)), randn(rng, (K, N)), randn(rng, (M, N)), M, N, K)
+        need = C.c_int64()
+        assert lib.tnh_gemm_tuning_report(ctx.h, None, 0, C.byref(need)) == 0
+        full = C.create_string_buffer(int(need.value))
+        assert lib.tnh_gemm_tuning_report(ctx.h, full, int(need.value), C.byref(need)) == 0
+        assert len(full.value) + 1 == need.value and full.value.startswith(b"f32 NN M=1000 N=900 K=600")
+        short = C.create_string_buffer(8)
+        assert lib.tnh_gemm_tuning_report(ctx.h, short, 8, C.byref(need)) == 0
+        assert short.value == full.value[:7] and need.value == len(full.value) + 1
+        with pytest.raises(tnhip.TnhError):
+            ctx.call("tnh_gemm_autotune", 3)
+    finally:
+        ctx.gemm_autotune(True)
+        ctx.gemm_autotune_reset()
