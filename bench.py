@@ -440,6 +440,34 @@ def matvec_chi_sweep(ctx, ops, Tensor, torch, chis=(256, 512, 1024), seed=0, D=5
     return out
 
 
+def svd_vs_cpu(ctx, ops, Tensor, torch, n=2048, seed=0):
+    """Truncated SVD of one seeded n x n N(0,1) matrix (kept rank n / 2, as a chi = n / 2
+    split) on the GPU (warm) and LAPACK gesdd through numpy on the host cores (the routine
+    the reference's svd calls, src/tensors.jl:190), with the largest singular-value
+    difference between them relative to the largest singular value."""
+    g = torch.Generator(device="cpu").manual_seed(seed + 7)
+    Mh = torch.randn(n, n, generator=g, dtype=torch.float64)
+    Md = Mh.to("cuda")
+    M = Tensor.wrap(Md.data_ptr(), (n, n), np.float64, owner=Md, ctx=ctx)
+    times = []
+    for _ in range(2):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        U, S, Vt, disc, fb = ops.svd_trunc(M, 0.0, n // 2, 1, want=(True, True, True))
+        ctx.synchronize()
+        times.append(time.perf_counter() - t0)
+        s_gpu = np.sort(S.numpy())[::-1]
+        del U, S, Vt
+    # the GPU read the buffer column-major, i.e. M^T: the same singular values
+    t0 = time.perf_counter()
+    s_cpu = np.linalg.svd(Mh.numpy(), compute_uv=True, full_matrices=False)[1]
+    cpu_s = time.perf_counter() - t0
+    k = min(len(s_gpu), n // 2)
+    return {"n": n, "kept": int(k), "gpu_seconds": times[1], "cpu_seconds": cpu_s,
+            "cpu_threads": torch.get_num_threads(), "speedup_vs_cpu": cpu_s / times[1],
+            "max_abs_ds_rel": float(np.max(np.abs(s_gpu[:k] - s_cpu[:k])) / s_cpu[0])}
+
+
 def tile_choices(ctx):
     """The autotuner's table in short form: mode, transposes, M x N x K -> chosen tile."""
     return ["%s %s %dx%dx%d -> %s (rule %s)" % (e["mode"], e["trans"], e["M"], e["N"], e["K"], e["tile"], e["rule"])
@@ -609,6 +637,8 @@ def main(argv=None):
             del U, S, Vt
         extra["svd_2chi"] = {"seconds": times[1], "seconds_first": times[0], "kept": kept, "discarded": disc,
                              "fallback": fb, "n": 2 * chi}
+        if not a.no_cpu:
+            extra["svd_n2048_vs_cpu"] = svd_vs_cpu(ctx, ops, Tensor, torch, n=2048, seed=a.seed)
     if not a.no_dmrg:
         # one untimed pass first: workspaces, schedules and GEMM tiles are set up on first use
         bond_steps(ctx, ops, Tensor, dL64, dR64, dL32, dR32, dW, dx, chi, torch)
