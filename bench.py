@@ -315,12 +315,15 @@ def qjmc_tfim_n40(ntraj=512, steps=200, sample_traj=16, sample_steps=20, nworker
                           "jump_records_equal_batched": [j for j in jumps] == a["jumps"][0]}}
 
 
-def tebd_tfim_n40(steps=80, dt=0.05, chi_max=256, cutoff=1e-10):
+def tebd_tfim_n40(steps=80, dt=0.05, chi_max=256, cutoff=1e-10, cpu_steps=0):
     """Real-time TEBD of the critical TFIM quench, N = 40, g = 1, from all spins up:
     second-order Trotter (exp(-i dt H), 60 two-site gates per step), cutoff 1e-10, chi_max
     256, the centre moved by truncated splits before each gate (applygates!, gatelist.jl:
     191-223) and the state renormalised every step. Complex arithmetic throughout. Reports
-    seconds per Trotter step at the end of the run and the bond dimension reached."""
+    seconds per Trotter step at the end of the run and the bond dimension reached. With
+    cpu_steps, the final state also takes cpu_steps more steps in the CPU port (numpy/LAPACK
+    on the host cores) and on the GPU, timed side by side, with the largest difference of
+    the local <sz_i> after them."""
     from tnhip import productMPS, spinhalf
     from tnhip.models import OpList
     from tnhip.tebd import DeviceMPS, applygates, trotterize
@@ -345,10 +348,42 @@ def tebd_tfim_n40(steps=80, dt=0.05, chi_max=256, cutoff=1e-10):
         per_step.append(time.perf_counter() - t0)
         bonds.append(psi.maxbonddim())
     total = time.perf_counter() - t_all
-    return {"steps": steps, "dt": dt, "gates_per_step": ngates, "seconds": total,
-            "final_maxbonddim": bonds[-1], "last_step_seconds": per_step[-1],
-            "last_step_gates_per_s": ngates / per_step[-1],
-            "step_seconds_at": {str(bonds[i]): per_step[i] for i in (9, 39, 59, steps - 1) if i < steps}}
+    out = {"steps": steps, "dt": dt, "gates_per_step": ngates, "seconds": total,
+           "final_maxbonddim": bonds[-1], "last_step_seconds": per_step[-1],
+           "last_step_gates_per_s": ngates / per_step[-1],
+           "step_seconds_at": {str(bonds[i]): per_step[i] for i in (9, 39, 59, steps - 1) if i < steps}}
+    if cpu_steps:
+        import torch
+
+        from oracle.mps import MPS
+        from oracle.mps import normalize as normalize_cpu
+        from oracle.tebd import applygates_mps
+        from tnhip.tebd import inner
+
+        zs = OpList(N)
+        for i in range(1, N + 1):
+            zs.add("z", i)
+        p = MPS([t.numpy() for t in psi.sites], psi.center)
+        rows = list(zip(gates.sites, gates.gates))
+        t0 = time.perf_counter()
+        for _ in range(cpu_steps):
+            applygates_mps(p, rows, cutoff=cutoff, maxdim=chi_max)
+            normalize_cpu(p)
+        cpu_s = (time.perf_counter() - t0) / cpu_steps
+        t0 = time.perf_counter()
+        for _ in range(cpu_steps):
+            applygates(psi, gates, cutoff=cutoff, maxdim=chi_max)
+            psi.normalize()
+        psi.ctx.synchronize()
+        gpu_s = (time.perf_counter() - t0) / cpu_steps
+        # both states measured by the same (GPU) contraction of <sz_i>
+        z_gpu = np.real(inner(st, psi, zs))
+        z_cpu = np.real(inner(st, DeviceMPS(p.tensors, p.center, ctx=psi.ctx), zs))
+        out["cpu_port_sample"] = {"steps": cpu_steps, "from_maxbonddim": bonds[-1],
+                                  "cpu_seconds_per_step": cpu_s, "gpu_seconds_per_step": gpu_s,
+                                  "cpu_threads": torch.get_num_threads(), "speedup_vs_cpu": cpu_s / gpu_s,
+                                  "max_abs_dsz": float(np.max(np.abs(z_gpu - z_cpu)))}
+    return out
 
 
 def bond_step(ctx, ops, Tensor, dL, dR, dW, dx, prec, chi, torch):
@@ -540,7 +575,7 @@ def main(argv=None):
         print(json.dumps({"dmrg_heisenberg_N32": dmrg_heisenberg_n32(cpu=not a.no_cpu)}))
         return 0
     if a.tebd_only:
-        print(json.dumps({"tebd_tfim_N40": tebd_tfim_n40()}))
+        print(json.dumps({"tebd_tfim_N40": tebd_tfim_n40(cpu_steps=0 if a.no_cpu else 2)}))
         return 0
     if a.qjmc_only:
         print(json.dumps({"qjmc_tfim_N40": qjmc_tfim_n40(sample_traj=0)}))
@@ -650,7 +685,7 @@ def main(argv=None):
     if not a.no_extra_legs:
         extra["gradient_tfim_N64"] = gradient_tfim_n64()
         extra["qjmc_tfim_N40"] = qjmc_tfim_n40()
-        extra["tebd_tfim_N40"] = tebd_tfim_n40()
+        extra["tebd_tfim_N40"] = tebd_tfim_n40(cpu_steps=0 if a.no_cpu else 2)
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline()

@@ -89,3 +89,68 @@ def expect_dense(v, N, op, site):
     """<v| op_site |v> for a one-site operator."""
     w = apply_dense(v, site, op, N)
     return np.vdot(v, w)
+
+
+def rows_from_layers(layers, d=2):
+    """Trotter layers of dense gates (kron order, site slowest) as the rows of a GateList:
+    (sites, tensors) with two-site gates G[o1, i1, o2, i2] and one-site gates O[o, i]."""
+    rows = []
+    for layer in layers:
+        sites, tensors = [], []
+        for site, U in layer:
+            sites.append(site)
+            if U.shape[0] == d * d:
+                tensors.append(np.transpose(U.reshape(d, d, d, d), (0, 2, 1, 3)))
+            else:
+                tensors.append(np.asarray(U))
+        rows.append((sites, tensors))
+    return rows
+
+
+def applygate_mps(psi, site, gate, direction, cutoff=0.0, maxdim=0, mindim=1):
+    """applygate(psi, site, gate, direction; cutoff, maxdim, mindim, error=false),
+    src/structures/mps/gatelist.jl:137-171, on an oracle MPS: theta = A1 A2 (:142-145),
+    theta'[a,o1,o2,b] = sum G[o1,i1,o2,i2] theta[a,i1,i2,b] (:149-154), split without
+    normalisation (:157). A one-site gate O[o, i] acts on the site, which becomes the centre,
+    and the centre then moves one site on (replacesites!'s one-site branch, gmps.jl:204-213).
+    Returns the discarded weight."""
+    from .mps import movecenter, replacesites2
+
+    g = np.asarray(gate)
+    if g.ndim == 4:
+        theta = np.tensordot(psi[site], psi[site + 1], axes=([2], [0]))
+        theta = np.einsum("xiyj,aijb->axyb", g, theta)
+        A1, A2, _, disc = replacesites2(theta, direction, False, cutoff, maxdim, mindim)
+        psi[site], psi[site + 1] = A1, A2
+        psi.center = site if direction else site + 1
+        return disc
+    if g.ndim == 2:
+        psi[site] = np.einsum("st,atb->asb", g, psi[site])
+        psi.center = site
+        nxt = site + 1 - 2 * int(bool(direction))
+        if 0 < nxt <= len(psi):
+            movecenter(psi, nxt)
+        return 0.0
+    raise ValueError("gates act on one or two sites")
+
+
+def applygates_mps(psi, rows, cutoff=0.0, maxdim=0, mindim=1):
+    """applygates!(psi, gates; kwargs...), gatelist.jl:191-223: per row, the direction that
+    starts at the end of the row nearer the centre (:201), then for each gate a truncated
+    centre move to the gate (:208-214) and the gate (:217). `rows` as rows_from_layers."""
+    from .mps import movecenter
+
+    def size(g):
+        return np.ndim(g) // 2
+
+    for sites, gates in rows:
+        if not sites:
+            continue
+        first, last = sites[0], sites[-1] + size(gates[-1]) - 1
+        direction = not (abs(psi.center - first) < abs(psi.center - last))
+        n = len(gates)
+        for i in range(1, n + 1):
+            k = n + 1 - i if direction else i
+            ctr = sites[k - 1] if not direction else sites[k - 1] + size(gates[k - 1]) - 1
+            movecenter(psi, ctr, cutoff=cutoff, maxdim=maxdim, mindim=mindim)
+            applygate_mps(psi, sites[k - 1], gates[k - 1], direction, cutoff, maxdim, mindim)

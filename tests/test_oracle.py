@@ -269,3 +269,32 @@ def test_oracle_dmrg_matches_ed(rng):
     psi = random_mps(N, 2, 4, rng)
     psi, e = dmrg(psi, Ws, maxdim=32, cutoff=1e-12, maxsweeps=8, krylovdim=4)
     assert abs(e - exact.tfim_critical_ground_energy(N)) < 1e-9
+
+
+@pytest.mark.parametrize("evol", ["real", "imag"])
+def test_applygates_mps_matches_dense(evol):
+    """The oracle's MPS applygates! (gatelist.jl:191-223) reproduces the dense Trotter
+    evolution when nothing is truncated. Odd N, so every second layer ends in a one-site
+    gate; x and z fields and a non-symmetric start, so a transposed gate index or a
+    mirrored site order fails."""
+    from oracle.mps import normalize, product_mps, to_dense
+    from oracle.tebd import applygates_mps, rows_from_layers, tebd_dense, trotter_layers
+
+    N, dt, steps = 7, 0.05, 6
+    terms = ([(["z", "z"], [i, i + 1], -1.0) for i in range(1, N)]
+             + [(["x"], [i], -0.7) for i in range(1, N + 1)]
+             + [(["z"], [i], -0.2) for i in range(1, N + 1)])
+    layers = trotter_layers(terms, N, dt, order=2, evol=evol)
+    up, dn = np.array([1.0, 0.0]), np.array([0.0, 1.0])
+    states = [up, up, dn, up, dn, dn, up]
+    psi = product_mps(states)
+    rows = rows_from_layers(layers)
+    assert any(np.ndim(g) == 2 for _, gs in rows for g in gs)
+    for _ in range(steps):
+        applygates_mps(psi, rows)
+        normalize(psi)
+    v0 = states[0]
+    for s in states[1:]:
+        v0 = np.kron(v0, s)
+    v = tebd_dense(v0.astype(complex), layers, N, steps)
+    assert np.allclose(to_dense(psi), v, atol=1e-10)
