@@ -220,120 +220,6 @@ __global__ __launch_bounds__(256) void jac_colupd_kernel(double *__restrict__ X,
   }
 }
 
-// Two-sided mode, both sides in one pass: G[P, K] <- Q_P^H G[P, K] Q_K for every pair of
-// pairs (P = blockIdx.x rows, K = blockIdx.y columns), Q of an inactive pair being the
-// identity; the diagonal tiles take the diagonalised blocks D. One 32 x 32 tile and the
-// two rotations in LDS; G is read and written once per round.
-template <bool CPLX>
-__global__ __launch_bounds__(256) void jac_twoside_kernel(double *__restrict__ G, int n, int64_t imG,
-                                                          const int *__restrict__ pairs,
-                                                          const double *__restrict__ Qin,
-                                                          const double *__restrict__ Din,
-                                                          const int *__restrict__ active) {
-  const int P = blockIdx.x, K = blockIdx.y, t = threadIdx.x;
-  const bool aP = active[P] != 0, aK = active[K] != 0;
-  if (!aP && !aK) return;
-  __shared__ double tr[JP][JP + 1], ti[CPLX ? JP : 1][JP + 1];
-  __shared__ double pr_[JP][JP + 1], pi_[CPLX ? JP : 1][JP + 1];
-  __shared__ double kr_[JP][JP + 1], ki_[CPLX ? JP : 1][JP + 1];
-  __shared__ int rowmap[JP], colmap[JP];
-  if (t < JP) {
-    rowmap[t] = pair_col(pairs + 2 * P, t, n);
-    colmap[t] = pair_col(pairs + 2 * K, t, n);
-  }
-  __syncthreads();
-  if (P == K) {  // aP holds here
-    const double *D = Din + int64_t(P) * 2 * JP * JP;
-    for (int e = t; e < JP * JP; e += 256) {
-      const int i = e % JP, j = e / JP;
-      const int r = rowmap[i], c = colmap[j];
-      if (r < 0 || c < 0) continue;
-      G[r + int64_t(c) * n] = D[e];
-      if (CPLX) G[imG + r + int64_t(c) * n] = D[JP * JP + e];
-    }
-    return;
-  }
-  const double *QP = Qin + int64_t(P) * 2 * JP * JP;
-  const double *QK = Qin + int64_t(K) * 2 * JP * JP;
-  for (int e = t; e < JP * JP; e += 256) {
-    const int i = e % JP, j = e / JP;
-    const int r = rowmap[i], c = colmap[j];
-    const bool ok = r >= 0 && c >= 0;
-    tr[i][j] = ok ? G[r + int64_t(c) * n] : 0.0;
-    if (CPLX) ti[i][j] = ok ? G[imG + r + int64_t(c) * n] : 0.0;
-    if (aP) {
-      pr_[i][j] = QP[i * JP + j];
-      if (CPLX) pi_[i][j] = QP[JP * JP + i * JP + j];
-    }
-    if (aK) {
-      kr_[i][j] = QK[i * JP + j];
-      if (CPLX) ki_[i][j] = QK[JP * JP + i * JP + j];
-    }
-  }
-  __syncthreads();
-  constexpr int PER = JP * JP / 256;  // entries per thread
-  double sr[PER], si[PER];
-  if (aK) {  // T <- T Q_K
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int e = t + 256 * q, i = e % JP, j = e / JP;
-      double ar = 0, ai = 0;
-      for (int k = 0; k < JP; ++k) {
-        const double a = tr[i][k], c = kr_[k][j];
-        ar += a * c;
-        if (CPLX) {
-          const double b = ti[i][k], d = ki_[k][j];
-          ar -= b * d;
-          ai += a * d + b * c;
-        }
-      }
-      sr[q] = ar;
-      si[q] = ai;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int e = t + 256 * q, i = e % JP, j = e / JP;
-      tr[i][j] = sr[q];
-      if (CPLX) ti[i][j] = si[q];
-    }
-    __syncthreads();
-  }
-  if (aP) {  // T <- Q_P^H T
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int e = t + 256 * q, i = e % JP, j = e / JP;
-      double ar = 0, ai = 0;
-      for (int k = 0; k < JP; ++k) {
-        const double a = pr_[k][i], c = tr[k][j];  // conj(Q_P[k][i]) T[k][j]
-        ar += a * c;
-        if (CPLX) {
-          const double b = -pi_[k][i], d = ti[k][j];
-          ar -= b * d;
-          ai += a * d + b * c;
-        }
-      }
-      sr[q] = ar;
-      si[q] = ai;
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int e = t + 256 * q, i = e % JP, j = e / JP;
-      sr[q] = tr[i][j];
-      si[q] = CPLX ? ti[i][j] : 0.0;
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < PER; ++q) {
-    const int e = t + 256 * q, i = e % JP, j = e / JP;
-    const int r = rowmap[i], c = colmap[j];
-    if (r < 0 || c < 0) continue;
-    G[r + int64_t(c) * n] = sr[q];
-    if (CPLX) G[imG + r + int64_t(c) * n] = si[q];
-  }
-}
-
 typedef double jv4d __attribute__((ext_vector_type(4)));
 
 // Real two-sided mode on the matrix cores, exploiting the symmetry of G: one workgroup per
@@ -1010,15 +896,9 @@ static JacobiResult jacobi_blocks(tnh_ctx *ctx, bool two_sided, double *X, int r
           hipLaunchKernelGGL((jac_eig_kernel<false, B, NT>), dim3(s.npairs), dim3(NT), 0, ctx->stream, X,
                              imX, n, partial, nsplit, two_sided, pr, n, tol, abstol, max_inner, cross, Qb, D,
                              act, conv + sw);
-        if constexpr (B == JB) {
-          if (two_sided && cplx)
-            hipLaunchKernelGGL(jac_twoside_kernel<true>, dim3(s.npairs, s.npairs), dim3(256), 0,
-                               ctx->stream, X, n, imX, pr, Qb, D, act);
-        } else {
-          if (two_sided && cplx)
-            hipLaunchKernelGGL(jac_twoside_herm_kernel<B>, dim3(s.npairs * (s.npairs + 1) / 2), dim3(256), 0,
-                               ctx->stream, X, n, imX, pr, Qb, D, act);
-        }
+        if (two_sided && cplx)
+          hipLaunchKernelGGL(jac_twoside_herm_kernel<B>, dim3(s.npairs * (s.npairs + 1) / 2), dim3(256), 0,
+                             ctx->stream, X, n, imX, pr, Qb, D, act);
         if (two_sided && !cplx)
           hipLaunchKernelGGL(jac_twoside_sym_kernel<B>, dim3(s.npairs * (s.npairs + 1) / 2), dim3(256), 0,
                              ctx->stream, X, n, pr, Qb, D, act);
