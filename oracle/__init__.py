@@ -2,8 +2,9 @@
 
 This package restates, in numpy/LAPACK FP64, the algorithms of lcauser/TensorNetworks.jl
 that the HIP library replaces (each function cites the reference file:line it
-follows). It is imported only by tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg, as the checker — never by the product library.
+follows). It is imported only by tests/, __graft_entry__.smoke() and bench.py's CPU
+legs (cpu_baseline and the CPU-port timings beside the GPU legs), to compare against —
+never by the product library.
 
 The reference is Julia and cannot run here (no Julia toolchain; its dependencies
 TensorOperations.jl, KrylovKit.jl are absent), and its test suite holds no golden
