@@ -556,9 +556,9 @@ def main(argv=None):
         del dst
         return 0
     if a.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the matvec would shard theta along its right bond with a reduce-scatter; "
-              "see the design notes). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (the matvec "
+              "would shard theta along its right bond, with a reduce-scatter of the "
+              "contraction over that bond). Run with --gpus 1.", file=sys.stderr)
         return 2
 
     import torch
