@@ -291,8 +291,10 @@ int tnh_qjmc_run(tnh_ctx *ctx, const tnh_qjmc_problem *problem, int ntraj, const
  * one primitive (centre move, two-site gate split, one-site operator, normalisation) to
  * every trajectory, one workgroup each, with the Gram eigensolve of each split in LDS.
  * Replaces the per-trajectory loop over qjmc_simulation (qjmc.jl:28-167) for the small bond
- * dimensions of dissipative chains. Returns TNH_ERR_UNSUPPORTED (state discarded) when a
- * split would keep more than the slot capacity; tnh_qjmc_run has no such limit. */
+ * dimensions of dissipative chains. A trajectory whose split would keep more than the slot
+ * capacity stops there: its maxbond entry is set to -1 and its other outputs are undefined,
+ * while the other trajectories complete (the call returns TNH_OK); rerun the marked ones
+ * with tnh_qjmc_run, which has no such limit, on their own rows of uniforms. */
 int tnh_qjmc_run_batched(tnh_ctx *ctx, const tnh_qjmc_problem *problem, int ntraj,
                          const double *uniforms, int64_t nuniform, double *meas, int *njumps,
                          int *jumps, int *jump_steps, int maxjumps, int *maxbond);

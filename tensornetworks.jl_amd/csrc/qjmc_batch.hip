@@ -761,13 +761,13 @@ struct Engine {
   }
 };
 
-void check_flags(tnh_ctx *ctx, const Engine &E) {
+// Reads the trajectories' flags: a trajectory that outgrew its slots is marked in `dead` (it
+// stops; the caller reruns it elsewhere), any other flag is an error of the whole call.
+void check_flags(tnh_ctx *ctx, const Engine &E, std::vector<char> &dead) {
   std::vector<int> f(E.T);
   to_host(ctx, f.data(), E.b.flags, sizeof(int) * E.T);
   for (int t = 0; t < E.T; ++t) {
-    TNH_REQUIRE(!(f[t] & F_OVERFLOW), TNH_ERR_UNSUPPORTED,
-                "qjmc batched: trajectory " + std::to_string(t) + " needs a bond dimension above " +
-                    std::to_string(E.b.CH) + " (the per-trajectory runner has no such limit)");
+    if (f[t] & F_OVERFLOW) dead[t] = 1;
     TNH_REQUIRE(!(f[t] & F_NOCONV), TNH_ERR_NOCONV,
                 "qjmc batched: Gram eigensolve did not converge (trajectory " + std::to_string(t) + ")");
     TNH_REQUIRE(!(f[t] & F_NUMERIC), TNH_ERR_NUMERIC,
@@ -924,16 +924,22 @@ extern "C" int tnh_qjmc_run_batched(tnh_ctx *ctx, const tnh_qjmc_problem *p, int
     };
     measure();
     std::vector<double> rates(std::max(1, nesc));
+    std::vector<char> dead(ntraj, 0);  // trajectories that outgrew their slots
     for (int step = 1; step <= p->nsteps; ++step) {
-      for (auto &x : E.tq) E.push_applygates(x);
+      for (int t = 0; t < ntraj; ++t)
+        if (!dead[t]) E.push_applygates(E.tq[t]);
       E.run_queues();
-      for (int t = 0; t < ntraj; ++t) (void)draw(t);  // qjmc.jl:64: drawn and unused
-      normalize_all(all);
-      check_flags(ctx, E);
+      std::vector<int> live;
+      for (int t = 0; t < ntraj; ++t)
+        if (!dead[t]) live.push_back(t);
+      for (int t : live) (void)draw(t);  // qjmc.jl:64: drawn and unused
+      normalize_all(live);
+      check_flags(ctx, E, dead);
       std::vector<int> jumpers;
       if (nesc > 0) {
         expect(0, nesc);
         for (int t = 0; t < ntraj; ++t) {
+          if (dead[t]) continue;
           double er = 0;
           for (int q = 0; q < nesc; ++q) {
             const double c = p->jump_coeff[q];
@@ -965,7 +971,8 @@ extern "C" int tnh_qjmc_run_batched(tnh_ctx *ctx, const tnh_qjmc_problem *p, int
           }
         }
       } else {
-        for (int t = 0; t < ntraj; ++t) (void)draw(t);  // the jump test draws even with no channels
+        for (int t = 0; t < ntraj; ++t)
+          if (!dead[t]) (void)draw(t);  // the jump test draws even with no channels
       }
       if (!jumpers.empty()) {
         E.run_queues();
@@ -973,14 +980,14 @@ extern "C" int tnh_qjmc_run_batched(tnh_ctx *ctx, const tnh_qjmc_problem *p, int
       }
       if (step % p->savesteps == 0) measure();
     }
-    check_flags(ctx, E);
+    check_flags(ctx, E, dead);
     std::vector<int> bonds(size_t(ntraj) * (N + 1));
     to_host(ctx, bonds.data(), E.b.bond, sizeof(int) * bonds.size());
     for (int t = 0; t < ntraj; ++t) {
-      njumps[t] = nj[t];
+      njumps[t] = dead[t] ? 0 : nj[t];
       int mb = 0;
       for (int i = 1; i < N; ++i) mb = std::max(mb, bonds[size_t(t) * (N + 1) + i]);
-      maxbond[t] = mb;
+      maxbond[t] = dead[t] ? -1 : mb;
     }
   });
   // stream-ordered cache: later work on the stream reuses these only after this run's kernels

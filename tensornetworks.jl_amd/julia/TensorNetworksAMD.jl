@@ -265,6 +265,8 @@ end
 #   end
 
 # ------------------------------- many QJMC trajectories (tnh_qjmc_run(_batched))
+# The batched call marks a trajectory that outgrew its slots with maxbond = -1;
+# qjmc_trajectories! reruns those on the per-trajectory runner.
 const TNH_ERR_UNSUPPORTED = Cint(-7)
 
 # Layout of tnh_qjmc_problem (include/tnhip.h); the pointers refer to arrays the caller
@@ -300,19 +302,38 @@ function qjmc_trajectories!(prob::Ref{QjmcProblem}, U::Matrix{Float64}, meas::Ar
     nu, ntraj = size(U)
     maxjumps = size(jumps, 1)
     if batched
-        rc = ccall((:tnh_qjmc_run_batched, libtnhip), Cint,
-                   (Ptr{Cvoid}, Ptr{QjmcProblem}, Cint, Ptr{Float64}, Int64, Ptr{Float64},
-                    Ptr{Cint}, Ptr{Cint}, Ptr{Cint}, Cint, Ptr{Cint}),
-                   ctx.h, prob, ntraj, U, nu, meas, njumps, jumps, jsteps, maxjumps, maxb)
-        rc == 0 && return nothing
-        rc == TNH_ERR_UNSUPPORTED || check(rc, ctx.h, "tnh_qjmc_run_batched")
+        check(ccall((:tnh_qjmc_run_batched, libtnhip), Cint,
+                    (Ptr{Cvoid}, Ptr{QjmcProblem}, Cint, Ptr{Float64}, Int64, Ptr{Float64},
+                     Ptr{Cint}, Ptr{Cint}, Ptr{Cint}, Cint, Ptr{Cint}),
+                    ctx.h, prob, ntraj, U, nu, meas, njumps, jumps, jsteps, maxjumps, maxb),
+              ctx.h, "tnh_qjmc_run_batched")
+        over = findall(<(0), maxb)  # trajectories that outgrew the batched slots
+        isempty(over) && return nothing
+        # rerun only those on the per-trajectory runner, each with its own uniforms
+        m2 = zeros(Float64, size(meas)[1:end-1]..., length(over))
+        nj2 = zeros(Cint, length(over))
+        j2 = zeros(Cint, maxjumps, length(over))
+        s2 = zeros(Cint, maxjumps, length(over))
+        b2 = zeros(Cint, length(over))
+        _qjmc_workers!(prob, U[:, over], m2, nj2, j2, s2, b2, nworkers, ctx)
+        selectdim(meas, ndims(meas), over) .= m2
+        njumps[over] .= nj2
+        jumps[:, over] .= j2
+        jsteps[:, over] .= s2
+        maxb[over] .= b2
+        return nothing
     end
+    _qjmc_workers!(prob, U, meas, njumps, jumps, jsteps, maxb, nworkers, ctx)
+    return nothing
+end
+
+function _qjmc_workers!(prob, U, meas, njumps, jumps, jsteps, maxb, nworkers, ctx)
+    nu, ntraj = size(U)
     check(ccall((:tnh_qjmc_run, libtnhip), Cint,
                 (Ptr{Cvoid}, Ptr{QjmcProblem}, Cint, Ptr{Float64}, Int64, Cint, Ptr{Float64},
                  Ptr{Cint}, Ptr{Cint}, Ptr{Cint}, Cint, Ptr{Cint}),
-                ctx.h, prob, ntraj, U, nu, nworkers, meas, njumps, jumps, jsteps, maxjumps, maxb),
+                ctx.h, prob, ntraj, U, nu, nworkers, meas, njumps, jumps, jsteps, size(jumps, 1), maxb),
           ctx.h, "tnh_qjmc_run")
-    return nothing
 end
 
 export Context, GPUProjMPS, dmrg_gpu, mpo_energy_grad, QjmcProblem, qjmc_trajectories!

@@ -131,11 +131,12 @@ def qjmc_trajectories(st: Sitetypes, init_names, H: OpList, jumpops: OpList, tma
     be single-site (multi-site ones run through qjmc_simulation). engine: "batched" advances
     all trajectories together (tnh_qjmc_run_batched: one launch per primitive for every
     trajectory, bond dimension at most min(16, 32 / d)); "workers" runs them one by one on
-    `nworkers` streams (tnh_qjmc_run, no bond limit); "auto" tries the batched engine and
-    reruns on the workers when a bond dimension outgrows its slots. Returns a dict with
+    `nworkers` streams (tnh_qjmc_run, no bond limit); "auto" runs the batched engine and
+    reruns on the workers only the trajectories whose bond dimension outgrew its slots
+    (each with its own uniforms; "engine" then reads "batched+workers"). Returns a dict with
     `jumps` and `jumptimes` (lists per trajectory, 1-based indices, as the reference stores
     them), `measurements` (ntraj x nsave x nobs, coefficient * <O>), `times`, `maxbonddim`."""
-    from ._lib import QjmcProblem
+    from ._lib import QjmcProblem, TnhError
     from .core import Context
 
     ctx = ctx or Context.default()
@@ -201,17 +202,26 @@ def qjmc_trajectories(st: Sitetypes, init_names, H: OpList, jumpops: OpList, tma
         raise ValueError("engine must be 'auto', 'batched' or 'workers'")
     used = "workers"
     if engine in ("auto", "batched"):
-        from ._lib import TnhError
-
-        try:
-            ctx.call("tnh_qjmc_run_batched", C.byref(p), ntraj, dp(U), nu, dp(meas), ip(njumps), ip(jumps),
-                     ip(jsteps), int(maxjumps), ip(maxb))
-            used = "batched"
-        except TnhError as e:
-            if engine == "batched" or "bond dimension above" not in str(e):
-                raise
-            meas[...] = 0
-            njumps[...] = 0
+        ctx.call("tnh_qjmc_run_batched", C.byref(p), ntraj, dp(U), nu, dp(meas), ip(njumps), ip(jumps),
+                 ip(jsteps), int(maxjumps), ip(maxb))
+        used = "batched"
+        over = np.flatnonzero(maxb < 0)  # trajectories that outgrew the batched slots
+        if over.size:
+            if engine == "batched":
+                raise TnhError(-7, f"qjmc batched: {over.size} trajectories need a bond dimension above "
+                                   f"the slot capacity (first: {int(over[0])}); use engine='auto' or "
+                                   "'workers'", "tnh_qjmc_run_batched")
+            # rerun only those on the per-trajectory runner, each with its own uniforms
+            Us = np.ascontiguousarray(U[over])
+            m2 = np.zeros((over.size,) + meas.shape[1:])
+            nj2 = np.zeros(over.size, dtype=np.int32)
+            j2 = np.zeros((over.size, maxjumps), dtype=np.int32)
+            s2 = np.zeros((over.size, maxjumps), dtype=np.int32)
+            b2 = np.zeros(over.size, dtype=np.int32)
+            ctx.call("tnh_qjmc_run", C.byref(p), int(over.size), dp(Us), nu, int(nworkers), dp(m2), ip(nj2),
+                     ip(j2), ip(s2), int(maxjumps), ip(b2))
+            meas[over], njumps[over], jumps[over], jsteps[over], maxb[over] = m2, nj2, j2, s2, b2
+            used = "batched+workers"
     if used == "workers":
         ctx.call("tnh_qjmc_run", C.byref(p), ntraj, dp(U), nu, int(nworkers), dp(meas), ip(njumps), ip(jumps),
                  ip(jsteps), int(maxjumps), ip(maxb))

@@ -116,7 +116,8 @@ def test_batched_matches_workers_truncated(ctx):
 def test_batched_capacity_overflow(ctx):
     """Without truncation the bond dimension of a 10-site chain reaches 32, above the
     batched slots (16 for spin-1/2): engine="batched" raises, engine="auto" reruns the
-    trajectories on the per-trajectory workers and returns their result."""
+    marked trajectories (here all of them) on the per-trajectory workers and returns their
+    result."""
     from tnhip import spinhalf, TnhError
     from tnhip.qjmc import qjmc_trajectories
 
@@ -128,7 +129,7 @@ def test_batched_capacity_overflow(ctx):
         qjmc_trajectories(spinhalf(), ["up"] * N, H, J, steps * dt, dt, U, engine="batched", **kw)
     a = qjmc_trajectories(spinhalf(), ["up"] * N, H, J, steps * dt, dt, U, engine="auto", **kw)
     b = qjmc_trajectories(spinhalf(), ["up"] * N, H, J, steps * dt, dt, U, engine="workers", **kw)
-    assert a["engine"] == "workers"
+    assert a["engine"] == "batched+workers"
     assert max(a["maxbonddim"]) == 32
     assert a["jumps"] == b["jumps"]
     assert np.abs(a["measurements"] - b["measurements"]).max() <= 1e-12
