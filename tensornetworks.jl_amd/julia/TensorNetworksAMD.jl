@@ -267,7 +267,6 @@ end
 # ------------------------------- many QJMC trajectories (tnh_qjmc_run(_batched))
 # The batched call marks a trajectory that outgrew its slots with maxbond = -1;
 # qjmc_trajectories! reruns those on the per-trajectory runner.
-const TNH_ERR_UNSUPPORTED = Cint(-7)
 
 # Layout of tnh_qjmc_problem (include/tnhip.h); the pointers refer to arrays the caller
 # keeps alive (GC.@preserve) for the duration of the call.
