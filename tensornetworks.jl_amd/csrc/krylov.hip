@@ -65,6 +65,52 @@ __global__ __launch_bounds__(256) void mdot_kernel(const double *__restrict__ V,
   }
 }
 
+// out[2 i + {0,1}] = <v_i, w> of a short vector (n <= kOneBlockDot), one 1024-thread workgroup
+// per vector and no partials pass: each thread loads four elements of v_i and w before summing
+// any, so the workgroup keeps 4096 loads in flight
+constexpr int64_t kOneBlockDot = 16384;
+__global__ __launch_bounds__(1024) void mdot1_kernel(const double *__restrict__ V, int64_t vstride,
+                                                     const double *__restrict__ w, int64_t n,
+                                                     bool cplx, double *__restrict__ out) {
+  const int i = blockIdx.x;
+  const double *v = V + i * vstride;
+  double sr = 0, si = 0;
+  for (int64_t k0 = threadIdx.x; k0 < n; k0 += 4 * 1024) {
+    double vr[4], wr[4], vi[4], wi[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t k = k0 + u * 1024;
+      const bool in = k < n;
+      vr[u] = in ? v[k] : 0.0;
+      wr[u] = in ? w[k] : 0.0;
+      vi[u] = (in && cplx) ? v[n + k] : 0.0;
+      wi[u] = (in && cplx) ? w[n + k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      sr += vr[u] * wr[u] + vi[u] * wi[u];
+      si += vr[u] * wi[u] - vi[u] * wr[u];
+    }
+  }
+  __shared__ double red[2][16];
+  sr = wsum(sr);
+  si = wsum(si);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = sr;
+    red[1][threadIdx.x >> 6] = si;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0;
+    for (int q = 0; q < 16; ++q) {
+      a += red[0][q];
+      b += red[1][q];
+    }
+    out[2 * i] = a;
+    out[2 * i + 1] = b;
+  }
+}
+
 // out[2 i + {0,1}] = sum of the nblk partials of vector i (one wave per vector)
 __global__ void mdot_final_kernel(const double *__restrict__ part, int nblk, int nv,
                                   double *__restrict__ out) {
@@ -84,7 +130,7 @@ __global__ void mdot_final_kernel(const double *__restrict__ part, int nblk, int
 }
 
 // w -= sum_i c_i v_i with the coefficients read from device memory (c[2 i], c[2 i + 1] = re,
-// im, as mdot_final_kernel writes them), so the Gram-Schmidt pass needs no host round trip
+// im, as the dot kernels write them), so the Gram-Schmidt pass needs no host round trip
 __global__ __launch_bounds__(256) void vsub_dev_kernel(double *__restrict__ w,
                                                        const double *__restrict__ V, int64_t vstride,
                                                        int nv, const double *__restrict__ c, int64_t n,
@@ -266,6 +312,16 @@ double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, in
   double *hdev = static_cast<double *>(ctx->workspace(703, sizeof(double) * kStep * kMaxVec));
   double *hres = ctx->host_scalars;
   const int g = grid1(ctx, n);
+  // out[2 i], out[2 i + 1] = <V[i], w> for the nv vectors V[0..nv) (stride vstride)
+  auto dots = [&](const double *Vp, int nv, const double *w, double *out) {
+    if (n <= kOneBlockDot) {
+      hipLaunchKernelGGL(mdot1_kernel, dim3(nv), dim3(1024), 0, ctx->stream, Vp, vstride, w, n, cplx, out);
+    } else {
+      hipLaunchKernelGGL(mdot_kernel, dim3(kDotBlocks, nv), dim3(256), 0, ctx->stream, Vp, vstride, w, n,
+                         cplx, part);
+      hipLaunchKernelGGL(mdot_final_kernel, dim3(nv), dim3(64), 0, ctx->stream, part, kDotBlocks, nv, out);
+    }
+  };
 
   auto view = [&](double *p) {
     tnh_buf b = *theta;
@@ -275,10 +331,7 @@ double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, in
   };
   auto norm_of = [&](const double *w) {
     // ||w||^2 via the same kernel with V replaced by w itself
-    hipLaunchKernelGGL(mdot_kernel, dim3(kDotBlocks, 1), dim3(256), 0, ctx->stream, w, vstride, w,
-                       n, cplx, part);
-    hipLaunchKernelGGL(mdot_final_kernel, dim3(1), dim3(64), 0, ctx->stream, part, kDotBlocks, 1,
-                       dres);
+    dots(w, 1, w, dres);
     TNH_HIP(hipGetLastError());
     to_host(ctx, hres, dres, sizeof(double) * 2);
     return std::sqrt(std::max(0.0, hres[0]));
@@ -311,17 +364,12 @@ double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, in
       double *hj = hdev + int64_t(j) * kStep;
       for (int pass = 0; pass < 2; ++pass) {
         double *cj = hj + pass * 2 * kMaxVec;
-        hipLaunchKernelGGL(mdot_kernel, dim3(kDotBlocks, j + 1), dim3(256), 0, ctx->stream, V, vstride,
-                           Wv, n, cplx, part);
-        hipLaunchKernelGGL(mdot_final_kernel, dim3(j + 1), dim3(64), 0, ctx->stream, part, kDotBlocks,
-                           j + 1, cj);
+        dots(V, j + 1, Wv, cj);
         hipLaunchKernelGGL(vsub_dev_kernel, dim3(g), dim3(256), 0, ctx->stream, Wv, V, vstride, j + 1, cj,
                            n, cplx);
       }
       double *nj = hj + 4 * kMaxVec;
-      hipLaunchKernelGGL(mdot_kernel, dim3(kDotBlocks, 1), dim3(256), 0, ctx->stream, Wv, vstride, Wv, n,
-                         cplx, part);
-      hipLaunchKernelGGL(mdot_final_kernel, dim3(1), dim3(64), 0, ctx->stream, part, kDotBlocks, 1, nj);
+      dots(Wv, 1, Wv, nj);
       hipLaunchKernelGGL(normalize_copy_kernel, dim3(g), dim3(256), 0, ctx->stream, V + (j + 1) * vstride,
                          Wv, nj, vstride);
       TNH_HIP(hipGetLastError());
@@ -363,9 +411,7 @@ double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, in
     }
     hipLaunchKernelGGL(lincomb_kernel, dim3(g), dim3(256), 0, ctx->stream, Wv, V, vstride, m, yc, n,
                        cplx);
-    hipLaunchKernelGGL(mdot_kernel, dim3(kDotBlocks, 1), dim3(256), 0, ctx->stream, Wv, vstride, Wv, n,
-                       cplx, part);
-    hipLaunchKernelGGL(mdot_final_kernel, dim3(1), dim3(64), 0, ctx->stream, part, kDotBlocks, 1, dres);
+    dots(Wv, 1, Wv, dres);
     TNH_HIP(hipGetLastError());
     copy_device(ctx, V + vstride, V + int64_t(m) * vstride, sizeof(double) * vstride);
     hipLaunchKernelGGL(normalize_copy_kernel, dim3(g), dim3(256), 0, ctx->stream, V, Wv, dres, vstride);
