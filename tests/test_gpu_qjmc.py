@@ -135,6 +135,30 @@ def test_batched_capacity_overflow(ctx):
     assert np.abs(a["measurements"] - b["measurements"]).max() <= 1e-12
 
 
+def test_batched_partial_overflow(ctx):
+    """Dissipative TFIM (g = 1, gamma = 0.1, cutoff 1e-12), 8 trajectories of 70 steps: most
+    trajectories reach bond dimension 24-32, above the batched slots (16 for spin-1/2), while
+    a few stay at 16 and complete in the batched engine. engine="auto" reruns only the
+    marked ones on the per-trajectory workers, and every trajectory then matches the
+    workers' own run: jump records exactly, bond dimensions, measurements to 1e-9."""
+    from tnhip import spinhalf
+    from tnhip.qjmc import qjmc_trajectories
+
+    N, g, gamma, dt, steps = 10, 1.0, 0.1, 0.05, 70
+    H, J = model(N, g, gamma)
+    U = np.random.default_rng(31).random((8, 4 * steps + 8))
+    kw = dict(observables=zlist(N), cutoff=1e-12, nworkers=4)
+    a = qjmc_trajectories(spinhalf(), ["up"] * N, H, J, steps * dt, dt, U, engine="auto", **kw)
+    b = qjmc_trajectories(spinhalf(), ["up"] * N, H, J, steps * dt, dt, U, engine="workers", **kw)
+    fit = [t for t in range(len(U)) if b["maxbonddim"][t] <= 16]
+    assert 0 < len(fit) < len(U)  # some trajectories fit the slots, some do not
+    assert a["engine"] == "batched+workers"
+    assert a["maxbonddim"] == b["maxbonddim"]
+    assert a["jumps"] == b["jumps"]
+    assert a["jumptimes"] == b["jumptimes"]
+    assert np.abs(a["measurements"] - b["measurements"]).max() <= 1e-9
+
+
 def test_batched_matches_workers_spin_one(ctx):
     """d = 3 (spin-1: slot capacity 10, Gram matrices up to 30 x 30): spin-1 XXZ-like chain
     with a transverse field and S- decay, truncated to maxdim 8. The batched engine against
