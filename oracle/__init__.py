@@ -14,7 +14,11 @@ contraction order with the optimal order, Hermiticity, LAPACK singular values,
 dense exact diagonalisation and closed forms (critical transverse-field Ising:
 E0(N) = 1 - csc(pi/(4N+2))). See tests/test_oracle.py.
 
-Functions with no such pin say "parity unpinned" in their docstring:
-  * krylov.eigsolve_ground's per-iteration Ritz sequence (KrylovKit's exact restart
-    sequence is version-dependent); only its converged eigenpair is pinned by ED.
+Every function is pinned (tests/test_oracle.py, tests/test_oracle_pins.py,
+tests/test_oracle_qjmc.py, tests/test_models.py). What stays "parity unpinned" is a
+property of third-party code that cannot run here, not of a function of this package:
+  * that KrylovKit keeps exactly one Ritz vector per restart at krylovdim <= 3 (taken from
+    its published algorithm; krylov.eigsolve_ground's own per-cycle Ritz values are pinned
+    as Rayleigh-Ritz values on the Krylov spaces K_k(H, x0), K_k(H, y1), ...);
+  * Julia's global RNG stream (randomMPS, QJMC's rand()): inputs are drawn here instead.
 """

@@ -10,8 +10,12 @@ re-orthogonalisation, restarted Krylov-Schur style keeping the wanted Ritz vecto
 wanted pair). This restatement keeps exactly one Ritz vector per restart, so one
 cycle costs krylovdim matvecs first and krylovdim - 1 after each restart.
 
-Parity unpinned for the per-iteration Ritz sequence (version-dependent); the
-converged eigenpair is pinned by dense diagonalisation (tests/test_oracle.py).
+Pins (tests/test_oracle_pins.py, tests/test_oracle.py): cycle c returns the lowest
+Rayleigh-Ritz pair of H on K_k(H, y_{c-1}) (y_0 = x0, y_c the Ritz vector of cycle c),
+computed independently from a QR basis of the monomial Krylov vectors, with
+k + (c - 1)(k - 1) matvecs; the converged pair is pinned by dense diagonalisation.
+Parity unpinned only for KrylovKit itself: that its version keeps exactly one vector per
+restart for these parameters cannot be checked without running it.
 """
 from __future__ import annotations
 

@@ -1,0 +1,180 @@
+"""Pins of oracle functions that the other CPU tests leave open: the tensor exponential
+and trace of src/tensors.jl, the second-order Trotter split of gatelist.jl:75-121 (by the
+dt^3 scaling of its one-step error against the exact propagator), and the per-cycle Ritz
+values of the restarted Lanczos that stands in for KrylovKit's eigsolve (each cycle is the
+Rayleigh-Ritz problem of a Krylov space the mathematics fixes)."""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from oracle import exact
+from oracle.krylov import eigsolve_ground
+from oracle.tebd import apply_dense, gate_from_tensor, trotter_layers
+from oracle.tensors import expm_tensor, trace
+
+
+def randn(rng, shape, cplx=False):
+    a = rng.standard_normal(shape)
+    return a + 1j * rng.standard_normal(shape) if cplx else a
+
+
+def herm(rng, n, cplx=True):
+    X = randn(rng, (n, n), cplx)
+    return 0.5 * (X + X.conj().T)
+
+
+# ----------------------------------------------------------------- tensors.jl
+
+
+def test_expm_tensor_of_a_product_gate_is_the_kron_exponential(rng):
+    """G[o1,i1,o2,i2] = a[o1,i1] b[o2,i2]: with outer indices [2, 4] the matrix is
+    G[(o1,o2),(i1,i2)] (column-major, o1 fastest) = kron(b, a), whose exponential is taken
+    here through its eigendecomposition (an independent route from scipy's Pade)."""
+    a, b = herm(rng, 2), herm(rng, 3)
+    G = np.einsum("ij,kl->ijkl", a, b)
+    E = expm_tensor(G, [2, 4])
+    w, V = np.linalg.eigh(np.kron(b, a))
+    M = (V * np.exp(w)) @ V.conj().T
+    Mt = np.transpose(E, (0, 2, 1, 3)).reshape(6, 6, order="F")
+    assert np.allclose(Mt, M, atol=1e-13)
+
+
+def test_expm_tensor_inverse_and_other_implementation(rng):
+    """exp(G) exp(-G) = 1 contracted over the inner/outer pairs, and agreement with the
+    kron-order gate builder of the TEBD reference (a transposed index fails one of them)."""
+    d = 2
+    G = randn(rng, (d, d, d, d), True)
+    Ep, Em = expm_tensor(G, [2, 4]), expm_tensor(-G, [2, 4])
+    prod = np.einsum("aibj,icjd->abcd", Ep, Em)  # [o1,o2,i1,i2]
+    assert np.allclose(prod, np.einsum("ac,bd->abcd", np.eye(d), np.eye(d)), atol=1e-12)
+    U = gate_from_tensor(G, 1.0)  # rows (o1, o2) with o1 slowest
+    assert np.allclose(np.transpose(Ep, (0, 2, 1, 3)).reshape(d * d, d * d), U, atol=1e-12)
+
+
+def test_expm_tensor_diagonal_is_elementwise():
+    diag = np.array([0.3, -1.2, 2.0, 0.0])
+    M = np.diag(diag)  # [(o1,o2),(i1,i2)] column-major
+    G = np.transpose(M.reshape(2, 2, 2, 2, order="F"), (0, 2, 1, 3))
+    E = expm_tensor(G, [2, 4])
+    Em = np.transpose(E, (0, 2, 1, 3)).reshape(4, 4, order="F")
+    assert np.allclose(Em, np.diag(np.exp(diag)), atol=1e-14)
+
+
+def test_host_gate_exponential_matches_oracle(rng):
+    """The host-side exp of tnhip.tebd (used to build Trotter gates) against the oracle."""
+    from tnhip.tebd import expm_tensor as host_expm
+
+    G = randn(rng, (2, 2, 2, 2), True)
+    assert np.allclose(host_expm(0.1 * G, [2, 4]), expm_tensor(0.1 * G, [2, 4]), atol=1e-14)
+    g1 = randn(rng, (3, 3), True)
+    assert np.allclose(host_expm(g1, [2]), expm_tensor(g1, [2]), atol=1e-13)
+
+
+def test_trace_pairs_the_given_indices(rng):
+    x = randn(rng, (3, 4, 3, 2), True)
+    t = trace(x, 1, 3)
+    ref = sum(x[i, :, i, :] for i in range(3))
+    assert t.shape == (4, 2) and np.allclose(t, ref)
+    y = randn(rng, (2, 5, 5))
+    assert np.allclose(trace(y, 2, 3), [sum(y[a, i, i] for i in range(5)) for a in range(2)])
+
+
+# ----------------------------------------------------------- Trotter splitting
+
+
+def _layers_operator(layers, N, d=2):
+    dim = d ** N
+    cols = []
+    for c in range(dim):
+        v = np.zeros(dim, dtype=complex)
+        v[c] = 1.0
+        for layer in layers:
+            for site, U in layer:
+                v = apply_dense(v, site, U, N)
+        cols.append(v)
+    return np.stack(cols, axis=1)
+
+
+def _chain_terms(N):
+    return ([(["z", "z"], [i, i + 1], -1.0) for i in range(1, N)]
+            + [(["x"], [i], -0.8) for i in range(1, N + 1)]
+            + [(["z"], [i], -0.3 * (-1) ** i) for i in range(1, N + 1)])
+
+
+@pytest.mark.parametrize("evol", ["imag", "real"])
+def test_second_order_trotter_step_error_scales_as_dt_cubed(evol):
+    """One step of the layers (odd dt/2, even dt, odd dt/2) against exp(tau H) with
+    tau = dt ("imag") or -i dt ("real"): the local error of a symmetric split is O(dt^3), so
+    halving dt divides it by 8; a first-order split (or a mis-weighted symmetric one)
+    divides it by 4 at most. Odd N, so every second layer ends in a one-site gate."""
+    N = 5
+    terms = _chain_terms(N)
+    H = exact.dense_hamiltonian(N, terms)
+
+    def err(dt, order):
+        layers = trotter_layers(terms, N, dt, order=order, evol=evol)
+        tau = dt if evol == "imag" else -1j * dt
+        return np.linalg.norm(_layers_operator(layers, N) - sla.expm(tau * H), 2)
+
+    r2 = err(0.02, 2) / err(0.01, 2)
+    r1 = err(0.02, 1) / err(0.01, 1)
+    assert 7.6 < r2 < 8.4, r2
+    assert 3.6 < r1 < 4.4, r1
+    # and the symmetric split is the far more accurate one at the same dt
+    assert err(0.01, 2) < 0.2 * err(0.01, 1)
+
+
+def test_trotter_layer_structure():
+    """Second order: range-2 terms give layers (odd, even, odd) with times dt/2, dt, dt/2,
+    the last layer being the same gates as the first (gatelist.jl:112-117)."""
+    N = 6
+    terms = _chain_terms(N)
+    dt = 0.1
+    layers = trotter_layers(terms, N, dt, order=2, evol="imag")
+    assert len(layers) == 3
+    # the even layer ends in a one-site block at site 6 (the range-2 block is cut there)
+    assert [s for s, _ in layers[0]] == [1, 3, 5] and [s for s, _ in layers[1]] == [2, 4, 6]
+    assert layers[1][-1][1].shape == (2, 2)
+    assert layers[2] is layers[0]
+    full = trotter_layers(terms, N, 2 * dt, order=1, evol="imag")
+    # the odd layer at dt/2 squared is the odd layer of a first-order step at dt
+    for (s1, U1), (s2, U2) in zip(layers[0], trotter_layers(terms, N, dt, order=1, evol="imag")[0]):
+        assert s1 == s2 and np.allclose(U1 @ U1, U2, atol=1e-13)
+    assert len(full) == 2
+
+
+# ------------------------------------------------------------------- Krylov
+
+
+def _rayleigh_ritz_lowest(A, x, k):
+    """Lowest Ritz pair of A on K_k(A, x), from an orthonormal basis of the monomial
+    Krylov vectors (QR), independent of the Lanczos recurrences."""
+    K = [x / np.linalg.norm(x)]
+    for _ in range(k - 1):
+        w = A @ K[-1]
+        K.append(w / np.linalg.norm(w))
+    Q, _ = np.linalg.qr(np.stack(K, axis=1))
+    w, V = np.linalg.eigh(Q.conj().T @ A @ Q)
+    return w[0], Q @ V[:, 0]
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_lanczos_cycles_are_rayleigh_ritz_on_fixed_krylov_spaces(rng, cplx):
+    """Cycle 1 is Rayleigh-Ritz on K_k(A, x0). A restart keeps the Ritz vector y and the
+    residual direction; since A y = theta y + b v_k, the next cycle's space is K_k(A, y).
+    So the Ritz value after c cycles is Rayleigh-Ritz on K_k(A, y_{c-1}), and the matvec
+    count is k + (c - 1)(k - 1)."""
+    n, k = 30, 4
+    A = herm(rng, n, cplx)
+    x0 = randn(rng, n, cplx)
+    y = x0
+    for cycles in (1, 2, 3):
+        e, v, nmv = eigsolve_ground(lambda x: A @ x, x0, krylovdim=k, maxiter=cycles, tol=0.0)
+        e_rr, v_rr = _rayleigh_ritz_lowest(A, y, k)
+        assert nmv == k + (cycles - 1) * (k - 1)
+        assert abs(e - e_rr) <= 1e-11 * np.abs(np.linalg.eigvalsh(A)).max(), (cycles, e, e_rr)
+        assert abs(abs(np.vdot(v, v_rr)) - 1) <= 1e-9
+        y = v
+    # the sequence decreases monotonically towards the ground state
+    es = [eigsolve_ground(lambda x: A @ x, x0, k, c, 0.0)[0] for c in (1, 2, 3, 6)]
+    assert all(a >= b - 1e-12 for a, b in zip(es, es[1:]))
