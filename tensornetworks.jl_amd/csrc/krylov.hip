@@ -378,6 +378,13 @@ double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, in
     to_host(ctx, hh.data(), hdev + int64_t(start) * kStep, sizeof(double) * hh.size());
     for (int j = start; j < kdim; ++j) {
       const double *hj = hh.data() + size_t(j - start) * kStep;
+      // A non-finite coefficient (NaN or Inf in theta or the environments) is an error: it
+      // must not reach the host eigensolver, whose ordering of the eigenvalues needs a
+      // consistent comparison.
+      bool finite = std::isfinite(hj[4 * kMaxVec]);
+      for (int pass = 0; pass < 2; ++pass)
+        for (int i = 0; i < 2 * (j + 1); ++i) finite = finite && std::isfinite(hj[pass * 2 * kMaxVec + i]);
+      TNH_REQUIRE(finite, TNH_ERR_NUMERIC, "eigsolve: non-finite Krylov coefficient");
       ++nmv;
       for (int pass = 0; pass < 2; ++pass)
         for (int i = 0; i <= j; ++i) {

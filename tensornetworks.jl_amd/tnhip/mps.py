@@ -174,7 +174,9 @@ def dmrg(psi: GMPS, *Hs: GMPS, nsites=2, krylovdim=3, kryloviter=2, ishermitian=
                      the precision's noise floor, measured on the critical TFIM at N = 100:
                      1e-5 for "f32", whose passes stall at ~1e-6, and 1e-4 for "bf16x3") or
                      when only refine_sweeps passes remain before maxsweeps; at least
-                     refine_sweeps FP64 passes are run before stopping.
+                     refine_sweeps FP64 passes are run before stopping, except that
+                     maxsweeps always ends the run (with maxsweeps <= refine_sweeps every
+                     pass is FP64).
     """
     if nsites != 2:
         raise NotImplementedError("only the two-site update (nsites=2) is on the GPU path")
@@ -209,8 +211,8 @@ def dmrg(psi: GMPS, *Hs: GMPS, nsites=2, krylovdim=3, kryloviter=2, ishermitian=
         return abs(x - y) if abs(x) < 1e-10 else abs((x - y) / x)
 
     while not converged:
-        if prec != "f64" and (sweeps >= maxsweeps - refine_sweeps > 0 if maxsweeps else False):
-            prec = "f64"
+        if prec != "f64" and maxsweeps and sweeps >= maxsweeps - refine_sweeps:
+            prec = "f64"  # the last refine_sweeps passes before maxsweeps (all, if fewer)
         t_sweep = time.perf_counter()
         pass_prec = prec
         cost, D, disc = st.sweep(direction, krylovdim, kryloviter, 1e-14, cutoff, maxdim, mindim, prec)
@@ -232,8 +234,10 @@ def dmrg(psi: GMPS, *Hs: GMPS, nsites=2, krylovdim=3, kryloviter=2, ishermitian=
                 converged = True
             if sweeps >= maxsweeps and maxsweeps != 0:
                 converged = True
-            if converged and precision != "f64" and fp64_passes < refine_sweeps:
-                converged = False                         # finish with FP64 passes
+            # finish with FP64 passes, but never beyond maxsweeps (dmrg.jl:80-82 stops there)
+            if (converged and precision != "f64" and fp64_passes < refine_sweeps
+                    and not (maxsweeps and sweeps >= maxsweeps)):
+                converged = False
                 prec = "f64"
         grad = abs(diff(cost, lastcost))
         lastcost = cost
