@@ -170,3 +170,29 @@ def inner_oplist(psi, oplist_terms, phi=None):
             left = np.einsum("ab,asc,bsd->cd", left, A, B)
         out.append(coeff * left[0, 0])
     return np.array(out)
+
+
+def inner_oplist_env(psi, oplist_terms, phi=None):
+    """inner(st, psi, oplist, phi), src/structures/mps/mps.jl:87-134, with the reference's
+    environment blocks (ProjMPS(psi, phi; rank=1), :90): left blocks of sites 1..i and right
+    blocks of sites i..N are built once, and each term contracts only its own sites between
+    them. Returns coeff * <psi| O_term |phi> per term (terms as in inner_oplist)."""
+    phi = psi if phi is None else phi
+    N = len(psi)
+    left = [np.ones((1, 1))]
+    for s in range(1, N + 1):
+        left.append(np.einsum("ab,asc,bsd->cd", left[-1], np.conj(psi[s]), phi[s]))
+    right = [None] * (N + 2)
+    right[N + 1] = np.ones((1, 1))
+    for s in range(N, 0, -1):
+        right[s] = np.einsum("asc,bsd,cd->ab", np.conj(psi[s]), phi[s], right[s + 1])
+    out = []
+    for mats, sites, coeff in oplist_terms:
+        env = left[sites[0] - 1]
+        for s in range(sites[0], sites[-1] + 1):
+            B = phi[s]
+            if s in sites:
+                B = np.einsum("ts,asb->atb", mats[sites.index(s)], B)
+            env = np.einsum("ab,asc,bsd->cd", env, np.conj(psi[s]), B)
+        out.append(coeff * np.einsum("ab,ab->", env, right[sites[-1] + 1]))
+    return np.array(out)

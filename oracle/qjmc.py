@@ -158,3 +158,69 @@ def lindblad_dense(rho0, H_terms, jump_terms, N, times, st_ops=OPS):
     for t in times:
         out.append((sla.expm(t * Lv) @ r0).reshape(dim, dim, order="F"))
     return out
+
+
+def qjmc_mps(states, H_terms, jump_terms, N, dt, tmax, U, save=0.0, obs=(), cutoff=1e-12, maxdim=0,
+             mindim=1, order=2, st_ops=OPS):
+    """One trajectory of qjmc_simulation, src/algorithms/mps/qjmc.jl:28-167 (classical
+    branch), on an MPS: from the product state of the local vectors `states` (productMPS,
+    mps.jl:68-76), per step applygates! with truncation (:60), the discarded draw (:64),
+    normalize! (:91), the emission rates c^2 <L^H L> (qjmc_emission_rates, :170-220), the jump
+    test rand() > exp(-sum(rates) dt) (:97), the channel (:99), and on a jump applyop! at the
+    centre 1 followed by movecenter!(psi, N) and the truncated movecenter!(psi, 1) and
+    normalize! (:101-107). U is this trajectory's row of uniforms. obs: one-site operators
+    (name or matrix, site) measured as <psi|O|psi> at time 0 and at every save point
+    (QJMCOperators, :252-260). Returns (jumps, jumptimes, times, measurements[nsave, nobs],
+    the final maximum bond dimension)."""
+    from .mps import inner_oplist_env, movecenter, normalize, product_mps
+    from .tebd import applygates_mps, rows_from_layers
+
+    U = np.asarray(U, dtype=np.float64).reshape(-1)
+    rows = rows_from_layers(qjmc_layers(H_terms, jump_terms, N, dt, order=order, st_ops=st_ops))
+    esc = escape_terms(jump_terms, st_ops)
+    save = dt if save == 0 else save
+    steps = int(math.ceil(round(tmax / dt, 5)))
+    savesteps = int(math.ceil(round(save / dt)))
+    psi = product_mps(states)
+    pos = [0]
+
+    def draw():
+        if pos[0] >= U.size:
+            raise ValueError("not enough uniforms supplied")
+        pos[0] += 1
+        return float(U[pos[0] - 1])
+
+    obs_terms = [([_mat(o, st_ops)], [s], 1.0) for o, s in obs]
+
+    def measure():
+        return inner_oplist_env(psi, obs_terms) if obs_terms else np.zeros(0, complex)
+
+    jumps, jtimes, times, meas = [], [], [0.0], [measure()]
+    time = 0.0
+    for i in range(1, steps + 1):
+        applygates_mps(psi, rows, cutoff=cutoff, maxdim=maxdim, mindim=mindim)
+        draw()  # qjmc.jl:64, unused on the classical branch
+        normalize(psi)
+        rates = np.abs(inner_oplist_env(psi, esc))
+        er = rates.sum()
+        prob = math.exp(-er * dt)
+        if draw() > prob:
+            r = draw()
+            hit = np.nonzero(r < np.cumsum(rates) / er)[0]
+            if hit.size == 0:
+                raise ValueError("no jump channel selected (findfirst returned nothing)")
+            idx = int(hit[0])
+            ops, sites, _ = jump_terms[idx]
+            movecenter(psi, 1)
+            for o, s in zip(ops, sites):  # applyop!, mps.jl:141-151
+                psi[s] = np.einsum("ts,asb->atb", _mat(o, st_ops), psi[s])
+            movecenter(psi, N)
+            movecenter(psi, 1, cutoff=cutoff, maxdim=maxdim, mindim=mindim)
+            normalize(psi)
+            jumps.append(idx + 1)
+            jtimes.append(time + dt)
+        time += dt
+        if i % savesteps == 0:
+            times.append(time)
+            meas.append(measure())
+    return jumps, jtimes, np.array(times), np.stack(meas), psi.maxbonddim()

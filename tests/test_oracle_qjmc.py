@@ -2,6 +2,7 @@
 evolution, and the uniform-consumption order of qjmc.jl:59-112 on a case whose jump
 time is fixed by the uniforms."""
 import numpy as np
+import pytest
 
 from oracle.qjmc import emission_rates_dense, escape_terms, lindblad_dense, qjmc_dense
 from oracle.exact import OPS, dense_hamiltonian
@@ -67,3 +68,46 @@ def test_rates_two_site_jump():
     r = emission_rates_dense(v[:, None], [(["s-", "s-"], [1, 2], 0.5)], N)
     P = dense_hamiltonian(N, [(["pu", "pu"], [1, 2], 1.0)])
     assert abs(r[0, 0] - 0.25 * np.vdot(v, P @ v).real) < 1e-14
+
+
+def test_inner_oplist_env_matches_brute_force():
+    """The environment form of inner(st, psi, oplist, psi) against the brute-force sweep
+    and the dense state, with one-site, neighbouring and gapped two-site terms."""
+    from oracle.exact import OPS, dense_hamiltonian
+    from oracle.mps import inner_oplist, inner_oplist_env, random_mps, to_dense
+
+    rng = np.random.default_rng(5)
+    psi = random_mps(6, 2, 4, rng)
+    psi[3] = psi[3] * (1 + 0.5j)  # complex, unnormalised
+    terms = [([OPS["z"]], [2], 1.0), ([OPS["x"], OPS["y"]], [3, 4], 0.5), ([OPS["s+"], OPS["s-"]], [1, 5], -2.0)]
+    a, b = inner_oplist_env(psi, terms), inner_oplist(psi, terms)
+    assert np.allclose(a, b, atol=1e-13)
+    v = to_dense(psi)
+    for (mats, sites, c), x in zip(terms, a):
+        O = dense_hamiltonian(6, [(mats, sites, c)])
+        assert abs(x - np.vdot(v, O @ v)) < 1e-12
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_mps_trajectory_matches_dense(seed):
+    """The MPS restatement of qjmc_simulation (oracle.qjmc.qjmc_mps) against the dense
+    trajectory on the same uniforms, without truncation: identical jump records and times,
+    <sz_i> at every save point to 1e-10. N = 6, dissipative TFIM with strong decay so that
+    the case contains jumps."""
+    from oracle.qjmc import qjmc_dense, qjmc_mps
+
+    N, g, gamma, dt, tmax = 6, 0.5, 0.5, 0.05, 1.0
+    H = [(["z", "z"], [i, i + 1], -1.0) for i in range(1, N)] + [(["x"], [i], -g) for i in range(1, N + 1)]
+    J = [(["s-"], [i], gamma ** 0.5) for i in range(1, N + 1)]
+    U = np.random.default_rng(seed).random((2, 200))
+    v0 = np.zeros(2 ** N, complex)
+    v0[0] = 1.0
+    obs = [("z", i) for i in range(1, N + 1)]
+    jr, tr, times, meas = qjmc_dense(v0, H, J, N, dt, tmax, U, save=0.25, obs=obs)
+    assert sum(len(j) for j in jr) >= 1
+    up = np.array([1.0, 0.0])
+    for m in range(U.shape[0]):
+        j, t, tm, ms, chi = qjmc_mps([up] * N, H, J, N, dt, tmax, U[m], save=0.25, obs=obs, cutoff=0.0)
+        assert j == jr[m]
+        assert np.allclose(t, tr[m], atol=1e-12) and np.allclose(tm, times)
+        assert np.abs(ms - meas[m]).max() <= 1e-10
