@@ -162,7 +162,7 @@ class DMRGState:
 def dmrg(psi: GMPS, *Hs: GMPS, nsites=2, krylovdim=3, kryloviter=2, ishermitian=True, minsweeps=1,
          maxsweeps=1000, tol=1e-10, tolgrad=1e-5, numconverges=4, verbose=True, cutoff=1e-12,
          maxdim=1000, mindim=1, coeffs=None, precision="f64", refine_sweeps=2, refine_tol=None,
-         history=None, state_out=None):
+         history=None, state_out=None, checkpoint=None):
     """dmrg(psi, H::GMPS...; kwargs...), src/algorithms/mps/dmrg.jl:128-154 and :1-99.
 
     psi is updated in place and (psi, cost) returned, as in the reference. The whole bond
@@ -177,6 +177,9 @@ def dmrg(psi: GMPS, *Hs: GMPS, nsites=2, krylovdim=3, kryloviter=2, ishermitian=
                      refine_sweeps FP64 passes are run before stopping, except that
                      maxsweeps always ends the run (with maxsweeps <= refine_sweeps every
                      pass is FP64).
+      checkpoint     a path: after every pass psi is written there in the reference's
+                     HDF5 layout (HDF5.write of GMPS, gmps.jl:359-370, group "psi"); a run
+                     resumes by reading it back (tnhip.checkpoint.read_mps) and calling dmrg.
     """
     if nsites != 2:
         raise NotImplementedError("only the two-site update (nsites=2) is on the GPU path")
@@ -245,6 +248,10 @@ def dmrg(psi: GMPS, *Hs: GMPS, nsites=2, krylovdim=3, kryloviter=2, ishermitian=
         if history is not None:
             history.append({"sweep": sweeps, "energy": cost, "maxbonddim": D, "discarded": disc,
                             "precision": pass_prec, "seconds": t_sweep})
+        if checkpoint is not None:
+            from .checkpoint import write_mps
+
+            write_mps(checkpoint, GMPS(1, psi.dim, st.tensors(), st.center()))
         if verbose:
             print(f"Sweep={sweeps}, energy={cost:.12f}, maxbonddim={D} ")
     psi.tensors = st.tensors()
