@@ -153,6 +153,17 @@ int tnh_heff_apply(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *W1, const tnh_
                    const tnh_buf *R, const tnh_buf *x, tnh_buf *out, double coeff_re,
                    double coeff_im, int precision);
 
+/* The same matvec as tnh_heff_apply (same kernels, bitwise the same result) without any
+ * device allocation: no counterpart in the reference; for callers that cannot allocate
+ * (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc). Every
+ * temporary comes from a static 64 MiB device array per GPU, held until the process ends;
+ * shapes whose scratch bound exceeds it fail with TNH_ERR_NOMEM before any work is queued
+ * (FP64 real operands: chi up to ~450 at D = 5, d = 2). All calls on one GPU share that
+ * array, so calls that overlap on one GPU (other contexts, other threads) race. */
+int tnh_heff_apply_noalloc(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *W1, const tnh_buf *W2,
+                           const tnh_buf *R, const tnh_buf *x, tnh_buf *out, double coeff_re,
+                           double coeff_im, int precision);
+
 /* Left environment update L'[b,w',b'] = sum L[a,w,a'] conj(A[a,s,b]) W[w,s,t,w'] A[a',t,b'].
  * Replaces buildleft!(::ProjMPS, idx), src/structures/mps/projmps.jl:50-66.
  * *Lout is (re)allocated when NULL or of the wrong shape. Pass Lprev = NULL for the

@@ -347,6 +347,20 @@ namespace {
 
 // The 1x1x1 block of ones used outside the chain (abstractprojmps.jl:33-46).
 const tnh_buf *edge_block(tnh_ctx *ctx) {
+  if (ctx->arena) {  // inside a call that allocates nothing: a block carved from its scratch
+    tnh_buf &e = ctx->arena->edge;
+    if (!e.ptr) {
+      e.ptr = ctx->workspace(800, 8);
+      fill(ctx, e.ptr, TNH_F64, 1, 1.0);
+      e.dtype = TNH_F64;
+      e.ndim = 3;
+      e.dims[0] = e.dims[1] = e.dims[2] = 1;
+      e.numel = 1;
+      e.bytes = 8;
+      e.owned = false;
+    }
+    return &e;
+  }
   if (!ctx->edge.ptr) {
     void *p = ctx->workspace(800, 8);
     double one = 1.0;
@@ -582,6 +596,36 @@ void heff_apply(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *W1, const tnh_buf
     TNH_REQUIRE(is_complex(out->dtype), TNH_ERR_ARG, "complex coefficient needs a complex output");
     cscale(ctx, out->ptr, out->dtype, out->numel, cr, ci);
   }
+}
+
+// Upper bound of the device workspace heff_apply takes for these operands: every slot it
+// requests counted separately (no reuse assumed), each rounded up to 256 bytes. Callers
+// that bring their own scratch size it with this before the call.
+size_t heff_scratch_bound(const tnh_buf *L, const tnh_buf *W1, const tnh_buf *W2, const tnh_buf *R,
+                          const tnh_buf *x, int prec) {
+  require_dims(W1, 4, "W1");
+  require_dims(W2, 4, "W2");
+  require_dims(x, 4, "theta");
+  auto up = [](size_t b) { return (std::max<size_t>(b, 1) + 255) & ~size_t(255); };
+  const int64_t cl = x->dims[0], d = x->dims[1], cr_ = x->dims[3];
+  const int64_t D1 = W1->dims[0], D3 = W2->dims[3];
+  const bool wc_cplx = is_complex(W1->dtype) || is_complex(W2->dtype);
+  const bool t1c = (L && is_complex(L->dtype)) || is_complex(x->dtype);
+  const bool t2c = t1c || wc_cplx;
+  const size_t rs = prec == TNH_PREC_F64 ? 8 : 4;
+  const size_t I = size_t(D1 * d * d), J = size_t(d * d * D3);
+  const size_t n1 = size_t(cl * D1 * d * d * cr_), n2 = size_t(cl * d * d * D3 * cr_);
+  size_t b = up(sizeof(double) * I * J * 2) + up(n1 * rs * (t1c ? 2 : 1)) + up(n2 * rs * (t2c ? 2 : 1));
+  if (!L || !R) b += up(8);
+  if (prec == TNH_PREC_F32 && is_double(x->dtype) && x->numel >= (int64_t(1) << 20))
+    b += up(size_t(x->numel) * 4 * (is_complex(x->dtype) ? 2 : 1));
+  if (prec == TNH_PREC_BF16X3) {
+    // hi/lo bf16 planes of both operands of both GEMM stages, k padded to 32
+    auto planes = [&](int64_t rows, int64_t k) { return up(size_t(2) * rows * ((k + 31) / 32 * 32) * 2); };
+    b += planes(cl * D1, cl) + planes(d * d * cr_, cl);           // stage 1: L, theta
+    b += planes(cl * d * d, D3 * cr_) + planes(cr_, D3 * cr_);    // stage 3: T2, R
+  }
+  return b;
 }
 
 // ------------------------------------------------------------ environments

@@ -9,6 +9,7 @@
 // ------------------------------------------------------------------ allocator
 
 void *tnh_ctx::alloc(size_t bytes, size_t *cap_out) {
+  if (arena) throw tnh::Error(TNH_ERR_NOMEM, "device allocation requested inside an entry point that allocates nothing");
   if (bytes == 0) bytes = 256;
   bytes = (bytes + 255) & ~size_t(255);
   // reuse a cached block of size in [bytes, 2*bytes]
@@ -58,6 +59,18 @@ void tnh_ctx::trim() {
 }
 
 void *tnh_ctx::workspace(int slot, size_t bytes) {
+  if (arena) {
+    auto it = arena->slots.find(slot);
+    if (it != arena->slots.end() && it->second.second >= bytes) return it->second.first;
+    const size_t need = (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
+    if (arena->off + need > arena->cap)
+      throw tnh::Error(TNH_ERR_NOMEM, "static scratch exhausted (" + std::to_string(arena->off + need) +
+                                          " bytes needed, " + std::to_string(arena->cap) + " available)");
+    void *p = arena->base + arena->off;
+    arena->off += need;
+    arena->slots[slot] = {p, need};
+    return p;
+  }
   auto &w = ws[slot];
   if (w.second < bytes) {
     if (w.first) release(w.first, w.second);
@@ -262,6 +275,8 @@ int tnh_ctx_create(int device, tnh_ctx **out) {
     TNH_HIP(hipGetDeviceProperties(&prop, device));
     ctx->cu_count = prop.multiProcessorCount;
     TNH_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    // this device's copy of the static scratch of tnh_heff_apply_noalloc (once per device)
+    noalloc_init(device);
     TNH_HIP(hipHostMalloc((void **)&ctx->host_scalars, sizeof(double) * tnh_ctx::kHostScalars,
                           hipHostMallocDefault));
   });

@@ -125,6 +125,7 @@ void gemm(tnh_ctx *ctx, const Gemm &g, int prec) {
                                          q.vecC, size_class(q.M), size_class(q.N), size_class(q.K)};
     auto it = ctx->gemm_tiles.find(key);
     if (it != ctx->gemm_tiles.end()) return it->second.tile;
+    if (ctx->arena) return rule;  // timing needs a scratch output: not inside a no-allocation call
     hipStreamCaptureStatus cap_status = hipStreamCaptureStatusNone;
     TNH_HIP(hipStreamIsCapturing(ctx->stream, &cap_status));
     if (cap_status != hipStreamCaptureStatusNone) return rule;  // no synchronisation while captured

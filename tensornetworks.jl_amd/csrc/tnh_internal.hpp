@@ -90,6 +90,17 @@ struct tnh_ctx {
   std::map<int, std::pair<void *, size_t>> ws;
   void *workspace(int slot, size_t bytes);
 
+  // Caller-provided device scratch, set only for the duration of an entry point that must
+  // not allocate: workspace() then carves its slots from it (a slot is reused while it is
+  // large enough) and alloc() refuses, so no hipMalloc runs inside such a call.
+  struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, off = 0;
+    std::map<int, std::pair<void *, size_t>> slots;
+    tnh_buf edge;  // the ones(1,1,1) block, carved from the arena when needed
+  };
+  Arena *arena = nullptr;
+
   // The 1x1x1 ones block standing for environments beyond the chain ends.
   tnh_buf edge;
 
@@ -218,6 +229,11 @@ void dot_f64(tnh_ctx *ctx, int64_t n, const double *x, const double *y, double *
 // -------------------------------------------------------- hot-path operators
 void heff_apply(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *W1, const tnh_buf *W2,
                 const tnh_buf *R, const tnh_buf *x, tnh_buf *out, double cr, double ci, int prec);
+size_t heff_scratch_bound(const tnh_buf *L, const tnh_buf *W1, const tnh_buf *W2, const tnh_buf *R,
+                          const tnh_buf *x, int prec);
+// Resolve this device's copy of the static scratch of tnh_heff_apply_noalloc (the device
+// must be current); called once per device when a context is created.
+void noalloc_init(int device);
 void env_left(tnh_ctx *ctx, const tnh_buf *Lp, const tnh_buf *A, const tnh_buf *W,
               tnh_buf **Lout, int prec);
 void env_right(tnh_ctx *ctx, const tnh_buf *Rp, const tnh_buf *A, const tnh_buf *W,
@@ -237,6 +253,7 @@ void replacesites2(tnh_ctx *ctx, const tnh_buf *theta, int direction, int normal
                    double *disc_out, double *schmidt, int schmidt_cap, bool force_fallback);
 void mps_movecenter(tnh_ctx *ctx, int N, tnh_buf **s, int center, int idx, double cutoff,
                     int maxdim, int mindim);
+
 void mps_moveleft(tnh_ctx *ctx, tnh_buf **A, tnh_buf **B, double cutoff, int maxdim, int mindim);
 void mps_moveright(tnh_ctx *ctx, tnh_buf **A, tnh_buf **B, double cutoff, int maxdim, int mindim);
 

@@ -103,6 +103,7 @@ SIGNATURES = {
     "tnh_buf_scale": (C.c_int, [ctx_p, buf_p, C.c_double, C.c_double]),
     "tnh_apply_site_op": (C.c_int, [ctx_p, C.POINTER(buf_p), buf_p]),
     "tnh_heff_apply": (C.c_int, [ctx_p, buf_p, buf_p, buf_p, buf_p, buf_p, buf_p, C.c_double, C.c_double, C.c_int]),
+    "tnh_heff_apply_noalloc": (C.c_int, [ctx_p, buf_p, buf_p, buf_p, buf_p, buf_p, buf_p, C.c_double, C.c_double, C.c_int]),
     "tnh_env_left": (C.c_int, [ctx_p, buf_p, buf_p, buf_p, C.POINTER(buf_p), C.c_int]),
     "tnh_env_right": (C.c_int, [ctx_p, buf_p, buf_p, buf_p, C.POINTER(buf_p), C.c_int]),
     "tnh_env_calculate": (C.c_int, [ctx_p, buf_p, buf_p, buf_p, buf_p, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),

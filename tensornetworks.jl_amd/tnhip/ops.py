@@ -41,6 +41,17 @@ def heff_apply(L, W1, W2, R, x, out=None, coeff=1.0, precision="f64") -> Tensor:
     return out
 
 
+def heff_apply_noalloc(L, W1, W2, R, x, out, coeff=1.0, precision="f64") -> Tensor:
+    """heff_apply into a caller-provided `out` without any device allocation: the library
+    takes its temporaries from a static per-GPU scratch (tnh_heff_apply_noalloc) and
+    raises TnhError (TNH_ERR_NOMEM) for shapes whose scratch would not fit."""
+    ctx = _ctx(x)
+    c = complex(coeff)
+    ctx.call("tnh_heff_apply_noalloc", handle(L), W1.h, W2.h, handle(R), x.h, out.h, c.real, c.imag,
+             precision_code(precision))
+    return out
+
+
 def env_left(Lprev, A, W, out=None, precision="f64") -> Tensor:
     """buildleft!, projmps.jl:50-66. Lprev = None is the edge block."""
     ctx = _ctx(A)
