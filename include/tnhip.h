@@ -255,6 +255,39 @@ int tnh_inner_oplist(tnh_ctx *ctx, int N, const tnh_buf *const *psi, int nterms,
 int tnh_mpo_expect_grad(tnh_ctx *ctx, int N, tnh_buf *const *psi, tnh_buf *const *W, int block,
                         double *energy_re, double *energy_im, tnh_buf **dW);
 
+/* ------------------------------------------------------------------- MPOs */
+
+/* MPO of an operator list: replaces MPO(st, H; cutoff, maxdim, mindim),
+ * src/structures/mps/mpo.jl:323-459. The finite-state automaton (channel 1 = nothing placed,
+ * the last = term complete, one channel per bond that a multi-site term crosses, identities
+ * in a term's gaps) is built on the host, then the SVD compression sweeps of :442-457 run on
+ * the device (left to right W_s <- U S, W_{s+1} <- V W_{s+1}; right to left W_s <- S V,
+ * W_{s-1} <- W_{s-1} U; each split truncated by (cutoff, maxdim, mindim) on one-sided-Jacobi
+ * singular values). cutoff < 0 skips the compression. Term t acts on sites
+ * first_site[t] .. first_site[t] + range[t] - 1 (1-based) with range[t] d x d operator
+ * matrices in `ops` (column-major O[s_out, s_in], interleaved complex, identity where the
+ * term has no operator: the layout of tnh_inner_oplist) and coefficient
+ * coeffs[2t] + i coeffs[2t + 1]. W receives N new (D_l, d, d, D_r) tensors, F64 when every
+ * entry is real, else C128. */
+int tnh_mpo_from_oplist(tnh_ctx *ctx, int N, int d, int nterms, const int *first_site, const int *range,
+                        const double *ops, const double *coeffs, double cutoff, int maxdim, int mindim,
+                        tnh_buf **W);
+/* The compression sweeps of MPO(st, H), src/structures/mps/mpo.jl:442-457, on N device MPO
+ * tensors (handles replaced in place, bond dimensions may shrink). */
+int tnh_mpo_compress(tnh_ctx *ctx, int N, tnh_buf **W, double cutoff, int maxdim, int mindim);
+/* phi = O psi: replaces MPOMPSProduct(O, psi; cutoff, maxdim, mindim),
+ * src/structures/mps/mpo.jl:125-147 (applyMPO(O, psi), :105-123):
+ *   phi[i][(w,a), s, (w',b)] = sum_t O[i][w,s,t,w'] psi[i][a,t,b]   (w fastest in each bond),
+ * then the centre moves from site 1 to N without truncation and back to site 1 with
+ * (cutoff, maxdim, mindim). phi receives N new device tensors; FP64 / C128 operands. */
+int tnh_mpo_apply(tnh_ctx *ctx, int N, const tnh_buf *const *O, const tnh_buf *const *psi, double cutoff,
+                  int maxdim, int mindim, tnh_buf **phi);
+/* O = O1 O2: replaces MPOMPOProduct(O1, O2; cutoff, maxdim, mindim), mpo.jl:149-169:
+ *   O[i][(w1,w2), o, t, (w1',w2')] = sum_k O1[i][w1,o,k,w1'] O2[i][w2,k,t,w2'],
+ * then the same two centre sweeps as tnh_mpo_apply (sites viewed as (D_l, d*d, D_r)). */
+int tnh_mpo_mpo_product(tnh_ctx *ctx, int N, const tnh_buf *const *O1, const tnh_buf *const *O2,
+                        double cutoff, int maxdim, int mindim, tnh_buf **O);
+
 /* ------------------------------------------------ quantum-jump trajectories */
 
 /* One QJMC problem: the Trotter gates of qjmc_gates (src/algorithms/mps/qjmc.jl:9-26, i.e.

@@ -976,12 +976,6 @@ static JacobiResult jacobi(tnh_ctx *ctx, bool two_sided, double *X, int rows, in
 
 // ------------------------------------------------------------- decomposition
 
-struct SvdOut {
-  int m = 0, n = 0, k = 0;
-  std::vector<double> s;  // all singular values, descending
-  double discarded = 0;
-  bool fallback = false;
-};
 
 // Host copy of the truncation rule, src/tensors.jl:202-215.
 static int truncation_rank(const std::vector<double> &S, double cutoff, int maxdim, int mindim,

@@ -254,6 +254,22 @@ void replacesites2(tnh_ctx *ctx, const tnh_buf *theta, int direction, int normal
 void mps_movecenter(tnh_ctx *ctx, int N, tnh_buf **s, int center, int idx, double cutoff,
                     int maxdim, int mindim);
 
+// Outcome of one truncated decomposition: all singular values (descending), kept rank,
+// discarded weight and whether the one-sided fallback ran.
+struct SvdOut {
+  int m = 0, n = 0, k = 0;
+  std::vector<double> s;
+  double discarded = 0;
+  bool fallback = false;
+};
+// Factor the m x n matrix of `src` (any dtype; column-major, its leading dims are rows) as
+// left * right with the truncation rule: side 0 -> left = U_k (orthonormal columns),
+// right = U_k^H M; side 1 -> right = V_k^H (orthonormal rows), left = M V_k. left / right
+// are (re)allocated with dims ldims / rdims whose last / first entry is replaced by k.
+// force_fallback runs the one-sided Jacobi on M (accurate small singular values).
+int split_matrix(tnh_ctx *ctx, const tnh_buf *src, int m, int n, int side, double cutoff,
+                 int maxdim, int mindim, bool force_fallback, int nl, const int64_t *ldims,
+                 int nr, const int64_t *rdims, tnh_buf **A1, tnh_buf **A2, SvdOut *o);
 void mps_moveleft(tnh_ctx *ctx, tnh_buf **A, tnh_buf **B, double cutoff, int maxdim, int mindim);
 void mps_moveright(tnh_ctx *ctx, tnh_buf **A, tnh_buf **B, double cutoff, int maxdim, int mindim);
 

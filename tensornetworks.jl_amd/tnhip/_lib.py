@@ -114,6 +114,10 @@ SIGNATURES = {
     "tnh_mps_movecenter": (C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]),
     "tnh_applygate2": (C.c_int, [ctx_p, C.POINTER(buf_p), C.POINTER(buf_p), buf_p, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "tnh_inner_oplist": (C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "tnh_mpo_from_oplist": (C.c_int, [ctx_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double, C.c_int, C.c_int, C.POINTER(buf_p)]),
+    "tnh_mpo_compress": (C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.c_double, C.c_int, C.c_int]),
+    "tnh_mpo_apply": (C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.POINTER(buf_p), C.c_double, C.c_int, C.c_int, C.POINTER(buf_p)]),
+    "tnh_mpo_mpo_product": (C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.POINTER(buf_p), C.c_double, C.c_int, C.c_int, C.POINTER(buf_p)]),
     "tnh_qjmc_run": (C.c_int, [ctx_p, C.POINTER(QjmcProblem), C.c_int, C.POINTER(C.c_double), C.c_int64, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
     "tnh_qjmc_run_batched": (C.c_int, [ctx_p, C.POINTER(QjmcProblem), C.c_int, C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
     "tnh_mpo_expect_grad":(C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.POINTER(buf_p), C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(buf_p)]),

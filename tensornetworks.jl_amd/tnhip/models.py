@@ -253,9 +253,16 @@ class GMPS:
         return max([0] + [self.tensors[i].shape[0] for i in range(1, len(self))])
 
 
-def MPO(st: Sitetypes, H: OpList) -> GMPS:
-    """MPO(st, H), src/structures/mps/mpo.jl:323-459 (see mpo_tensors)."""
-    return GMPS(2, st.dim, mpo_tensors(st, H), 0)
+def MPO(st: Sitetypes, H: OpList, cutoff=1e-15, maxdim=0, mindim=1, compress=True) -> GMPS:
+    """MPO(st, H; cutoff=1e-15, maxdim=0, mindim=1), src/structures/mps/mpo.jl:323-459: the
+    automaton MPO followed by the reference's SVD compression sweeps, which run on the GPU
+    (tnh_mpo_from_oplist). compress=False returns the host-built automaton alone
+    (mpo_tensors), as CPU-only callers need."""
+    if not compress:
+        return GMPS(2, st.dim, mpo_tensors(st, H), 0)
+    from .mpo import mpo_device
+
+    return GMPS(2, st.dim, [t.numpy() for t in mpo_device(st, H, cutoff, maxdim, mindim)], 0)
 
 
 def productMPS(st: Sitetypes, names) -> GMPS:

@@ -107,7 +107,9 @@ def test_heff_matvec_and_split_full_size(ctx, chi):
     print(f"chi={chi}: building inputs", flush=True)
     rng = np.random.default_rng(0)
     D = 5
-    W = np.asfortranarray(MPO(spinhalf(), heisenberg_oplist(4)).tensors[1])  # bulk (5, 2, 2, 5)
+    # the automaton bulk tensor (5, 2, 2, 5); the compressed MPO of a 4-site chain has D = 4
+    # at its first bond, where one site spans only d^2 = 4 operators
+    W = np.asfortranarray(MPO(spinhalf(), heisenberg_oplist(4), compress=False).tensors[1])
     assert W.shape == (D, 2, 2, D) and not np.iscomplexobj(W)
 
     def env():
@@ -199,7 +201,7 @@ def test_tfim_n100_precisions_vs_closed_form_and_cpu(ctx):
 def test_gradient_inhomogeneous_tfim_n64_chi512(ctx):
     """E(J, h) = <psi|H|psi> for the N = 64 inhomogeneous TFIM (J_i ~ U(0.8, 1.2),
     h_i ~ U(0.5, 1.5) from seed 1) at its DMRG ground state held at bond dimension 512
-    (converged at cutoff 1e-14, then two passes with mindim = maxdim = 512): the device
+    (converged at cutoff 1e-14, then six passes with mindim = maxdim = 512): the device
     gradient (checkpoints every 8 sites) against central differences of the device energy
     (step 1e-5) and against Hellmann-Feynman from free fermions, dE0/dJ_i and dE0/dh_i by
     central differences of the exact BdG ground energy."""
@@ -220,7 +222,9 @@ def test_gradient_inhomogeneous_tfim_n64_chi512(ctx):
     # after 30 passes); a 10-vector Krylov space with 6 restarts converges in ~5 passes
     kw = dict(krylovdim=10, kryloviter=6, verbose=False)
     psi, E = dmrg(psi, W, maxsweeps=12, cutoff=1e-14, maxdim=128, **kw)
-    psi, E = dmrg(psi, W, maxsweeps=2, cutoff=0.0, maxdim=512, mindim=512, **kw)
+    # a split of a (chi, 2, 2, chi) theta keeps at most 2 chi values: from chi ~ 30 the bond
+    # reaches 512 after five passes at mindim = maxdim = 512, cutoff 0
+    psi, E = dmrg(psi, W, maxsweeps=6, cutoff=0.0, maxdim=512, mindim=512, **kw)
     e0 = tfim_bdg_ground_energy(J, h)
     print(f"N=64 gradient state: chi={psi.maxbonddim()} |E-E0|/N={abs(E - e0) / N:.2e}", flush=True)
     assert psi.maxbonddim() == 512

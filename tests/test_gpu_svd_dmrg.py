@@ -141,7 +141,7 @@ def tfim_w():
         H.add(["z", "z"], [i, i + 1], -1.0)
     for i in range(1, 5):
         H.add("x", i, -0.9)
-    return MPO(spinhalf(), H).tensors[1]
+    return MPO(spinhalf(), H, compress=False).tensors[1]
 
 
 @pytest.mark.parametrize("cplx", [False, True])

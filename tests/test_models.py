@@ -32,7 +32,7 @@ def test_heisenberg_mpo_matches_dense(pauli):
     st = spinhalf()
     N = 6
     H = heisenberg_oplist(N, pauli)
-    M = MPO(st, H)
+    M = MPO(st, H, compress=False)
     assert max(w.shape[3] for w in M.tensors[:-1]) == 5  # D = 2 + 3 channels
     assert np.allclose(exact.mpo_to_dense(M.tensors), dense_from_oplist(H))
     # s+/s- form is real; the Pauli-y form is complex
@@ -53,7 +53,7 @@ def test_tfim_with_fields_and_long_range_term():
     for i in range(1, N):
         H.add(["z", "z"], [i, i + 1], 1.2)
     H.add(["x", "z"], [1, 4], 0.3)  # range-4 term with identity gaps
-    M = MPO(st, H)
+    M = MPO(st, H, compress=False)
     assert np.allclose(exact.mpo_to_dense(M.tensors), dense_from_oplist(H))
 
 
