@@ -43,7 +43,7 @@ def heisenberg_bulk_w():
         H.add(["s+", "s-"], [i, i + 1], 0.5)
         H.add(["s-", "s+"], [i, i + 1], 0.5)
         H.add(["z", "z"], [i, i + 1], 0.25)
-    return MPO(spinhalf(), H).tensors[1]  # (5, 2, 2, 5), real
+    return MPO(spinhalf(), H, compress=False).tensors[1]  # automaton bulk tensor (5, 2, 2, 5), real
 
 
 def make_inputs(chi, D, seed, torch):
@@ -216,6 +216,65 @@ def dmrg_tfim_n100(chi, sweeps, refine_tol=None, precisions=("f64", "f32"), cpu=
     res["config"] = {"N": N, "g": 1.0, "chi_max": chi, "passes": sweeps, "cutoff": 1e-12, "krylovdim": 3,
                      "refine_tol": refine_tol, "E0_exact": e_exact}
     return res
+
+
+def dmrg_tfim_n100_chi2048(chi=2048, precisions=("f64", "f32", "bf16x3"), warm_passes=12, profile=False):
+    """Half-sweeps of two-site DMRG on the critical TFIM, N = 100, at bond dimension chi:
+    a random chi = 16 start (seed 0) is converged at chi_max 64 (cutoff 1e-12, warm_passes
+    passes), then every pass keeps mindim = maxdim = chi with cutoff 0, so each bond not
+    capped by the chain ends (2^min(i, N - i) < chi) runs at chi = 2048. One FP64 pass grows
+    the bonds; then one timed pass per precision (matvecs and environment updates in it;
+    Krylov vectors, Gram eigensolve and reductions FP64), each followed by one FP64 pass whose
+    energy is compared with the all-FP64 pass. Reports seconds per half-sweep and |dE|/N
+    against the closed form and between the precisions."""
+    from tnhip import MPO, Context, randomMPS, spinhalf
+    from tnhip.mps import DMRGState, dmrg
+
+    N = 100
+    e_exact = 1.0 - 1.0 / np.sin(np.pi / (4 * N + 2))
+    M = MPO(spinhalf(), tfim_oplist(N))
+    psi = randomMPS(2, N, 16, rng=np.random.default_rng(0))
+    t0 = time.perf_counter()
+    psi, e_small = dmrg(psi, M, maxsweeps=warm_passes, minsweeps=warm_passes, tol=0.0, cutoff=1e-12, maxdim=64,
+                        verbose=False)
+    setup_s = time.perf_counter() - t0
+    ctx = Context.default()
+    st = DMRGState(psi, [M])
+    state = {"direction": False}
+
+    def one_pass(prec):
+        if profile:
+            ctx.profile_reset()
+            ctx.profile(True)
+        ctx.synchronize()
+        t = time.perf_counter()
+        e, D, disc = st.sweep(state["direction"], 3, 2, 1e-14, 0.0, chi, chi, prec)
+        el = time.perf_counter() - t
+        state["direction"] = not state["direction"]
+        r = {"seconds": el, "energy": e, "abs_dE_per_site_vs_exact": abs(e - e_exact) / N, "maxbonddim": D,
+             "precision": prec, "direction": "left" if not state["direction"] else "right"}
+        if profile:
+            r["kernel_ms"] = {f: round(ctx.profile_read(f)["ms"], 2)
+                              for f in ("gemm_f64", "gemm_f32", "gemm_bf16x3", "mix", "jacobi")}
+            ctx.profile(False)
+        print(f"chi={chi} pass {prec}: {el:.2f} s, E={e:.12f}, chi_max={D}", file=sys.stderr, flush=True)
+        return r
+
+    out = {"config": {"N": N, "chi": chi, "bonds_at_chi": sum(1 for b in range(1, N) if 2 ** min(b, N - b) >= chi),
+                      "krylovdim": 3, "maxiter": 2, "warm_passes_chi64": warm_passes, "E0_exact": e_exact},
+           "setup_seconds_chi64": setup_s, "energy_chi64": e_small}
+    out["grow_f64"] = one_pass("f64")
+    ref = one_pass("f64")
+    out["f64"] = ref
+    for prec in precisions:
+        if prec == "f64":
+            continue
+        out[prec] = one_pass(prec)
+        out[prec + "_then_f64"] = r2 = one_pass("f64")
+        out[f"abs_dE_per_site_{prec}_refined_vs_f64"] = abs(r2["energy"] - ref["energy"]) / N
+        out[f"{prec}_half_sweep_speedup_vs_f64"] = ref["seconds"] / out[prec]["seconds"]
+    st.close()
+    return out
 
 
 def gradient_tfim_n64(chi=512, block=8):
@@ -528,6 +587,10 @@ def main(argv=None):
     ap.add_argument("--legs-only", action="store_true", help="run only the gradient and QJMC legs")
     ap.add_argument("--dmrg-small-only", action="store_true",
                     help="run only the N=32 Heisenberg DMRG leg (GPU, and the CPU port unless --no-cpu)")
+    ap.add_argument("--sweep2048-only", action="store_true",
+                    help="run only the N=100 TFIM half-sweeps at bond dimension 2048 (FP64, FP32, BF16x3)")
+    ap.add_argument("--profile-passes", action="store_true",
+                    help="with --sweep2048-only: kernel-family times per pass (HIP events)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--tebd-only", action="store_true", help="run only the real-time TEBD leg")
     ap.add_argument("--qjmc-only", action="store_true",
@@ -570,6 +633,9 @@ def main(argv=None):
     if a.dmrg_only:
         print(json.dumps({"dmrg_tfim_N100": dmrg_tfim_n100(a.dmrg_chi, a.dmrg_passes, a.refine_tol,
                                                            ("f64", "f32", "bf16x3"), cpu=not a.no_cpu)}))
+        return 0
+    if a.sweep2048_only:
+        print(json.dumps({"dmrg_tfim_N100_chi%d_sweeps" % chi: dmrg_tfim_n100_chi2048(chi, profile=a.profile_passes)}))
         return 0
     if a.dmrg_small_only:
         print(json.dumps({"dmrg_heisenberg_N32": dmrg_heisenberg_n32(cpu=not a.no_cpu)}))
