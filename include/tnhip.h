@@ -74,10 +74,12 @@ const char *tnh_version(void);
 
 /* Kernel timing (no counterpart in the reference, which has no profiling): while
  * enabled, HIP events on the context's stream bracket every launch of the kernel
- * families "gemm_f64", "gemm_f32", "gemm_bf16x3" (matrix-core GEMMs), "mix" (MPO /
- * gate mixing stage) and "jacobi" (SVD sweeps). tnh_profile_read synchronises and
- * returns the launch count, the summed kernel time and the summed algorithmic work
- * (flops = 2 * multiply-adds; bytes = operands read once + results written once). */
+ * families "gemm_f64", "gemm_f32", "gemm_bf16x3" (matrix-core GEMMs) and "mix" (MPO /
+ * gate mixing stage), and the intervals "jacobi" (one block-Jacobi eigensolve of a split,
+ * its "flops" field counting outer sweeps) and "dmrg_env", "dmrg_lanczos", "dmrg_split"
+ * (the phases of each bond step of tnh_dmrg_sweep). tnh_profile_read synchronises and
+ * returns the count, the summed time and the summed algorithmic work (flops = 2 *
+ * multiply-adds; bytes = operands read once + results written once). */
 int tnh_profile_enable(tnh_ctx *ctx, int on);
 int tnh_profile_reset(tnh_ctx *ctx);
 int tnh_profile_read(tnh_ctx *ctx, const char *family, int64_t *launches, double *total_ms,

@@ -22,6 +22,10 @@ struct tnh_dmrg {
   std::vector<tnh_buf *> shadowL, shadowR;    // FP32 copies of the bond's blocks, per term
   tnh_buf *theta = nullptr;
   tnh_buf *tmp = nullptr;
+  // Per bond and sweep direction, the eigenvectors of the last split's Gram matrix there:
+  // the next split of that bond in the same direction starts its Jacobi from them
+  // (p x p FP64 each, p = 2 chi: 134 MB per bond at chi = 2048).
+  std::vector<tnh_buf *> warm[2];
   int env_center = 0;   // ProjMPS centre (0 = blocks not built)
   int psi_center = 0;
 
@@ -39,6 +43,8 @@ struct tnh_dmrg {
     for (auto &b : shadowR) del(b);
     del(theta);
     del(tmp);
+    for (auto &v : warm)
+      for (auto &b : v) del(b);
   }
 
   // block(projV, idx), abstractprojmps.jl:33-46 (nullptr = edge block)
@@ -175,6 +181,8 @@ int tnh_dmrg_create(tnh_ctx *ctx, int N, const tnh_buf *const *psi, int nterms,
     st->shadowL.assign(nterms, nullptr);
     st->shadowR.assign(nterms, nullptr);
     st->coeffs.assign(nterms, 1.0);
+    st->warm[0].assign(N, nullptr);
+    st->warm[1].assign(N, nullptr);
     for (int t = 0; t < nterms; ++t) {
       if (coeffs) st->coeffs[t] = coeffs[t];
       for (int i = 0; i < N; ++i) copy_in(ctx, mpo[size_t(t) * N + i], cplx, &st->W[t][i]);
@@ -212,7 +220,12 @@ int tnh_dmrg_sweep(tnh_dmrg *st, int direction, const tnh_dmrg_params *p, double
     for (int j = 1; j <= N - 1; ++j) {  // dmrg.jl:35
       const int site = direction ? N + 1 - j : j;
       const int site1 = direction ? site - 1 : site;
+      // with profiling on, HIP events bracket the phases of the bond step (the interval
+      // includes the host round trips inside a phase)
+      hipEvent_t ph = ctx->prof_begin();
       st->env_movecenter(site, prec);  // :41
+      ctx->prof_end("dmrg_env", ph, 0, 0);
+      ph = ctx->prof_begin();
       // the psi centre must sit on the bond: it does after the previous split
       two_site_theta(ctx, st->psi[site1 - 1], st->psi[site1], &st->theta);  // :44-47
       if (prec != TNH_PREC_F64) make_shadows(st, site1);
@@ -220,10 +233,14 @@ int tnh_dmrg_sweep(tnh_dmrg *st, int direction, const tnh_dmrg_params *p, double
       cost = eigsolve_ground_op(
           ctx, [&](const tnh_buf *x, tnh_buf *y) { bond_matvec(st, site1, prec, x, y); },
           st->theta, p->krylovdim, p->maxiter, p->lanczos_tol, &nmv);  // :50-55
+      ctx->prof_end("dmrg_lanczos", ph, 0, 0);
+      ph = ctx->prof_begin();
       int k = 0;
       double disc = 0;
       replacesites2(ctx, st->theta, direction, 1, p->cutoff, p->maxdim, p->mindim,
-                    &st->psi[site1 - 1], &st->psi[site1], &k, &disc, nullptr, 0, false);  // :57
+                    &st->psi[site1 - 1], &st->psi[site1], &k, &disc, nullptr, 0, false,
+                    &st->warm[direction ? 1 : 0][site1 - 1]);  // :57
+      ctx->prof_end("dmrg_split", ph, 0, 0);
       st->psi_center = direction ? site1 : site1 + 1;
       maxdisc = std::max(maxdisc, disc);
     }

@@ -663,6 +663,56 @@ __global__ void gather_cols_kernel(const double *src, int rows, int64_t imS, con
   if (cplx) out[imO + e] = f * src[imS + s];
 }
 
+// Fixes the gauge of each column of a (planar) basis: the column is multiplied by the phase
+// that makes its entry of largest modulus (the first such entry) real and positive. The
+// eigensolver's own sign or phase choice then no longer reaches the split factors, so a bond
+// basis comes out the same each time the same subspace is split. One workgroup per column.
+__global__ __launch_bounds__(256) void canon_phase_kernel(double *U, int rows, int64_t imU, bool cplx) {
+  const int j = blockIdx.x;
+  double *ur = U + int64_t(j) * rows;
+  double *ui = cplx ? U + imU + int64_t(j) * rows : nullptr;
+  __shared__ double bm[256];
+  __shared__ int bi[256];
+  double best = -1.0;
+  int at = 0;
+  for (int r = threadIdx.x; r < rows; r += 256) {
+    const double m = cplx ? ur[r] * ur[r] + ui[r] * ui[r] : ur[r] * ur[r];
+    if (m > best) {
+      best = m;
+      at = r;
+    }
+  }
+  bm[threadIdx.x] = best;
+  bi[threadIdx.x] = at;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      const double o = bm[threadIdx.x + w];
+      const int oi = bi[threadIdx.x + w];
+      if (o > bm[threadIdx.x] || (o == bm[threadIdx.x] && oi < bi[threadIdx.x])) {
+        bm[threadIdx.x] = o;
+        bi[threadIdx.x] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  const int r0 = bi[0];
+  const double mr = ur[r0], mi = cplx ? ui[r0] : 0.0;
+  const double mag = sqrt(mr * mr + mi * mi);
+  __syncthreads();  // every thread has read the pivot before any thread rescales it
+  if (!(mag > 0.0)) return;
+  const double fr = mr / mag, fi = -mi / mag;  // conj(u) / |u|
+  for (int r = threadIdx.x; r < rows; r += 256) {
+    if (cplx) {
+      const double a = ur[r], b = ui[r];
+      ur[r] = a * fr - b * fi;
+      ui[r] = a * fi + b * fr;
+    } else {
+      ur[r] = ur[r] * fr;
+    }
+  }
+}
+
 // X[i, c] *= f[i] for a column-major matrix with `rows` rows (tot elements over all planes)
 __global__ void scale_rows_kernel(double *X, int rows, int64_t tot, const double *f) {
   const int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
@@ -1017,7 +1067,8 @@ static int truncation_rank(const std::vector<double> &S, double cutoff, int maxd
 // side 1 ("right"): returns the right factor basis Vk (n x k) with M ~ (M Vk) Vk^H.
 // The basis buffer is workspace slot 620 (valid until the next SVD).
 static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx, int side,
-                         double cutoff, int maxdim, int mindim, bool force_fallback, SvdOut *o) {
+                         double cutoff, int maxdim, int mindim, bool force_fallback, SvdOut *o,
+                         tnh_buf **warm = nullptr) {
   o->m = m;
   o->n = n;
   const int p = side == 0 ? m : n;  // basis dimension
@@ -1091,10 +1142,59 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
       scale(ctx, G, TNH_F64, pp * planes, std::ldexp(1.0, -ex));
       dmax = std::ldexp(dmax, -ex);
     }
-    hipLaunchKernelGGL(eye_kernel, dim3(unsigned((pp + 255) / 256)), dim3(256), 0, ctx->stream, V, p,
-                       pp, cplx);
+    // Warm start: with an orthonormal basis Q of this Gram's space from an earlier split of
+    // the same bond (its eigenvectors then), the Jacobi runs on Q^H G Q, nearly diagonal when
+    // the matrix has changed little, and accumulates its rotations onto V = Q, so that V ends
+    // as the eigenvectors of G itself. Same convergence test, so the same accuracy.
+    const int dtp = cplx ? TNH_C128 : TNH_F64;
+    const bool use_warm = warm && *warm && (*warm)->ndim == 2 && (*warm)->dims[0] == p &&
+                          (*warm)->dims[1] == p && (*warm)->dtype == dtp;
+    if (use_warm) {
+      const double *Q = static_cast<const double *>((*warm)->ptr);
+      double *T = static_cast<double *>(ctx->workspace(618, sizeof(double) * pp * planes));
+      Gemm g;
+      g.M = p;
+      g.N = p;
+      g.K = p;
+      g.lda = p;
+      g.ldb = p;
+      g.ldc = p;
+      COperand Go, Qo, To;
+      Go.re = G;
+      Go.im = cplx ? G + pp : nullptr;
+      Qo.re = Q;
+      Qo.im = cplx ? Q + pp : nullptr;
+      To.re = T;
+      To.im = cplx ? T + pp : nullptr;
+      cgemm(ctx, g, Go, Qo, T, cplx ? T + pp : nullptr, TNH_F64, TNH_PREC_F64);  // T = G Q
+      Gemm h = g;
+      h.transA = true;
+      COperand Qh = Qo;
+      Qh.conj = true;
+      cgemm(ctx, h, Qh, To, G, cplx ? G + pp : nullptr, TNH_F64, TNH_PREC_F64);  // G = Q^H T
+      hipLaunchKernelGGL(hermitize_kernel, dim3(unsigned((pp + 255) / 256)), dim3(256), 0, ctx->stream,
+                         G, p, pp, cplx);
+      copy_device(ctx, V, Q, sizeof(double) * pp * planes);
+      // the noise floor below is relative to max_i G_ii of the matrix the Jacobi sees
+      double *dv = static_cast<double *>(ctx->workspace(612, sizeof(double) * p));
+      hipLaunchKernelGGL(diag_kernel, dim3((p + 255) / 256), dim3(256), 0, ctx->stream, G, p, dv);
+      TNH_HIP(hipGetLastError());
+      to_host(ctx, val.data(), dv, sizeof(double) * p);
+      dmax = 0;
+      for (double v : val) dmax = std::max(dmax, std::fabs(v));
+    } else {
+      hipLaunchKernelGGL(eye_kernel, dim3(unsigned((pp + 255) / 256)), dim3(256), 0, ctx->stream, V, p,
+                         pp, cplx);
+    }
     const double abstol = 4.0 * 2.220446049250313e-16 * std::sqrt(double(p)) * dmax;
+    hipEvent_t pj = ctx->prof_begin();
     JacobiResult jr = jacobi(ctx, true, G, p, p, cplx, V, tol, abstol, 40);
+    ctx->prof_end(use_warm ? "jacobi_warm" : "jacobi", pj, double(jr.sweeps), double(p));
+    if (jr.converged && warm) {  // the eigenvectors become the next split's warm start
+      const int64_t wd[2] = {p, p};
+      buf_ensure(ctx, warm, dtp, 2, wd);
+      copy_device(ctx, (*warm)->ptr, V, sizeof(double) * pp * planes);
+    }
     if (jr.converged) {
       double *dv = static_cast<double *>(ctx->workspace(612, sizeof(double) * p));
       hipLaunchKernelGGL(diag_kernel, dim3((p + 255) / 256), dim3(256), 0, ctx->stream, G, p, dv);
@@ -1174,6 +1274,8 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
   double *basis = static_cast<double *>(ctx->workspace(620, sizeof(double) * int64_t(p) * k * planes));
   hipLaunchKernelGGL(gather_cols_kernel, dim3(unsigned((int64_t(p) * k + 255) / 256)), dim3(256), 0,
                      ctx->stream, V, p, pp, dperm, nullptr, k, basis, int64_t(p) * k, cplx);
+  hipLaunchKernelGGL(canon_phase_kernel, dim3(unsigned(k)), dim3(256), 0, ctx->stream, basis, p,
+                     int64_t(p) * k, cplx);
   TNH_HIP(hipGetLastError());
   // One Newton-Schulz step toward the nearest orthonormal basis, U <- U (3I - U^H U) / 2:
   // the accumulated Jacobi rotations leave U^H U - I ~ 1e-13, this squares it away.
@@ -1239,11 +1341,13 @@ static double frob(tnh_ctx *ctx, const tnh_buf *b) {
 // Returns the kept rank; all singular values go to o->s.
 int split_matrix(tnh_ctx *ctx, const tnh_buf *src, int m, int n, int side, double cutoff,
                  int maxdim, int mindim, bool force_fallback, int nl, const int64_t *ldims,
-                 int nr, const int64_t *rdims, tnh_buf **A1, tnh_buf **A2, SvdOut *o);
+                 int nr, const int64_t *rdims, tnh_buf **A1, tnh_buf **A2, SvdOut *o,
+                 tnh_buf **warm);
 
 void replacesites2(tnh_ctx *ctx, const tnh_buf *theta, int direction, int normalize,
                    double cutoff, int maxdim, int mindim, tnh_buf **A1, tnh_buf **A2, int *k_out,
-                   double *disc_out, double *schmidt, int schmidt_cap, bool force_fallback) {
+                   double *disc_out, double *schmidt, int schmidt_cap, bool force_fallback,
+                   tnh_buf **warm) {
   TNH_REQUIRE(theta && theta->ndim == 4, TNH_ERR_SHAPE, "theta must be (chi_l, d, d, chi_r)");
   TNH_REQUIRE(A1 && A2, TNH_ERR_ARG, "A1/A2 out pointers are NULL");
   const int64_t cl = theta->dims[0], d1 = theta->dims[1], d2 = theta->dims[2], cr = theta->dims[3];
@@ -1253,7 +1357,7 @@ void replacesites2(tnh_ctx *ctx, const tnh_buf *theta, int direction, int normal
   const int side = direction ? 1 : 0;
   int64_t d1v[3] = {cl, d1, 0}, d2v[3] = {0, d2, cr};
   const int k = split_matrix(ctx, theta, m, n, side, cutoff, maxdim, mindim, force_fallback, 3,
-                             d1v, 3, d2v, A1, A2, &o);
+                             d1v, 3, d2v, A1, A2, &o, warm);
   if (normalize) {
     // normalize!(psi): divide the centre tensor by its norm, gmps.jl:46-51
     tnh_buf *c = side == 0 ? *A2 : *A1;
@@ -1269,11 +1373,12 @@ void replacesites2(tnh_ctx *ctx, const tnh_buf *theta, int direction, int normal
 
 int split_matrix(tnh_ctx *ctx, const tnh_buf *src, int m, int n, int side, double cutoff,
                  int maxdim, int mindim, bool force_fallback, int nl, const int64_t *ldims_in,
-                 int nr, const int64_t *rdims_in, tnh_buf **A1, tnh_buf **A2, SvdOut *o) {
+                 int nr, const int64_t *rdims_in, tnh_buf **A1, tnh_buf **A2, SvdOut *o,
+                 tnh_buf **warm) {
   TNH_REQUIRE(int64_t(m) * n == src->numel, TNH_ERR_SHAPE, "split: matrix size != tensor size");
   const bool cplx = is_complex(src->dtype);
   const double *M = as_f64(ctx, src, 630);
-  double *basis = decompose(ctx, M, m, n, cplx, side, cutoff, maxdim, mindim, force_fallback, o);
+  double *basis = decompose(ctx, M, m, n, cplx, side, cutoff, maxdim, mindim, force_fallback, o, warm);
   const int k = o->k;
   const int dt = cplx ? TNH_C128 : TNH_F64;
   int64_t d1v[8], d2v[8];

@@ -248,9 +248,13 @@ void cdot_host(tnh_ctx *ctx, const tnh_buf *x, const tnh_buf *y, double *re, dou
 using Matvec = std::function<void(const tnh_buf *x, tnh_buf *y)>;
 double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, int kdim,
                           int maxiter, double tol, int *nmv_out);
+// warm (optional, in/out): a p x p basis of the Gram space from an earlier split of the same
+// bond, used as the Jacobi's starting basis when its size and dtype match, and replaced by
+// this split's eigenvectors (block-Jacobi path, p > 64).
 void replacesites2(tnh_ctx *ctx, const tnh_buf *theta, int direction, int normalize,
                    double cutoff, int maxdim, int mindim, tnh_buf **A1, tnh_buf **A2, int *k_out,
-                   double *disc_out, double *schmidt, int schmidt_cap, bool force_fallback);
+                   double *disc_out, double *schmidt, int schmidt_cap, bool force_fallback,
+                   tnh_buf **warm = nullptr);
 void mps_movecenter(tnh_ctx *ctx, int N, tnh_buf **s, int center, int idx, double cutoff,
                     int maxdim, int mindim);
 
@@ -269,7 +273,8 @@ struct SvdOut {
 // force_fallback runs the one-sided Jacobi on M (accurate small singular values).
 int split_matrix(tnh_ctx *ctx, const tnh_buf *src, int m, int n, int side, double cutoff,
                  int maxdim, int mindim, bool force_fallback, int nl, const int64_t *ldims,
-                 int nr, const int64_t *rdims, tnh_buf **A1, tnh_buf **A2, SvdOut *o);
+                 int nr, const int64_t *rdims, tnh_buf **A1, tnh_buf **A2, SvdOut *o,
+                 tnh_buf **warm = nullptr);
 void mps_moveleft(tnh_ctx *ctx, tnh_buf **A, tnh_buf **B, double cutoff, int maxdim, int mindim);
 void mps_moveright(tnh_ctx *ctx, tnh_buf **A, tnh_buf **B, double cutoff, int maxdim, int mindim);
 

@@ -310,3 +310,28 @@ def test_dmrg_two_terms_sum(ctx, rng):
     psi, e = dmrg(psi, MPO(st, Hzz), MPO(st, Hx), coeffs=[1.0, 1.3], maxdim=64, cutoff=1e-12,
                   maxsweeps=10, krylovdim=4, verbose=False)
     assert abs(e - tfim_bdg_ground_energy(np.ones(N - 1), 1.3 * np.ones(N))) / N <= 1e-10
+
+
+@pytest.mark.parametrize("p", [512, 2048])
+@pytest.mark.parametrize("cutoff", [1e-10, 1e-12])
+@pytest.mark.parametrize("decay", [0.9, 0.8])
+def test_svd_cutoff_rank_matches_lapack_large(ctx, p, cutoff, decay):
+    """Cutoff-driven truncation at Gram sides 512 and 2048: a p x p matrix with singular
+    values decay^i (random orthogonal factors), whose discarded-weight tail crosses the
+    cutoff among values of relative size ~cutoff, where the Gram eigensolve carries an
+    absolute error ~eps sqrt(p) s_max^2. The kept rank equals the truncation rule applied
+    to LAPACK's singular values (tensors.jl:202-215) and the discarded weight agrees."""
+    from tnhip import Tensor, ops as T
+    from oracle.tensors import truncation_rank
+
+    rng = np.random.default_rng(p + int(100 * decay))
+    U, _ = np.linalg.qr(rng.standard_normal((p, p)))
+    V, _ = np.linalg.qr(rng.standard_normal((p, p)))
+    M = np.asfortranarray((U * decay ** np.arange(p)) @ V.T)
+    s_ref = np.linalg.svd(M, compute_uv=False)
+    k_ref, disc_ref = truncation_rank(s_ref, cutoff, 0, 1)
+    _, S, _, disc, fb = T.svd_trunc(Tensor.from_numpy(M), cutoff, 0, 1, want=(False, True, False))
+    assert not fb
+    assert S.shape[0] == k_ref, (S.shape[0], k_ref)
+    # each s^2 / sum s^2 of the Gram path carries an absolute error ~ eps sqrt(p)
+    assert abs(disc - disc_ref) <= 1e-13
