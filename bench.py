@@ -218,7 +218,8 @@ def dmrg_tfim_n100(chi, sweeps, refine_tol=None, precisions=("f64", "f32"), cpu=
     return res
 
 
-def dmrg_tfim_n100_chi2048(chi=2048, precisions=("f64", "f32", "bf16x3"), warm_passes=12, profile=False):
+def dmrg_tfim_n100_chi2048(chi=2048, precisions=("f64", "f32", "bf16x3"), warm_passes=12, profile=False,
+                           timed=None):
     """Half-sweeps of two-site DMRG on the critical TFIM, N = 100, at bond dimension chi:
     a random chi = 16 start (seed 0) is converged at chi_max 64 (cutoff 1e-12, warm_passes
     passes), then every pass keeps mindim = maxdim = chi with cutoff 0, so each bond not
@@ -255,7 +256,11 @@ def dmrg_tfim_n100_chi2048(chi=2048, precisions=("f64", "f32", "bf16x3"), warm_p
              "precision": prec, "direction": "left" if not state["direction"] else "right"}
         if profile:
             r["kernel_ms"] = {f: round(ctx.profile_read(f)["ms"], 2)
-                              for f in ("gemm_f64", "gemm_f32", "gemm_bf16x3", "mix", "jacobi")}
+                              for f in ("gemm_f64", "gemm_f32", "gemm_bf16x3", "mix", "jacobi", "jacobi_warm",
+                                        "dmrg_env", "dmrg_lanczos", "dmrg_split")}
+            for f in ("jacobi", "jacobi_warm"):
+                j = ctx.profile_read(f)
+                r[f + "_calls_sweeps"] = [int(j["launches"]), int(j["flops"])]
             ctx.profile(False)
         print(f"chi={chi} pass {prec}: {el:.2f} s, E={e:.12f}, chi_max={D}", file=sys.stderr, flush=True)
         return r
@@ -263,9 +268,22 @@ def dmrg_tfim_n100_chi2048(chi=2048, precisions=("f64", "f32", "bf16x3"), warm_p
     out = {"config": {"N": N, "chi": chi, "bonds_at_chi": sum(1 for b in range(1, N) if 2 ** min(b, N - b) >= chi),
                       "krylovdim": 3, "maxiter": 2, "warm_passes_chi64": warm_passes, "E0_exact": e_exact},
            "setup_seconds_chi64": setup_s, "energy_chi64": e_small}
-    out["grow_f64"] = one_pass("f64")
+    # a split keeps at most twice the bond dimension it started from: grow to chi first
+    out["grow_f64"] = []
+    while len(out["grow_f64"]) < 8:
+        r = one_pass("f64")
+        out["grow_f64"].append(r)
+        if r["maxbonddim"] >= chi:
+            break
+    # two more passes (one per direction): every split of the timed passes then starts its
+    # Jacobi from the eigenvectors of the same bond's previous split in that direction
+    out["grow_f64"].append(one_pass("f64"))
+    out["grow_f64"].append(one_pass("f64"))
     ref = one_pass("f64")
     out["f64"] = ref
+    if timed == "f64":  # diagnostics: the FP64 pass only
+        st.close()
+        return out
     for prec in precisions:
         if prec == "f64":
             continue
@@ -467,15 +485,17 @@ def bond_step(ctx, ops, Tensor, dL, dR, dW, dx, prec, chi, torch):
 
 
 def bond_steps(ctx, ops, Tensor, dL64, dR64, dL32, dR32, dW, dx, chi, torch):
-    """The bond step with FP64 and with FP32-MFMA matvecs (FP64 Lanczos vectors and SVD
-    in both), from the same theta."""
+    """The bond step with FP64, FP32-MFMA and split-BF16x3 matvecs (FP64 Lanczos vectors
+    and SVD in all), from the same random theta: a worst case for the split (a dense, flat
+    spectrum, cold-started Jacobi). The N = 100 TFIM half-sweeps at chi = 2048 are measured
+    separately (dmrg_tfim_n100_chi2048)."""
     steps_ = {}
-    for prec in ("f64", "f32"):
+    for prec in ("f64", "f32", "bf16x3"):
         dL, dR = (dL64, dR64) if prec == "f64" else (dL32, dR32)
         steps_[prec] = bond_step(ctx, ops, Tensor, dL, dR, dW, dx, prec, chi, torch)
-    steps_["bond_step_speedup_f32_vs_f64"] = steps_["f64"]["bond_step_s"] / steps_["f32"]["bond_step_s"]
-    steps_["lanczos_speedup_f32_vs_f64"] = steps_["f64"]["lanczos_s"] / steps_["f32"]["lanczos_s"]
-    steps_["half_sweep_estimate_s_N100"] = {p: 99 * steps_[p]["bond_step_s"] for p in ("f64", "f32")}
+    for prec in ("f32", "bf16x3"):
+        steps_[f"bond_step_speedup_{prec}_vs_f64"] = steps_["f64"]["bond_step_s"] / steps_[prec]["bond_step_s"]
+        steps_[f"lanczos_speedup_{prec}_vs_f64"] = steps_["f64"]["lanczos_s"] / steps_[prec]["lanczos_s"]
     return steps_
 
 
@@ -589,6 +609,8 @@ def main(argv=None):
                     help="run only the N=32 Heisenberg DMRG leg (GPU, and the CPU port unless --no-cpu)")
     ap.add_argument("--sweep2048-only", action="store_true",
                     help="run only the N=100 TFIM half-sweeps at bond dimension 2048 (FP64, FP32, BF16x3)")
+    ap.add_argument("--f64-pass-only", action="store_true",
+                    help="with --sweep2048-only: stop after the first timed FP64 pass")
     ap.add_argument("--profile-passes", action="store_true",
                     help="with --sweep2048-only: kernel-family times per pass (HIP events)")
     ap.add_argument("--seed", type=int, default=0)
@@ -635,7 +657,8 @@ def main(argv=None):
                                                            ("f64", "f32", "bf16x3"), cpu=not a.no_cpu)}))
         return 0
     if a.sweep2048_only:
-        print(json.dumps({"dmrg_tfim_N100_chi%d_sweeps" % chi: dmrg_tfim_n100_chi2048(chi, profile=a.profile_passes)}))
+        print(json.dumps({"dmrg_tfim_N100_chi%d_sweeps" % chi: dmrg_tfim_n100_chi2048(
+            chi, profile=a.profile_passes, timed="f64" if a.f64_pass_only else None)}))
         return 0
     if a.dmrg_small_only:
         print(json.dumps({"dmrg_heisenberg_N32": dmrg_heisenberg_n32(cpu=not a.no_cpu)}))
@@ -748,6 +771,7 @@ def main(argv=None):
         extra["dmrg_tfim_N100"] = dmrg_tfim_n100(a.dmrg_chi, a.dmrg_passes, a.refine_tol, ("f64", "f32", "bf16x3"),
                                                  cpu=not a.no_cpu)
         extra["dmrg_heisenberg_N32"] = dmrg_heisenberg_n32(cpu=not a.no_cpu)
+        extra["dmrg_tfim_N100_chi2048_sweeps"] = dmrg_tfim_n100_chi2048(2048)
     if not a.no_extra_legs:
         extra["gradient_tfim_N64"] = gradient_tfim_n64()
         extra["qjmc_tfim_N40"] = qjmc_tfim_n40()
