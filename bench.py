@@ -295,35 +295,90 @@ def dmrg_tfim_n100_chi2048(chi=2048, precisions=("f64", "f32", "bf16x3"), warm_p
     return out
 
 
-def gradient_tfim_n64(chi=512, block=8):
+def gradient_tfim_n64(chi=512, block=8, cpu=True):
     """E(J, h) = <psi|H(J, h)|psi> and its 127 coupling derivatives for the inhomogeneous TFIM,
-    N = 64, J_i ~ U(0.8, 1.2), h_i ~ U(0.5, 1.5) (seed 1), on a random right-canonical MPS of
-    bond dimension chi (seed 0; the timing does not depend on the state), checkpoints every
-    `block` sites. Accuracy: the gradient against inner()'s <O_t> (Hellmann-Feynman form)."""
-    from tnhip import randomMPS, spinhalf
+    N = 64, J_i ~ U(0.8, 1.2), h_i ~ U(0.5, 1.5) (seed 1), at its DMRG ground state held at
+    bond dimension chi (converged at chi_max 128, cutoff 1e-14, then six passes at
+    mindim = maxdim = chi), environments checkpointed every `block` sites. Accuracy: central
+    differences of the device energy (step 1e-5) on six couplings, and Hellmann-Feynman
+    against the free-fermion (BdG) ground energy's central differences for every coupling.
+    CPU baseline: the same 127 <O_t> = dE/dc_t from the oracle's environment contraction
+    (numpy/OpenBLAS on the host cores) on the same tensors."""
+    from tnhip import MPO, randomMPS, spinhalf
     from tnhip.grad import coupling_gradient
-    from tnhip.tebd import DeviceMPS, inner
+    from tnhip.models import OpList
+    from tnhip.mps import dmrg
+    from tnhip.tebd import DeviceMPS
 
     N = 64
     r = np.random.default_rng(1)
     J, h = r.uniform(0.8, 1.2, N - 1), r.uniform(0.5, 1.5, N)
-    from tnhip.models import OpList
-
     H = OpList(N)
     for i in range(1, N):
         H.add(["z", "z"], [i, i + 1], -J[i - 1])
     for i in range(1, N + 1):
         H.add("x", i, -h[i - 1])
     st = spinhalf()
-    psi = DeviceMPS.from_gmps(randomMPS(2, N, chi, rng=np.random.default_rng(0)))
-    coupling_gradient(st, psi, H, block=block)  # warm-up
-    psi.ctx.synchronize()
     t0 = time.perf_counter()
-    E, g = coupling_gradient(st, psi, H, block=block)
+    psi = randomMPS(2, N, 16, rng=np.random.default_rng(2))
+    kw = dict(krylovdim=10, kryloviter=6, verbose=False)
+    psi, _ = dmrg(psi, MPO(st, H), maxsweeps=12, cutoff=1e-14, maxdim=128, **kw)
+    psi, E_dmrg = dmrg(psi, MPO(st, H), maxsweeps=6, cutoff=0.0, maxdim=chi, mindim=chi, **kw)
+    prep_s = time.perf_counter() - t0
+    d = DeviceMPS.from_gmps(psi)
+    coupling_gradient(st, d, H, block=block)  # warm-up
+    d.ctx.synchronize()
+    t0 = time.perf_counter()
+    E, g = coupling_gradient(st, d, H, block=block)
     el = time.perf_counter() - t0
-    hf = inner(st, psi, H) / np.array(H.coeffs)
-    return {"seconds": el, "energy": float(E.real), "n_couplings": len(H.coeffs), "chi": psi.maxbonddim(),
-            "block": block, "max_abs_diff_vs_inner": float(np.abs(g - hf).max())}
+    step = 1e-5
+    fd_err = 0.0
+    for t in (0, 31, N - 2, N - 1, N - 1 + 40, 2 * N - 2):
+        Hp, Hm = H * 1.0, H * 1.0
+        Hp.coeffs[t] += step
+        Hm.coeffs[t] -= step
+        Ep, _ = coupling_gradient(st, d, Hp, block=block)
+        Em, _ = coupling_gradient(st, d, Hm, block=block)
+        fd_err = max(fd_err, abs((Ep - Em).real / (2 * step) - g[t].real))
+    out = {"seconds": el, "energy": float(E.real), "n_couplings": len(H.coeffs), "chi": psi.maxbonddim(),
+           "block": block, "state_prep_seconds": prep_s, "max_abs_fd_error": fd_err}
+
+    def bdg(Jv, hv):  # ground energy of the open TFIM from its BdG spectrum
+        A = np.diag(2 * hv) - np.diag(Jv, 1) - np.diag(Jv, -1)
+        B = np.diag(Jv, 1) - np.diag(Jv, -1)
+        w = np.linalg.eigvalsh(np.block([[A, B], [-B, -A]]))
+        return -0.5 * np.sum(w[w > 0])
+
+    e0 = bdg(J, h)
+    out["abs_dE_per_site_vs_bdg"] = abs(float(E.real) - e0) / N
+    eps = 1e-5
+    errs = []
+    for i in range(N - 1):
+        Jp, Jm = J.copy(), J.copy()
+        Jp[i] += eps
+        Jm[i] -= eps
+        errs.append(-g[i].real - (bdg(Jp, h) - bdg(Jm, h)) / (2 * eps))
+    for i in range(N):
+        hp, hm = h.copy(), h.copy()
+        hp[i] += eps
+        hm[i] -= eps
+        errs.append(-g[N - 1 + i].real - (bdg(J, hp) - bdg(J, hm)) / (2 * eps))
+    out["max_abs_hellmann_feynman_error"] = float(np.max(np.abs(errs)))
+    if cpu:
+        from oracle.mps import MPS as OMPS
+        from oracle.mps import inner_oplist_env
+
+        ops = {"z": np.diag([1.0, -1.0]), "x": np.array([[0.0, 1.0], [1.0, 0.0]])}
+        terms = [([ops[o] for o in oo], list(ss), 1.0) for oo, ss in zip(H.ops, H.sites)]
+        om = OMPS([np.asarray(t) for t in psi.tensors], psi.center)
+        t0 = time.perf_counter()
+        ref = inner_oplist_env(om, terms)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_port"] = {"seconds": cpu_s, "threads": os.cpu_count(),
+                           "max_abs_diff_vs_gpu": float(np.abs(ref - g).max()),
+                           "sample": "all 127 <O_t> from oracle/mps.py inner_oplist_env on the same tensors"}
+        out["speedup_vs_cpu_port"] = cpu_s / el
+    return out
 
 
 def qjmc_tfim_n40(ntraj=512, steps=200, sample_traj=16, sample_steps=20, nworkers=8, chi_max=256):
@@ -726,6 +781,9 @@ def main(argv=None):
     achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK["f32"], "unit": "TFLOP/s",
                 "frac": achieved / PEAK["f32"], "traffic": measured_traffic(chi),
+                "traffic_source": ("profiles/heff_matvec_chi2048_hbm_traffic.md: copy-calibrated rocprofv3 "
+                                   "FETCH_SIZE/WRITE_SIZE runs of this kernel, not measured in this run")
+                if chi == 2048 else None,
                 "kernel": "gemm_kernel<MODE_F32> (stage 1 L*theta and stage 3 T2*R^T)",
                 "flops_per_launch": gemm_flops, "avg_launch_ms": gemm_ms}
     mix = f32["mix"]

@@ -181,11 +181,11 @@ def inner_oplist_env(psi, oplist_terms, phi=None):
     N = len(psi)
     left = [np.ones((1, 1))]
     for s in range(1, N + 1):
-        left.append(np.einsum("ab,asc,bsd->cd", left[-1], np.conj(psi[s]), phi[s]))
+        left.append(np.einsum("ab,asc,bsd->cd", left[-1], np.conj(psi[s]), phi[s], optimize=True))
     right = [None] * (N + 2)
     right[N + 1] = np.ones((1, 1))
     for s in range(N, 0, -1):
-        right[s] = np.einsum("asc,bsd,cd->ab", np.conj(psi[s]), phi[s], right[s + 1])
+        right[s] = np.einsum("asc,bsd,cd->ab", np.conj(psi[s]), phi[s], right[s + 1], optimize=True)
     out = []
     for mats, sites, coeff in oplist_terms:
         env = left[sites[0] - 1]
@@ -193,6 +193,6 @@ def inner_oplist_env(psi, oplist_terms, phi=None):
             B = phi[s]
             if s in sites:
                 B = np.einsum("ts,asb->atb", mats[sites.index(s)], B)
-            env = np.einsum("ab,asc,bsd->cd", env, np.conj(psi[s]), B)
+            env = np.einsum("ab,asc,bsd->cd", env, np.conj(psi[s]), B, optimize=True)
         out.append(coeff * np.einsum("ab,ab->", env, right[sites[-1] + 1]))
     return np.array(out)

@@ -20,6 +20,7 @@
 #include <numeric>
 #include <vector>
 
+#include "dc_eig.hpp"
 #include "jacobi_lds.hpp"
 #include "tnh_internal.hpp"
 
@@ -816,6 +817,100 @@ void launch_tiny_eig(hipStream_t st, const double *G, int p, int64_t imG, double
     hipLaunchKernelGGL((tiny_eig_kernel<CPLX, 32, 1024>), dim3(1), dim3(1024), 0, st, G, p, imG, tol, V, imV, out);
 }
 
+// ---- small Hermitian eigenproblems by reduction and divide and conquer (dc_eig.hpp)
+
+// Execution policy of dc_eig on one workgroup of kDcThreads threads.
+struct DcDeviceEx {
+  template <class F> __device__ void par(int n, F &&f) const {
+    for (int t = threadIdx.x; t < n; t += dc::kDcThreads) f(t);
+  }
+  __device__ void sync() const { __syncthreads(); }
+  template <class F> __device__ double sum(int n, F &&f, double *red) const {
+    double s = 0;
+    for (int t = threadIdx.x; t < n; t += dc::kDcThreads) s += f(t);
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    double r = 0;
+    for (int w = 0; w < dc::kDcThreads / 64; ++w) r += red[w];
+    __syncthreads();
+    return r;
+  }
+};
+
+// Reduction and divide and conquer of the p x p Hermitian G (planar; p <= kDcPMax): the
+// eigenvalues (ascending, out[0..p)), the success flag out[p], the eigenvectors of the
+// tridiagonal matrix in Q, the Householder vectors in G and their factors in taus[0..2p).
+template <bool CPLX>
+__global__ __launch_bounds__(dc::kDcThreads) void dc_eig_kernel(double *G, int p, int64_t imG, double *taus,
+                                                                 double *Q, double *Q2, double *U, double *out) {
+  __shared__ dc::DcShared sh;
+  dc::dc_eig<CPLX>(DcDeviceEx{}, sh, G, CPLX ? G + imG : nullptr, p, taus, taus + p, Q, Q2, U, out);
+}
+
+// V = H_0 H_1 ... H_{p-2} Z: the tridiagonal eigenvectors Z taken back through the
+// Householder reflectors H_k = I - tau_k v_k v_k^H (v_k in column k of G below the diagonal).
+// One wave per 4 columns, 16 lanes per column; lane l keeps rows l, l + 16, ... of its column
+// in registers for the whole product, so the reflectors are the only memory traffic and the
+// column's dot product is a 16-lane shuffle reduction: no barriers.
+template <bool CPLX>
+__global__ __launch_bounds__(256) void dc_backtransform_kernel(const double *__restrict__ G, int p, int64_t imG,
+                                                               const double *__restrict__ taus,
+                                                               const double *__restrict__ Z, double *__restrict__ V,
+                                                               int64_t imV) {
+  constexpr int R = dc::kDcPMax / 16;
+  const int lane = threadIdx.x & 63, l = lane & 15;
+  const int c = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 4 + (lane >> 4);
+  const bool act = c < p;
+  const int64_t P = p;
+  double xr[R], xi[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = l + 16 * r;
+    xr[r] = (act && i < p) ? Z[i + c * P] : 0.0;
+    xi[r] = 0.0;
+  }
+  for (int k = p - 2; k >= 0; --k) {
+    const double tr = taus[k], ti = CPLX ? taus[p + k] : 0.0;
+    if (tr == 0.0 && ti == 0.0) continue;
+    const double *vr = G + k * P, *vi = CPLX ? G + imG + k * P : nullptr;
+    double sr = 0, si = 0;  // s = v^H x over rows k+1 ..
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int i = l + 16 * r;
+      if (i > k && i < p) {
+        const double a = vr[i], b = CPLX ? vi[i] : 0.0;
+        sr += a * xr[r] + b * xi[r];
+        if (CPLX) si += a * xi[r] - b * xr[r];
+      }
+    }
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) {
+      sr += __shfl_xor(sr, o, 64);
+      if (CPLX) si += __shfl_xor(si, o, 64);
+    }
+    const double fr = tr * sr - ti * si, fi = tr * si + ti * sr;  // tau s
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int i = l + 16 * r;
+      if (i > k && i < p) {
+        const double a = vr[i], b = CPLX ? vi[i] : 0.0;
+        xr[r] -= a * fr - b * fi;
+        if (CPLX) xi[r] -= a * fi + b * fr;
+      }
+    }
+  }
+  if (!act) return;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = l + 16 * r;
+    if (i < p) {
+      V[i + c * P] = xr[r];
+      if (CPLX) V[imV + i + c * P] = xi[r];
+    }
+  }
+}
+
 struct Schedule {
   int nb = 0, nbp = 0, rounds = 0, npairs = 0;
   int *dev = nullptr;  // [round][pair][2]
@@ -1081,8 +1176,8 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
   // relative off-diagonal tolerance 4 sqrt(p) eps (at least 1e-14), the scale of the
   // rounding that p-term inner products leave
   const double tol = std::max(1e-14, 4.0 * 2.220446049250313e-16 * std::sqrt(double(p)));
-  if (!force_fallback) {
-    // G = M M^H (side 0) or M^H M (side 1)
+  // G = M M^H (side 0) or M^H M (side 1)
+  auto form_gram = [&]() {
     Gemm g;
     g.M = p;
     g.N = p;
@@ -1102,7 +1197,46 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
       A.conj = true;
     }
     cgemm(ctx, g, A, B, G, cplx ? G + pp : nullptr, TNH_F64, TNH_PREC_F64);
-    if (p <= 2 * JP) {
+  };
+  auto take_eigenvalues = [&](const double *dout) {
+    std::vector<double> h(p + 1);
+    to_host(ctx, h.data(), dout, sizeof(double) * (p + 1));
+    if (h[p] != 1.0) return false;
+    for (int i = 0; i < p; ++i) {
+      if (!std::isfinite(h[i])) return false;
+      val[i] = std::sqrt(std::max(h[i], 0.0));
+    }
+    return true;
+  };
+  if (!force_fallback) {
+    form_gram();
+    if (p > 8 && p <= dc::kDcPMax) {
+      // reduction to tridiagonal form and divide and conquer in one workgroup, then the
+      // back-transformation of the eigenvectors through the Householder reflectors
+      double *taus = static_cast<double *>(ctx->workspace(624, sizeof(double) * 2 * p));
+      double *Qd = static_cast<double *>(ctx->workspace(625, sizeof(double) * pp));
+      double *Q2d = static_cast<double *>(ctx->workspace(626, sizeof(double) * pp));
+      double *Ud = static_cast<double *>(ctx->workspace(627, sizeof(double) * pp));
+      double *dout = static_cast<double *>(ctx->workspace(616, sizeof(double) * (p + 1)));
+      hipEvent_t pd = ctx->prof_begin();
+      const unsigned nbt = unsigned(((p + 3) / 4 + 3) / 4);
+      if (cplx) {
+        hipLaunchKernelGGL(dc_eig_kernel<true>, dim3(1), dim3(dc::kDcThreads), 0, ctx->stream, G, p, pp, taus,
+                           Qd, Q2d, Ud, dout);
+        hipLaunchKernelGGL(dc_backtransform_kernel<true>, dim3(nbt), dim3(256), 0, ctx->stream, G, p, pp, taus,
+                           Qd, V, pp);
+      } else {
+        hipLaunchKernelGGL(dc_eig_kernel<false>, dim3(1), dim3(dc::kDcThreads), 0, ctx->stream, G, p, pp, taus,
+                           Qd, Q2d, Ud, dout);
+        hipLaunchKernelGGL(dc_backtransform_kernel<false>, dim3(nbt), dim3(256), 0, ctx->stream, G, p, pp, taus,
+                           Qd, V, pp);
+      }
+      TNH_HIP(hipGetLastError());
+      ctx->prof_end("dc_eig", pd, 0.0, double(p));
+      ok = take_eigenvalues(dout);
+      if (!ok) form_gram();  // the reduction overwrote G: the Jacobi paths below start afresh
+    }
+    if (!ok && p <= 2 * JP) {
       // the whole problem fits one workgroup: a single launch and a single download
       double *dout = static_cast<double *>(ctx->workspace(616, sizeof(double) * (p + 1)));
       if (cplx)
@@ -1110,15 +1244,8 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
       else
         launch_tiny_eig<false>(ctx->stream, G, p, pp, tol, V, pp, dout);
       TNH_HIP(hipGetLastError());
-      std::vector<double> h(p + 1);
-      to_host(ctx, h.data(), dout, sizeof(double) * (p + 1));
-      if (h[p] == 1.0) {
-        ok = true;
-        for (int i = 0; i < p; ++i) {
-          if (!std::isfinite(h[i])) ok = false;
-          val[i] = std::sqrt(std::max(h[i], 0.0));
-        }
-      }
+      ok = take_eigenvalues(dout);
+      if (!ok) form_gram();
     }
   }
   if (!force_fallback && !ok && p > JP) {
@@ -1190,11 +1317,6 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
     hipEvent_t pj = ctx->prof_begin();
     JacobiResult jr = jacobi(ctx, true, G, p, p, cplx, V, tol, abstol, 40);
     ctx->prof_end(use_warm ? "jacobi_warm" : "jacobi", pj, double(jr.sweeps), double(p));
-    if (jr.converged && warm) {  // the eigenvectors become the next split's warm start
-      const int64_t wd[2] = {p, p};
-      buf_ensure(ctx, warm, dtp, 2, wd);
-      copy_device(ctx, (*warm)->ptr, V, sizeof(double) * pp * planes);
-    }
     if (jr.converged) {
       double *dv = static_cast<double *>(ctx->workspace(612, sizeof(double) * p));
       hipLaunchKernelGGL(diag_kernel, dim3((p + 255) / 256), dim3(256), 0, ctx->stream, G, p, dv);
@@ -1205,6 +1327,22 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
         if (!std::isfinite(v)) ok = false;
         v = std::sqrt(std::max(std::ldexp(v, ex), 0.0));
       }
+    }
+    if (ok && warm) {
+      // The eigenvectors become the next split's warm start, ordered by descending
+      // eigenvalue: the next Q^H G Q then holds its large eigenvalues in the leading column
+      // blocks, so pairs of two trailing (noise-level) blocks are below the floor from the
+      // first sweep on and their workgroups skip all work.
+      std::vector<int> order(p);
+      std::iota(order.begin(), order.end(), 0);
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return val[a] > val[b]; });
+      int *dord = static_cast<int *>(ctx->workspace(623, sizeof(int) * p));
+      to_device(ctx, dord, order.data(), sizeof(int) * p);
+      const int64_t wd[2] = {p, p};
+      buf_ensure(ctx, warm, dtp, 2, wd);
+      hipLaunchKernelGGL(gather_cols_kernel, dim3(unsigned((pp + 255) / 256)), dim3(256), 0, ctx->stream, V, p,
+                         pp, dord, nullptr, p, static_cast<double *>((*warm)->ptr), pp, cplx);
+      TNH_HIP(hipGetLastError());
     }
   }
   double *W = nullptr;  // the matrix whose columns are (unnormalised) singular directions
