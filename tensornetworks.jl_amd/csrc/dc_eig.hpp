@@ -11,50 +11,70 @@
 // single latency-bound workgroup wants.
 //
 // The algorithm is written once against an execution policy `Ex` that supplies `par(n, f)`
-// (f(t) for t in [0, n), spread over the workgroup), `sync()` (a workgroup barrier) and
-// `sum(n, f, scratch)` (a workgroup-wide sum returned to every thread). The device policy
-// (in the .hip file) maps these to threads and barriers; a host policy that runs the loops
-// sequentially compiles the same code for tests on the CPU.
+// (f(t) for t in [0, n), spread over the workgroup), `sync()` (a workgroup barrier),
+// `sum(n, f, scratch)` / `vmax(n, f, scratch)` (a workgroup-wide sum or maximum returned to
+// every thread, one barrier: consecutive calls must alternate their scratch buffers),
+// `gsize(p)` and `gsum(x, G)` (groups of G consecutive threads that share a secular root and
+// sum over the group), and `mark(id)`, a phase boundary that a timing policy can record. The
+// device policy (in the .hip file) maps these to threads, barriers and lane shuffles; a host
+// policy that runs the loops sequentially (G = 1) compiles the same code for tests on the
+// CPU. The merge products are the one place with a device-only path (matrix-core tiles);
+// the host runs the same sums as plain loops.
 #pragma once
 
 #include <cmath>
 #include <cstdint>
 
 #ifdef __HIPCC__
+#include <hip/hip_runtime.h>
 #define TNH_HD __host__ __device__
+#define TNH_INLINE __attribute__((always_inline))
 #else
 #define TNH_HD
+#define TNH_INLINE
 #endif
 
 namespace tnh {
 namespace dc {
 
-constexpr int kDcPMax = 512;   // largest matrix side
+constexpr int kDcPMax = 512;      // largest matrix side
 constexpr int kDcThreads = 1024;  // workgroup size the partial-sum layout is sized for
+constexpr int kDcLdsMat = 13800;  // doubles of LDS for the active matrix of the reduction
 constexpr double kEps = 2.220446049250313e-16;
 
-// Workgroup-shared state (LDS on the device). Per merge m (start b, split bm, end b2) the
-// per-position arrays hold the merge's entries at [b, b2).
+// Workgroup-shared state (LDS on the device). The reduction and the divide and conquer use
+// disjoint arrays, which share storage. Per merge m (start b, split bm, end b2) the
+// per-position arrays of the divide and conquer hold the merge's entries at [b, b2).
 struct DcShared {
-  double d[kDcPMax], e[kDcPMax], lam[kDcPMax];
-  double zt[kDcPMax], Dt[kDcPMax];  // coupling vector and diagonal, in column order
-  double Ds[kDcPMax], zs[kDcPMax];  // the same, sorted ascending within each merge
-  double dk[kDcPMax], zk[kDcPMax];  // the kept (not deflated) entries, compacted
-  double zh[kDcPMax], tau[kDcPMax];  // recomputed coupling vector; root offsets
-  double rc[kDcPMax], rs[kDcPMax];  // Givens rotations of the deflation, per merge
-  double vr[kDcPMax], vi[kDcPMax], yr[kDcPMax], yi[kDcPMax];  // Householder step vectors
-  double part[2 * kDcThreads];  // matrix-vector partial sums (real, imaginary)
-  double red[64];
-  double mrho[kDcPMax / 2];
-  int src[kDcPMax];   // sorted position -> column it came from
-  int kpos[kDcPMax];  // kept compact index -> sorted position (relative to the merge start)
-  int dpos[kDcPMax];  // deflated compact index -> sorted position
-  int org[kDcPMax];   // root -> compact index of the pole its offset is measured from
-  int fsrc[kDcPMax];  // final position -> item (root q < K, or K + deflated index)
-  int rj[kDcPMax], ri[kDcPMax];
-  int kflag[kDcPMax];  // sorted position kept (1) or deflated (0)
-  int mK[kDcPMax / 2], mrot[kDcPMax / 2];
+  double d[kDcPMax], e[kDcPMax];  // the tridiagonal matrix
+  double red[2][64];
   int status;
+  union {
+    struct {
+      double vr[kDcPMax], vi[kDcPMax], yr[kDcPMax], yi[kDcPMax];  // Householder step vectors
+      double part[2 * kDcThreads];  // matrix-vector partial sums (real, imaginary)
+      double tr[kDcPMax], ti[kDcPMax];  // reflector factors of the LDS-resident steps
+      double A[kDcLdsMat];              // the active matrix once it fits
+    } h;
+    struct {
+      double lam[kDcPMax];
+      double zt[kDcPMax], Dt[kDcPMax];  // coupling vector and diagonal, in column order
+      double Ds[kDcPMax], zs[kDcPMax];  // the same, sorted ascending within each merge
+      double dk[kDcPMax], zk[kDcPMax];  // the kept (not deflated) entries, compacted
+      double zh[kDcPMax], tau[kDcPMax];  // recomputed coupling vector; root offsets
+      double rc[kDcPMax], rs[kDcPMax];  // Givens rotations of the deflation, per merge
+      double cn[kDcPMax];               // reciprocal norms of the secular eigenvectors
+      double mrho[kDcPMax / 2];
+      int src[kDcPMax];   // sorted position -> column it came from
+      int kpos[kDcPMax];  // kept compact index -> sorted position (relative to the merge start)
+      int dpos[kDcPMax];  // deflated compact index -> sorted position
+      int org[kDcPMax];   // root -> compact index of the pole its offset is measured from
+      int finv[kDcPMax];  // item (root q < K, or K + deflated index) -> final position
+      int rj[kDcPMax], ri[kDcPMax];
+      int kflag[kDcPMax];  // sorted position kept (1) or deflated (0)
+      int mK[kDcPMax / 2], mrot[kDcPMax / 2];
+    } c;
+  } u;
 };
 
 TNH_HD inline bool finite(double v) { return fabs(v) <= 1.7976931348623157e308; }
@@ -81,15 +101,19 @@ TNH_HD inline MergeAt merge_of(int j, int s, int p) {
 // lambda as dl[*org] + tau with org the nearer pole, so that lambda - dl[i] is formed as
 // (dl[org] - dl[i]) + tau without cancellation. Each iteration fits the two poles around the
 // root (psi, the terms up to j, and phi, the rest, each matched in value and slope) and takes
-// the root of that model; a step leaving the current bracket is replaced by bisection.
-TNH_HD inline double secular_root(const double *dl, const double *zk, double rho, int j, int K, int *org,
-                                  int *iters) {
+// the root of that model; a step leaving the current bracket is replaced by bisection. The G
+// threads of a group (lane gl) each sum every G-th term and combine with ex.gsum; the sums
+// come out bitwise equal in every lane, so the group iterates in lockstep.
+template <class Ex>
+TNH_HD double secular_root(const Ex &ex, const double *dl, const double *zk, double rho, int j, int K, int gl,
+                           int G, int *org, int *iters) {
   double lo, hi;
   int o;
   if (j < K - 1) {
     const double gap = dl[j + 1] - dl[j], mid = 0.5 * gap;
-    double f = 1.0;
-    for (int k = 0; k < K; ++k) f += rho * zk[k] * zk[k] / ((dl[k] - dl[j]) - mid);
+    double f = 0.0;
+    for (int k = gl; k < K; k += G) f += rho * zk[k] * zk[k] / ((dl[k] - dl[j]) - mid);
+    f = 1.0 + ex.gsum(f, G);
     if (f >= 0) {  // root in the left half: measured from dl[j]
       o = j;
       lo = 0.0;
@@ -110,17 +134,21 @@ TNH_HD inline double secular_root(const double *dl, const double *zk, double rho
   int it = 0;
   for (; it < 96; ++it) {
     double psi = 0, dpsi = 0, phi = 0, dphi = 0;
-    for (int k = 0; k < K; ++k) {
-      const double df = (dl[k] - dor) - t;
-      const double q = rho * zk[k] * zk[k] / df;
+    for (int k = gl; k < K; k += G) {
+      const double r = 1.0 / ((dl[k] - dor) - t);
+      const double q = rho * zk[k] * zk[k] * r;
       if (k <= j) {
         psi += q;
-        dpsi += q / df;
+        dpsi += q * r;
       } else {
         phi += q;
-        dphi += q / df;
+        dphi += q * r;
       }
     }
+    psi = ex.gsum(psi, G);
+    dpsi = ex.gsum(dpsi, G);
+    phi = ex.gsum(phi, G);
+    dphi = ex.gsum(dphi, G);
     const double f = 1.0 + psi + phi;
     // f is zero to within the rounding of its own evaluation: t is a root
     if (fabs(f) <= 8.0 * kEps * (1.0 + fabs(psi) + fabs(phi))) break;
@@ -164,6 +192,96 @@ TNH_HD inline double secular_root(const double *dl, const double *zk, double rho
   return t;
 }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef double dc_v4d __attribute__((ext_vector_type(4)));
+#endif
+
+// The merged blocks' eigenvectors for the K kept items of every merge at level s:
+// Q[(b + r) + finv[b + x] p] = sum_{q < K} Q2[(b + r) + (b + kpos[b + q]) p] U[(b + q) + (b + x) p]
+// for r < n (the merge size), x < K. On the device each wave computes 32 x 32 tiles with
+// v_mfma_f64_16x16x4f64 (lane (g = lane / 16, rr = lane % 16) supplies A[rr][k_g] and
+// B[k_g][rr]; accumulator register i holds C[g + 4 i][rr]), operands straight from memory;
+// tiles past the merge size or past K are skipped whole.
+TNH_HD inline void merge_product(DcShared &sh, int s, int p, const double *Q2, const double *U, double *Q) {
+  const int64_t P = p;
+  const int nm = (p + 2 * s - 1) / (2 * s);
+  auto &c = sh.u.c;
+#if defined(__HIP_DEVICE_COMPILE__)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, rr = lane & 15;
+  const int tpd = (2 * s + 31) / 32, tpm = tpd * tpd;
+  for (int tile = wave; tile < nm * tpm; tile += kDcThreads / 64) {
+    const int m = tile / tpm, loc = tile % tpm;
+    const int b = 2 * s * m, bm = b + s < p ? b + s : p, b2 = b + 2 * s < p ? b + 2 * s : p;
+    if (bm >= b2) continue;
+    const int n = b2 - b, K = c.mK[m];
+    const int row0 = 32 * (loc % tpd), col0 = 32 * (loc / tpd);
+    if (row0 >= n || col0 >= K) continue;
+    dc_v4d acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y) acc[x][y] = dc_v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+    for (int q0 = 0; q0 < K; q0 += 4) {
+      const int q = q0 + g;
+      const bool qok = q < K;
+      const int64_t colA = qok ? (b + c.kpos[b + q]) * P : 0;
+      double fa[2], fb[2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        const int r = row0 + 16 * x + rr;
+        fa[x] = (qok && r < n) ? Q2[(b + r) + colA] : 0.0;
+      }
+#pragma unroll
+      for (int y = 0; y < 2; ++y) {
+        const int cc = col0 + 16 * y + rr;
+        fb[y] = (qok && cc < K) ? U[(b + q) + (b + cc) * P] : 0.0;
+      }
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[x], fb[y], acc[x][y], 0, 0, 0);
+    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = row0 + 16 * x + g + 4 * i, cc = col0 + 16 * y + rr;
+          if (r < n && cc < K) {
+            const int fp = c.finv[b + cc];
+            const double v = acc[x][y][i];
+            if (!finite(v) || fp < b || fp >= b2) {
+              sh.status = 0;
+            } else {
+              Q[(b + r) + fp * P] = v;
+            }
+          }
+        }
+  }
+#else
+  for (int m = 0; m < nm; ++m) {
+    const int b = 2 * s * m, bm = b + s < p ? b + s : p, b2 = b + 2 * s < p ? b + 2 * s : p;
+    if (bm >= b2) continue;
+    const int n = b2 - b, K = c.mK[m];
+    for (int x = 0; x < K; ++x) {
+      const int fp = c.finv[b + x];
+      if (fp < b || fp >= b2) {  // not reachable: the final rank is a permutation
+        sh.status = 0;
+        continue;
+      }
+      for (int r = 0; r < n; ++r) {
+        double v = 0.0;
+        for (int q = 0; q < K; ++q) v += Q2[(b + r) + (b + c.kpos[b + q]) * P] * U[(b + q) + (b + x) * P];
+        if (!finite(v)) sh.status = 0;
+        Q[(b + r) + fp * P] = v;
+      }
+    }
+  }
+#endif
+}
+
 // The reduction and divide-and-conquer on the device-resident p x p Hermitian matrix
 // (Gr, Gi planar, leading dimension p; Gi unused when !CPLX). On return: the Householder
 // vectors in the columns of G below the subdiagonal (v_k[0] = 1 stored), their factors in
@@ -174,70 +292,84 @@ template <bool CPLX, class Ex>
 TNH_HD void dc_eig(const Ex &ex, DcShared &sh, double *Gr, double *Gi, int p, double *taur, double *taui,
                    double *Q, double *Q2, double *U, double *out) {
   const int64_t P = p;
+  int rcount = 0;  // alternates the reduction scratch buffers
+  auto red = [&]() { return sh.red[(rcount++) & 1]; };
+  ex.mark(0);
   // exact Hermitian symmetrisation and a power-of-two scaling to max |G_ii| in [1, 2)
   ex.par(p * p, [&](int e) {
     const int i = e % p, j = e / p;
     if (i < j) {
-      const double ar = 0.5 * (Gr[i + j * P] + Gr[j + i * P]);
-      Gr[i + j * P] = ar;
-      Gr[j + i * P] = ar;
+      const double a = 0.5 * (Gr[i + j * P] + Gr[j + i * P]);
+      Gr[i + j * P] = a;
+      Gr[j + i * P] = a;
       if (CPLX) {
-        const double ai = 0.5 * (Gi[i + j * P] - Gi[j + i * P]);
-        Gi[i + j * P] = ai;
-        Gi[j + i * P] = -ai;
+        const double b = 0.5 * (Gi[i + j * P] - Gi[j + i * P]);
+        Gi[i + j * P] = b;
+        Gi[j + i * P] = -b;
       }
     } else if (i == j && CPLX) {
       Gi[i + i * P] = 0.0;
     }
   });
   ex.sync();
-  double dm = 0;
-  for (int i = 0; i < p; ++i) dm = fmax(dm, fabs(Gr[i + i * P]));
+  const double dm = ex.vmax(p, [&](int i) { return fabs(Gr[i + i * P]); }, red());
   const int ex2 = (dm > 0 && dm < 1e300) ? ilogb(dm) : 0;
   if (ex2 != 0) {
     const double sc = ldexp(1.0, -ex2);
-    ex.sync();  // every thread has read the diagonal
     ex.par(p * p, [&](int e) {
       Gr[e] *= sc;
       if (CPLX) Gi[e] *= sc;
     });
     ex.sync();
   }
-  // ---- Householder reduction: Q_h^H G Q_h = T, real off-diagonal (beta_k). Five
-  // barriers per step: the matrix-vector partial sums; y = B v with v^H y; the rank-2 update,
-  // which also sums the squared tail of the next step's column.
+  ex.mark(1);
+  // ---- Householder reduction: Q_h^H G Q_h = T, real off-diagonal (beta_k). Three barriers
+  // per step: the matrix-vector partial sums; y = B v with v^H y; the rank-2 update, which
+  // also sums the squared tail of the next step's column. The active matrix A_k = G[k.., k..]
+  // (element (i, j) at ar[i + j * lda]) is read from G until it fits in LDS, then copied
+  // there; from then on the reflectors and their factors also stay in LDS (v_k in the dead
+  // column 0 of A_k) until the end, so no barrier of those steps waits for a global store.
+  // The two phases are separate instances of the same step, each with pointers of one
+  // address space.
   double xn2 = ex.sum(p > 2 ? p - 2 : 0, [&](int i) {
     const double a = Gr[2 + i], b = CPLX ? Gi[2 + i] : 0.0;
     return a * a + b * b;
-  }, sh.red);
-  for (int k = 0; k + 1 < p; ++k) {
+  }, red());
+  // one step on A_k at (ar, ai) with leading dimension lda; v_k goes to (vcr, vci) (n entries)
+  // and (tau_k) to (*tpr, *tpi)
+  auto step = [&](int k, double *ar, double *ai, int lda, double *vcr, double *vci, double *tpr,
+                  double *tpi) TNH_INLINE {
     const int n = p - k - 1;
-    double *col_r = Gr + k * P + (k + 1), *col_i = CPLX ? Gi + k * P + (k + 1) : nullptr;
-    const double ar = col_r[0], ai = CPLX ? col_i[0] : 0.0;
-    const double dkk = Gr[k + k * P];
-    double tr = 0, ti = 0, beta = ar, sr = 0, si = 0;
-    if (xn2 != 0.0 || ai != 0.0) {
-      beta = -copysign(sqrt(ar * ar + ai * ai + xn2), ar);
-      tr = (beta - ar) / beta;
-      ti = -ai / beta;
+    const double *col_r = ar + 1, *col_i = CPLX ? ai + 1 : nullptr;  // column 0 of A_k, rows 1..
+    const double alr = col_r[0], ali = CPLX ? col_i[0] : 0.0;
+    const double dkk = ar[0];
+    double tr = 0, ti = 0, beta = alr, sr = 0, si = 0;
+    if (xn2 != 0.0 || ali != 0.0) {
+      beta = -copysign(sqrt(alr * alr + ali * ali + xn2), alr);
+      tr = (beta - alr) / beta;
+      ti = -ali / beta;
       // v = x / (alpha - beta), v_0 = 1
-      const double ur = ar - beta, ui = ai, den = ur * ur + ui * ui;
+      const double ur = alr - beta, ui = ali, den = ur * ur + ui * ui;
       sr = ur / den;
       si = -ui / den;
     }
     ex.par(1, [&](int) {
       sh.d[k] = dkk;
       sh.e[k] = beta;
-      taur[k] = tr;
-      taui[k] = ti;
+      *tpr = tr;
+      *tpi = ti;
     });
-    double *Br = Gr + (k + 1) * P + (k + 1), *Bi = CPLX ? Gi + (k + 1) * P + (k + 1) : nullptr;
-    if (tr == 0.0 && ti == 0.0) {  // nothing to reflect: the next column's tail norm
+    double *Br = ar + 1 + lda, *Bi = CPLX ? ai + 1 + lda : nullptr;
+    if (tr == 0.0 && ti == 0.0) {  // nothing to reflect (x is zero below row 0): v = e_0
+      ex.par(n, [&](int i) {
+        vcr[i] = i == 0 ? 1.0 : 0.0;
+        if (CPLX) vci[i] = 0.0;
+      });
       xn2 = ex.sum(n > 2 ? n - 2 : 0, [&](int i) {
         const double a = Br[2 + i], b = CPLX ? Bi[2 + i] : 0.0;
         return a * a + b * b;
-      }, sh.red);
-      continue;
+      }, red());
+      return;
     }
     auto vget = [&](int j, double &a, double &b) {
       if (j == 0) {
@@ -249,104 +381,152 @@ TNH_HD void dc_eig(const Ex &ex, DcShared &sh, double *Gr, double *Gi, int p, do
         b = xr * si + xi * sr;
       }
     };
-    // y = B v, B = G[k+1.., k+1..]: nc column chunks per row, partial sums in LDS; v is
-    // formed on the fly from the column (still x here)
-    const int nc = kDcThreads / n > 0 ? kDcThreads / n : 1;
+    // y = B v: nc (<= 16) column chunks per row, partial sums in LDS; v formed on the fly
+    int nc = kDcThreads / n > 0 ? kDcThreads / n : 1;
+    if (nc > 16) nc = 16;
     ex.par(n * nc, [&](int e) {
-      const int i = e % n, c = e / n;
+      const int i = e % n, cc = e / n;
       double sr2 = 0, si2 = 0;
-      for (int j = c; j < n; j += nc) {
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll 4
+#endif
+      for (int j = cc; j < n; j += nc) {
         double vrj, vij;
         vget(j, vrj, vij);
-        const double br = Br[i + j * P];
+        const double br = Br[i + j * lda];
         if (CPLX) {
-          const double bi = Bi[i + j * P];
+          const double bi = Bi[i + j * lda];
           sr2 += br * vrj - bi * vij;
           si2 += br * vij + bi * vrj;
         } else {
           sr2 += br * vrj;
         }
       }
-      sh.part[e] = sr2;
-      if (CPLX) sh.part[kDcThreads + e] = si2;
+      sh.u.h.part[e] = sr2;
+      if (CPLX) sh.u.h.part[kDcThreads + e] = si2;
     });
     ex.sync();
-    // y from the partials, v into LDS and into the column (for the back-transformation),
-    // and v^H y (real for Hermitian B)
+    // y from the partials, v into LDS (and into its slot), and v^H y (real for Hermitian B)
     const double vy = ex.sum(n, [&](int i) {
       double a = 0, b = 0;
-      for (int c = 0; c < nc; ++c) {
-        a += sh.part[i + c * n];
-        if (CPLX) b += sh.part[kDcThreads + i + c * n];
+      for (int cc = 0; cc < nc; ++cc) {
+        a += sh.u.h.part[i + cc * n];
+        if (CPLX) b += sh.u.h.part[kDcThreads + i + cc * n];
       }
-      sh.yr[i] = a;
-      sh.yi[i] = b;
+      sh.u.h.yr[i] = a;
+      sh.u.h.yi[i] = b;
       double va, vb;
       vget(i, va, vb);
-      sh.vr[i] = va;
-      sh.vi[i] = vb;
-      col_r[i] = va;
-      if (CPLX) col_i[i] = vb;
+      sh.u.h.vr[i] = va;
+      sh.u.h.vi[i] = vb;
+      vcr[i] = va;
+      if (CPLX) vci[i] = vb;
       return va * a + vb * b;
-    }, sh.red);
+    }, red());
     const double h = 0.5 * (tr * tr + ti * ti) * vy;
     // B -= w v^H + v w^H with w = tau y - h v; the updated column 0 of B is the next
     // step's x, whose tail (rows 2..) is summed here
     xn2 = ex.sum(n * n, [&](int e) {
       const int i = e % n, j = e / n;
-      const double wir = tr * sh.yr[i] - ti * sh.yi[i] - h * sh.vr[i];
-      const double wjr = tr * sh.yr[j] - ti * sh.yi[j] - h * sh.vr[j];
+      const double *vr = sh.u.h.vr, *vi = sh.u.h.vi, *yr = sh.u.h.yr, *yi = sh.u.h.yi;
+      const double wir = tr * yr[i] - ti * yi[i] - h * vr[i];
+      const double wjr = tr * yr[j] - ti * yi[j] - h * vr[j];
       double nr, ni = 0.0;
       if (CPLX) {
-        const double wii = tr * sh.yi[i] + ti * sh.yr[i] - h * sh.vi[i];
-        const double wji = tr * sh.yi[j] + ti * sh.yr[j] - h * sh.vi[j];
+        const double wii = tr * yi[i] + ti * yr[i] - h * vi[i];
+        const double wji = tr * yi[j] + ti * yr[j] - h * vi[j];
         // w_i conj(v_j) + v_i conj(w_j)
-        const double re = wir * sh.vr[j] + wii * sh.vi[j] + sh.vr[i] * wjr + sh.vi[i] * wji;
-        const double im = wii * sh.vr[j] - wir * sh.vi[j] + sh.vi[i] * wjr - sh.vr[i] * wji;
-        nr = Br[i + j * P] - re;
-        ni = Bi[i + j * P] - im;
-        Br[i + j * P] = nr;
-        Bi[i + j * P] = ni;
+        const double re = wir * vr[j] + wii * vi[j] + vr[i] * wjr + vi[i] * wji;
+        const double im = wii * vr[j] - wir * vi[j] + vi[i] * wjr - vr[i] * wji;
+        nr = Br[i + j * lda] - re;
+        ni = Bi[i + j * lda] - im;
+        Br[i + j * lda] = nr;
+        Bi[i + j * lda] = ni;
       } else {
-        nr = Br[i + j * P] - (wir * sh.vr[j] + sh.vr[i] * wjr);
-        Br[i + j * P] = nr;
+        nr = Br[i + j * lda] - (wir * vr[j] + vr[i] * wjr);
+        Br[i + j * lda] = nr;
       }
       return (j == 0 && i >= 2) ? nr * nr + ni * ni : 0.0;
-    }, sh.red);
+    }, red());
+  };
+  const int planes = CPLX ? 2 : 1;
+  int k = 0;
+  for (; k + 1 < p; ++k) {  // G-resident steps; v_k into G's column, tau_k into taur/taui
+    const int n = p - k - 1;
+    if ((n + 1) * (n + 1) * planes <= kDcLdsMat) break;
+    step(k, Gr + k * (P + 1), CPLX ? Gi + k * (P + 1) : nullptr, p, Gr + k * P + (k + 1),
+         CPLX ? Gi + k * P + (k + 1) : nullptr, taur + k, taui + k);
   }
+  double dlast;
+  if (k + 1 < p) {
+    const int k0 = k, m1 = p - k0;
+    double *Lr = sh.u.h.A, *Li = CPLX ? sh.u.h.A + m1 * m1 : nullptr;
+    ex.par(m1 * m1, [&](int e) {
+      const int i = e % m1, j = e / m1;
+      Lr[e] = Gr[(k0 + i) + (k0 + j) * P];
+      if (CPLX) Li[e] = Gi[(k0 + i) + (k0 + j) * P];
+    });
+    ex.sync();
+    for (; k + 1 < p; ++k) {  // LDS-resident steps; v_k into A_k's column 0 (rows 1..)
+      const int o = (k - k0) * (m1 + 1);
+      step(k, Lr + o, CPLX ? Li + o : nullptr, m1, Lr + o + 1, CPLX ? Li + o + 1 : nullptr, sh.u.h.tr + k,
+           sh.u.h.ti + k);
+    }
+    dlast = Lr[(m1 - 1) * (m1 + 1)];
+    ex.sync();
+    // the LDS reflectors and factors to G's columns and taur/taui
+    ex.par(m1 * m1, [&](int e) {
+      const int i = e % m1, j = e / m1;  // v_{k0 + j}, entry i (rows k0 + j + 1 + i)
+      if (j + 1 >= m1 || i >= m1 - 1 - j) return;
+      const int o = j * (m1 + 1) + 1 + i;
+      Gr[(k0 + j + 1 + i) + (k0 + j) * P] = Lr[o];
+      if (CPLX) Gi[(k0 + j + 1 + i) + (k0 + j) * P] = Li[o];
+    });
+    ex.par(m1 - 1, [&](int j) {
+      taur[k0 + j] = sh.u.h.tr[k0 + j];
+      taui[k0 + j] = sh.u.h.ti[k0 + j];
+    });
+  } else {
+    dlast = Gr[(p - 1) * (P + 1)];
+  }
+  ex.mark(2);
   ex.par(1, [&](int) {
-    sh.d[p - 1] = Gr[(p - 1) * (P + 1)];
+    sh.d[p - 1] = dlast;
     sh.status = 1;
   });
   ex.sync();
   // a non-finite input leaves a non-finite tridiagonal matrix: report it and stop
   const double bad = ex.sum(p, [&](int i) {
     return (finite(sh.d[i]) && (i + 1 >= p || finite(sh.e[i]))) ? 0.0 : 1.0;
-  }, sh.red);
+  }, red());
   if (bad != 0.0) {
     ex.par(p + 1, [&](int i) { out[i] = i < p ? NAN : 0.0; });
     return;
   }
   // ---- divide and conquer on T = tridiag(e, d, e): leaves of size 1 with every
   // off-diagonal torn (d_i -= |e_{i-1}| + |e_i|), merged pairwise level by level
+  auto &c = sh.u.c;
   ex.par(p, [&](int i) {
     double v = sh.d[i];
     if (i > 0) v -= fabs(sh.e[i - 1]);
     if (i + 1 < p) v -= fabs(sh.e[i]);
-    sh.lam[i] = v;
+    c.lam[i] = v;
   });
   ex.par(p * p, [&](int e) { Q[e] = (e % p == e / p) ? 1.0 : 0.0; });
   ex.sync();
-  for (int s = 1; s < p; s *= 2) {
+  const int G = ex.gsize(p);
+  int level = 0;
+  for (int s = 1; s < p; s *= 2, ++level) {
     const int nm = (p + 2 * s - 1) / (2 * s);
+    ex.mark(8 + 4 * level);
     // coupling vector z = Q^T u, u = e_{bm-1} + sign(e) e_{bm}: the last row of the left
     // block's eigenvectors and the first row of the right block's
     ex.par(p, [&](int j) {
       const MergeAt g = merge_of(j, s, p);
       if (g.pass()) return;
       const double sg = sh.e[g.bm - 1] >= 0 ? 1.0 : -1.0;
-      sh.zt[j] = j < g.bm ? Q[(g.bm - 1) + j * P] : sg * Q[g.bm + j * P];
-      sh.Dt[j] = sh.lam[j];
+      c.zt[j] = j < g.bm ? Q[(g.bm - 1) + j * P] : sg * Q[g.bm + j * P];
+      c.Dt[j] = c.lam[j];
     });
     ex.sync();
     // merge the two ascending halves (stable: the left half first among equals)
@@ -354,19 +534,19 @@ TNH_HD void dc_eig(const Ex &ex, DcShared &sh, double *Gr, double *Gi, int p, do
       const MergeAt g = merge_of(j, s, p);
       if (g.pass()) return;
       int rank;
-      const double kj = order_key(sh.Dt[j]);
+      const double kj = order_key(c.Dt[j]);
       if (j < g.bm) {
-        int c = 0;
-        for (int t = g.bm; t < g.b2; ++t) c += order_key(sh.Dt[t]) < kj;
-        rank = (j - g.b) + c;
+        int cnt = 0;
+        for (int t = g.bm; t < g.b2; ++t) cnt += order_key(c.Dt[t]) < kj;
+        rank = (j - g.b) + cnt;
       } else {
-        int c = 0;
-        for (int t = g.b; t < g.bm; ++t) c += order_key(sh.Dt[t]) <= kj;
-        rank = (j - g.bm) + c;
+        int cnt = 0;
+        for (int t = g.b; t < g.bm; ++t) cnt += order_key(c.Dt[t]) <= kj;
+        rank = (j - g.bm) + cnt;
       }
-      sh.Ds[g.b + rank] = sh.Dt[j];
-      sh.zs[g.b + rank] = sh.zt[j];
-      sh.src[g.b + rank] = j;
+      c.Ds[g.b + rank] = c.Dt[j];
+      c.zs[g.b + rank] = c.zt[j];
+      c.src[g.b + rank] = j;
     });
     ex.sync();
     // deflation, one thread per merge: normalise z (rho absorbs |z|^2); a component with
@@ -378,65 +558,65 @@ TNH_HD void dc_eig(const Ex &ex, DcShared &sh, double *Gr, double *Gi, int p, do
       const int n = b2 - b;
       double zn2 = 0, dmax = 0;
       for (int t = 0; t < n; ++t) {
-        zn2 += sh.zs[b + t] * sh.zs[b + t];
-        dmax = fmax(dmax, fabs(sh.Ds[b + t]));
+        zn2 += c.zs[b + t] * c.zs[b + t];
+        dmax = fmax(dmax, fabs(c.Ds[b + t]));
       }
       double rho = fabs(sh.e[bm - 1]);
       int K = 0, nd = 0, nr = 0;
-      for (int t = 0; t < n; ++t) sh.kflag[b + t] = 0;
+      for (int t = 0; t < n; ++t) c.kflag[b + t] = 0;
       if (zn2 > 0 && rho > 0) {
         const double zn = sqrt(zn2);
-        for (int t = 0; t < n; ++t) sh.zs[b + t] /= zn;
+        for (int t = 0; t < n; ++t) c.zs[b + t] /= zn;
         rho *= zn2;
         const double tol = 8.0 * kEps * fmax(dmax, rho);
         int last = -1;  // sorted position of the last kept entry
         for (int t = 0; t < n; ++t) {
-          const double zt = sh.zs[b + t];
+          const double zt = c.zs[b + t];
           if (rho * fabs(zt) <= tol) continue;
           if (last >= 0) {
-            const double zj = sh.zs[b + last], r = sqrt(zj * zj + zt * zt);
-            const double c = zt / r, sn = zj / r;
-            const double Dj = sh.Ds[b + last], Di = sh.Ds[b + t];
-            if (fabs((Di - Dj) * c * sn) <= tol) {
-              sh.zs[b + last] = 0.0;
-              sh.zs[b + t] = r;
-              sh.Ds[b + last] = Dj * c * c + Di * sn * sn;
-              sh.Ds[b + t] = Dj * sn * sn + Di * c * c;
-              sh.rj[b + nr] = last;
-              sh.ri[b + nr] = t;
-              sh.rc[b + nr] = c;
-              sh.rs[b + nr] = sn;
+            const double zj = c.zs[b + last], r = sqrt(zj * zj + zt * zt);
+            const double cs = zt / r, sn = zj / r;
+            const double Dj = c.Ds[b + last], Di = c.Ds[b + t];
+            if (fabs((Di - Dj) * cs * sn) <= tol) {
+              c.zs[b + last] = 0.0;
+              c.zs[b + t] = r;
+              c.Ds[b + last] = Dj * cs * cs + Di * sn * sn;
+              c.Ds[b + t] = Dj * sn * sn + Di * cs * cs;
+              c.rj[b + nr] = last;
+              c.ri[b + nr] = t;
+              c.rc[b + nr] = cs;
+              c.rs[b + nr] = sn;
               ++nr;
-              sh.kflag[b + last] = 0;
+              c.kflag[b + last] = 0;
             }
           }
-          sh.kflag[b + t] = 1;
+          c.kflag[b + t] = 1;
           last = t;
         }
       }
       // kept entries compacted in ascending sorted position (so ascending D), and the
       // deflated positions likewise
       for (int t = 0; t < n; ++t) {
-        if (sh.kflag[b + t]) {
-          sh.kpos[b + K] = t;
-          sh.dk[b + K] = sh.Ds[b + t];
-          sh.zk[b + K] = sh.zs[b + t];
+        if (c.kflag[b + t]) {
+          c.kpos[b + K] = t;
+          c.dk[b + K] = c.Ds[b + t];
+          c.zk[b + K] = c.zs[b + t];
           ++K;
         } else {
-          sh.dpos[b + nd++] = t;
+          c.dpos[b + nd++] = t;
         }
       }
-      sh.mK[m] = K;
-      sh.mrot[m] = nr;
-      sh.mrho[m] = rho;
+      c.mK[m] = K;
+      c.mrot[m] = nr;
+      c.mrho[m] = rho;
     });
     ex.sync();
     // working copy of the merge block's eigenvectors in sorted order, then the rotations
     ex.par(p * p, [&](int e) {
-      const int r = e % p, c = e / p;
-      const MergeAt g = merge_of(c, s, p);
+      const int r = e % p, col = e / p;
+      const MergeAt g = merge_of(col, s, p);
       if (g.pass() || r < g.b || r >= g.b2) return;
-      const int sc = sh.src[c];
+      const int sc = c.src[col];
       if (sc < g.b || sc >= g.b2) {  // not reachable: the rank above is a permutation
         sh.status = 0;
         return;
@@ -447,73 +627,82 @@ TNH_HD void dc_eig(const Ex &ex, DcShared &sh, double *Gr, double *Gi, int p, do
     ex.par(p, [&](int r) {
       const MergeAt g = merge_of(r, s, p);
       if (g.pass()) return;
-      const int nr = sh.mrot[g.m];
+      const int nr = c.mrot[g.m];
       for (int q = 0; q < nr; ++q) {
-        const int64_t cj = r + (g.b + sh.rj[g.b + q]) * P, ci = r + (g.b + sh.ri[g.b + q]) * P;
-        const double c = sh.rc[g.b + q], sn = sh.rs[g.b + q];
+        const int64_t cj = r + (g.b + c.rj[g.b + q]) * P, ci = r + (g.b + c.ri[g.b + q]) * P;
+        const double cs = c.rc[g.b + q], sn = c.rs[g.b + q];
         const double a = Q2[cj], bq = Q2[ci];
-        Q2[cj] = c * a - sn * bq;
-        Q2[ci] = sn * a + c * bq;
+        Q2[cj] = cs * a - sn * bq;
+        Q2[ci] = sn * a + cs * bq;
       }
     });
-    // roots of the secular equations (one thread per root)
-    ex.par(p, [&](int a) {
+    ex.mark(9 + 4 * level);
+    // roots of the secular equations: a group of G consecutive threads per root
+    ex.par(p * G, [&](int t) {
+      const int a = t / G, gl = t % G;
       const MergeAt g = merge_of(a, s, p);
       if (g.pass()) return;
-      const int j = a - g.b, K = sh.mK[g.m];
+      const int j = a - g.b, K = c.mK[g.m];
       if (j >= K) return;
       int o = 0, its = 0;
-      const double t = secular_root(sh.dk + g.b, sh.zk + g.b, sh.mrho[g.m], j, K, &o, &its);
-      sh.org[a] = o;
-      sh.tau[a] = t;
-      if (!(t == t) || its >= 96) sh.status = 0;
+      const double tt = secular_root(ex, c.dk + g.b, c.zk + g.b, c.mrho[g.m], j, K, gl, G, &o, &its);
+      if (gl == 0) {
+        c.org[a] = o;
+        c.tau[a] = tt;
+        if (!finite(tt) || its >= 96) sh.status = 0;
+      }
     });
     ex.sync();
     // Gu-Eisenstat: the coupling vector for which the computed roots are exact
     ex.par(p, [&](int a) {
       const MergeAt g = merge_of(a, s, p);
       if (g.pass()) return;
-      const int i = a - g.b, K = sh.mK[g.m];
+      const int i = a - g.b, K = c.mK[g.m];
       if (i >= K) return;
-      const double *dl = sh.dk + g.b;
+      const double *dl = c.dk + g.b;
       const double di = dl[i];
-      double prod = (dl[sh.org[a]] - di) + sh.tau[a];
+      double prod = (dl[c.org[a]] - di) + c.tau[a];
       for (int k = 0; k < K; ++k) {
         if (k == i) continue;
-        prod *= ((dl[sh.org[g.b + k]] - di) + sh.tau[g.b + k]) / (dl[k] - di);
+        prod *= ((dl[c.org[g.b + k]] - di) + c.tau[g.b + k]) / (dl[k] - di);
       }
-      sh.zh[a] = copysign(sqrt(fabs(prod) / sh.mrho[g.m]), sh.zk[a]);
+      c.zh[a] = copysign(sqrt(fabs(prod) / c.mrho[g.m]), c.zk[a]);
     });
     ex.sync();
+    ex.mark(10 + 4 * level);
     // eigenvectors of diag(dk) + rho zh zh^T: u_k(i) = zh_i / (dk_i - lambda_k), normalised
-    ex.par(p * p, [&](int e) {
-      const int r = e % p, c = e / p;
-      const MergeAt g = merge_of(c, s, p);
+    ex.par(p, [&](int col) {
+      const MergeAt g = merge_of(col, s, p);
       if (g.pass()) return;
-      const int i = r - g.b, k = c - g.b, K = sh.mK[g.m];
-      if (i < 0 || i >= K || k >= K) return;
-      const double *dl = sh.dk + g.b;
-      U[e] = sh.zh[r] / ((dl[i] - dl[sh.org[c]]) - sh.tau[c]);
+      const int k = col - g.b, K = c.mK[g.m];
+      if (k >= K) return;
+      const double *dl = c.dk + g.b;
+      const double lo = dl[c.org[col]], tk = c.tau[col];
+      double nn = 0;
+      for (int i = 0; i < K; ++i) {
+        const double v = c.zh[g.b + i] / ((dl[i] - lo) - tk);
+        nn += v * v;
+      }
+      c.cn[col] = 1.0 / sqrt(nn);
     });
     ex.sync();
-    ex.par(p, [&](int c) {
-      const MergeAt g = merge_of(c, s, p);
+    ex.par(p * p, [&](int e) {
+      const int r = e % p, col = e / p;
+      const MergeAt g = merge_of(col, s, p);
       if (g.pass()) return;
-      const int k = c - g.b, K = sh.mK[g.m];
-      if (k >= K) return;
-      double nn = 0;
-      for (int i = 0; i < K; ++i) nn += U[(g.b + i) + c * P] * U[(g.b + i) + c * P];
-      const double f = 1.0 / sqrt(nn);
-      for (int i = 0; i < K; ++i) U[(g.b + i) + c * P] *= f;
+      const int i = r - g.b, k = col - g.b, K = c.mK[g.m];
+      if (i < 0 || i >= K || k >= K) return;
+      const double *dl = c.dk + g.b;
+      U[e] = c.zh[r] / ((dl[i] - dl[c.org[col]]) - c.tau[col]) * c.cn[col];
     });
     // the merged block's eigenvalues in ascending order: the K roots and the deflated poles
     ex.par(p, [&](int a) {
       const MergeAt g = merge_of(a, s, p);
       if (g.pass()) return;
-      const int x = a - g.b, n = g.b2 - g.b, K = sh.mK[g.m];
-      const double *dl = sh.dk + g.b;
+      const int x = a - g.b, n = g.b2 - g.b, K = c.mK[g.m];
+      const double *dl = c.dk + g.b;
       auto val = [&](int y) {
-        return y < K ? dl[sh.org[g.b + y]] + sh.tau[g.b + y] : sh.Ds[g.b + sh.dpos[g.b + y - K]];
+        return y < K ? dl[c.org[g.b + y]] + c.tau[g.b + y] : c.Ds[g.b + c.dpos[g.b + y - K]];
       };
       const double vx = val(x), kx = order_key(vx);
       int rank = 0;
@@ -521,44 +710,44 @@ TNH_HD void dc_eig(const Ex &ex, DcShared &sh, double *Gr, double *Gi, int p, do
         const double ky = order_key(val(y));
         rank += (ky < kx) || (ky == kx && y < x);
       }
-      sh.fsrc[g.b + rank] = x;
-      sh.Dt[g.b + rank] = vx;
+      c.finv[g.b + x] = g.b + rank;
+      c.Dt[g.b + rank] = vx;
     });
     ex.sync();
+    ex.mark(11 + 4 * level);
+    // new eigenvectors of the merged block: Q2 (sorted, rotated) times U on the kept items;
+    // the deflated items' columns are copied
     ex.par(p, [&](int a) {
       const MergeAt g = merge_of(a, s, p);
       if (g.pass()) return;
-      sh.lam[a] = sh.Dt[a];
+      c.lam[a] = c.Dt[a];
     });
-    // new eigenvectors of the merged block: Q2 (sorted, rotated) times U on the kept columns
+    merge_product(sh, s, p, Q2, U, Q);
     ex.par(p * p, [&](int e) {
-      const int r = e % p, c = e / p;
-      const MergeAt g = merge_of(c, s, p);
+      const int r = e % p, col = e / p;  // col - b: the item's index within its merge
+      const MergeAt g = merge_of(col, s, p);
       if (g.pass() || r < g.b || r >= g.b2) return;
-      const int x = sh.fsrc[c], K = sh.mK[g.m];
-      if (x < 0 || x >= g.b2 - g.b) {  // not reachable: the rank above is a permutation
+      const int x = col - g.b, K = c.mK[g.m];
+      if (x < K) return;
+      const int fp = c.finv[col];
+      if (fp < g.b || fp >= g.b2) {  // not reachable: the final rank is a permutation
         sh.status = 0;
         return;
       }
-      double v;
-      if (x < K) {
-        v = 0.0;
-        for (int q = 0; q < K; ++q) v += Q2[r + (g.b + sh.kpos[g.b + q]) * P] * U[(g.b + q) + (g.b + x) * P];
-      } else {
-        v = Q2[r + (g.b + sh.dpos[g.b + x - K]) * P];
-      }
+      const double v = Q2[r + (g.b + c.dpos[g.b + x - K]) * P];
       if (!finite(v)) sh.status = 0;
-      Q[e] = v;
+      Q[r + fp * P] = v;
     });
     ex.sync();
   }
   ex.par(p, [&](int i) {
-    const double v = ldexp(sh.lam[i], ex2);
+    const double v = ldexp(c.lam[i], ex2);
     out[i] = v;
     if (!finite(v)) sh.status = 0;
   });
   ex.sync();
   ex.par(1, [&](int) { out[p] = sh.status ? 1.0 : 0.0; });
+  ex.mark(63);
 }
 
 }  // namespace dc

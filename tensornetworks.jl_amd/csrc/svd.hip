@@ -819,22 +819,46 @@ void launch_tiny_eig(hipStream_t st, const double *G, int p, int64_t imG, double
 
 // ---- small Hermitian eigenproblems by reduction and divide and conquer (dc_eig.hpp)
 
-// Execution policy of dc_eig on one workgroup of kDcThreads threads.
+// Execution policy of dc_eig on one workgroup of kDcThreads threads. The reductions take one
+// barrier (the caller alternates two scratch buffers, so a fast thread's next write cannot
+// overtake a slow thread's read); secular-root groups are G consecutive lanes of one wave.
 struct DcDeviceEx {
   template <class F> __device__ void par(int n, F &&f) const {
+#pragma unroll 4
     for (int t = threadIdx.x; t < n; t += dc::kDcThreads) f(t);
   }
   __device__ void sync() const { __syncthreads(); }
+  __device__ void mark(int) const {}
   template <class F> __device__ double sum(int n, F &&f, double *red) const {
     double s = 0;
+#pragma unroll 4
     for (int t = threadIdx.x; t < n; t += dc::kDcThreads) s += f(t);
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
     double r = 0;
     for (int w = 0; w < dc::kDcThreads / 64; ++w) r += red[w];
-    __syncthreads();
     return r;
+  }
+  template <class F> __device__ double vmax(int n, F &&f, double *red) const {
+    double s = 0;
+    for (int t = threadIdx.x; t < n; t += dc::kDcThreads) s = fmax(s, f(t));
+    for (int o = 32; o > 0; o >>= 1) s = fmax(s, __shfl_xor(s, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    double r = 0;
+    for (int w = 0; w < dc::kDcThreads / 64; ++w) r = fmax(r, red[w]);
+    return r;
+  }
+  // lanes per secular root: the largest power of two with p G <= kDcThreads, at most a wave
+  __device__ int gsize(int p) const {
+    int g = 1;
+    while (g < 64 && 2 * g * p <= dc::kDcThreads) g *= 2;
+    return g;
+  }
+  __device__ double gsum(double x, int G) const {
+    for (int o = G >> 1; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
   }
 };
 
@@ -850,39 +874,62 @@ __global__ __launch_bounds__(dc::kDcThreads) void dc_eig_kernel(double *G, int p
 
 // V = H_0 H_1 ... H_{p-2} Z: the tridiagonal eigenvectors Z taken back through the
 // Householder reflectors H_k = I - tau_k v_k v_k^H (v_k in column k of G below the diagonal).
-// One wave per 4 columns, 16 lanes per column; lane l keeps rows l, l + 16, ... of its column
-// in registers for the whole product, so the reflectors are the only memory traffic and the
-// column's dot product is a 16-lane shuffle reduction: no barriers.
-template <bool CPLX>
+// One wave per 4 columns, 16 lanes per column; lane l keeps rows l, l + 16, ... (R of them,
+// 16 R >= p) of its column in registers for the whole product, so the reflectors are the only
+// memory traffic and the column's dot product is a 16-lane shuffle reduction: no barriers.
+// The next reflector is loaded while the current one is applied.
+template <bool CPLX, int R>
 __global__ __launch_bounds__(256) void dc_backtransform_kernel(const double *__restrict__ G, int p, int64_t imG,
                                                                const double *__restrict__ taus,
                                                                const double *__restrict__ Z, double *__restrict__ V,
                                                                int64_t imV) {
-  constexpr int R = dc::kDcPMax / 16;
   const int lane = threadIdx.x & 63, l = lane & 15;
   const int c = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 4 + (lane >> 4);
   const bool act = c < p;
   const int64_t P = p;
-  double xr[R], xi[R];
+  double xr[R], xi[R], nr[R], ni[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int i = l + 16 * r;
     xr[r] = (act && i < p) ? Z[i + c * P] : 0.0;
     xi[r] = 0.0;
   }
-  for (int k = p - 2; k >= 0; --k) {
-    const double tr = taus[k], ti = CPLX ? taus[p + k] : 0.0;
-    if (tr == 0.0 && ti == 0.0) continue;
+  auto load = [&](int k, double *ar_, double *ai_) {  // v_k on this lane's rows (0 off its support)
     const double *vr = G + k * P, *vi = CPLX ? G + imG + k * P : nullptr;
-    double sr = 0, si = 0;  // s = v^H x over rows k+1 ..
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int i = l + 16 * r;
-      if (i > k && i < p) {
-        const double a = vr[i], b = CPLX ? vi[i] : 0.0;
-        sr += a * xr[r] + b * xi[r];
-        if (CPLX) si += a * xi[r] - b * xr[r];
-      }
+      const bool on = i > k && i < p;
+      ar_[r] = on ? vr[i] : 0.0;
+      ai_[r] = (CPLX && on) ? vi[i] : 0.0;
+    }
+  };
+  int k = p - 2;
+  double tnr = 0, tni = 0;
+  if (k >= 0) {
+    load(k, nr, ni);
+    tnr = taus[k];
+    tni = CPLX ? taus[p + k] : 0.0;
+  }
+  for (; k >= 0; --k) {
+    double cr[R], ci[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      cr[r] = nr[r];
+      ci[r] = ni[r];
+    }
+    const double tr = tnr, ti = tni;
+    if (k > 0) {
+      load(k - 1, nr, ni);
+      tnr = taus[k - 1];
+      tni = CPLX ? taus[p + k - 1] : 0.0;
+    }
+    if (tr == 0.0 && ti == 0.0) continue;
+    double sr = 0, si = 0;  // s = v^H x
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      sr += cr[r] * xr[r] + ci[r] * xi[r];
+      if (CPLX) si += cr[r] * xi[r] - ci[r] * xr[r];
     }
 #pragma unroll
     for (int o = 8; o > 0; o >>= 1) {
@@ -892,12 +939,8 @@ __global__ __launch_bounds__(256) void dc_backtransform_kernel(const double *__r
     const double fr = tr * sr - ti * si, fi = tr * si + ti * sr;  // tau s
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const int i = l + 16 * r;
-      if (i > k && i < p) {
-        const double a = vr[i], b = CPLX ? vi[i] : 0.0;
-        xr[r] -= a * fr - b * fi;
-        if (CPLX) xi[r] -= a * fi + b * fr;
-      }
+      xr[r] -= cr[r] * fr - ci[r] * fi;
+      if (CPLX) xi[r] -= cr[r] * fi + ci[r] * fr;
     }
   }
   if (!act) return;
@@ -909,6 +952,20 @@ __global__ __launch_bounds__(256) void dc_backtransform_kernel(const double *__r
       if (CPLX) V[imV + i + c * P] = xi[r];
     }
   }
+}
+
+template <bool CPLX>
+void launch_dc_backtransform(hipStream_t st, const double *G, int p, int64_t imG, const double *taus,
+                             const double *Z, double *V, int64_t imV) {
+  const dim3 grid(unsigned(((p + 3) / 4 + 3) / 4)), block(256);
+  if (p <= 64)
+    hipLaunchKernelGGL((dc_backtransform_kernel<CPLX, 4>), grid, block, 0, st, G, p, imG, taus, Z, V, imV);
+  else if (p <= 128)
+    hipLaunchKernelGGL((dc_backtransform_kernel<CPLX, 8>), grid, block, 0, st, G, p, imG, taus, Z, V, imV);
+  else if (p <= 256)
+    hipLaunchKernelGGL((dc_backtransform_kernel<CPLX, 16>), grid, block, 0, st, G, p, imG, taus, Z, V, imV);
+  else
+    hipLaunchKernelGGL((dc_backtransform_kernel<CPLX, 32>), grid, block, 0, st, G, p, imG, taus, Z, V, imV);
 }
 
 struct Schedule {
@@ -1219,17 +1276,14 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
       double *Ud = static_cast<double *>(ctx->workspace(627, sizeof(double) * pp));
       double *dout = static_cast<double *>(ctx->workspace(616, sizeof(double) * (p + 1)));
       hipEvent_t pd = ctx->prof_begin();
-      const unsigned nbt = unsigned(((p + 3) / 4 + 3) / 4);
       if (cplx) {
         hipLaunchKernelGGL(dc_eig_kernel<true>, dim3(1), dim3(dc::kDcThreads), 0, ctx->stream, G, p, pp, taus,
                            Qd, Q2d, Ud, dout);
-        hipLaunchKernelGGL(dc_backtransform_kernel<true>, dim3(nbt), dim3(256), 0, ctx->stream, G, p, pp, taus,
-                           Qd, V, pp);
+        launch_dc_backtransform<true>(ctx->stream, G, p, pp, taus, Qd, V, pp);
       } else {
         hipLaunchKernelGGL(dc_eig_kernel<false>, dim3(1), dim3(dc::kDcThreads), 0, ctx->stream, G, p, pp, taus,
                            Qd, Q2d, Ud, dout);
-        hipLaunchKernelGGL(dc_backtransform_kernel<false>, dim3(nbt), dim3(256), 0, ctx->stream, G, p, pp, taus,
-                           Qd, V, pp);
+        launch_dc_backtransform<false>(ctx->stream, G, p, pp, taus, Qd, V, pp);
       }
       TNH_HIP(hipGetLastError());
       ctx->prof_end("dc_eig", pd, 0.0, double(p));

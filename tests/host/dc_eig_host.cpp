@@ -2,6 +2,7 @@
 // same algorithm the device kernel instantiates, run with a sequential execution policy, plus
 // the back-transformation by the Householder vectors. Built by tests/test_dc_eig_host.py.
 #include <cstdlib>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -13,11 +14,19 @@ struct HostEx {
     for (int t = 0; t < n; ++t) f(t);
   }
   void sync() const {}
+  void mark(int) const {}
   template <class F> double sum(int n, F &&f, double *) const {
     double s = 0;
     for (int t = 0; t < n; ++t) s += f(t);
     return s;
   }
+  template <class F> double vmax(int n, F &&f, double *) const {
+    double s = 0;
+    for (int t = 0; t < n; ++t) s = std::fmax(s, f(t));
+    return s;
+  }
+  int gsize(int) const { return 1; }
+  double gsum(double x, int) const { return x; }
 };
 }  // namespace
 
