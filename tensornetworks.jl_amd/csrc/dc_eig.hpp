@@ -14,8 +14,10 @@
 // (f(t) for t in [0, n), spread over the workgroup), `sync()` (a workgroup barrier),
 // `sum(n, f, scratch)` / `vmax(n, f, scratch)` (a workgroup-wide sum or maximum returned to
 // every thread, one barrier: consecutive calls must alternate their scratch buffers),
-// `gsize(p)` and `gsum(x, G)` (groups of G consecutive threads that share a secular root and
-// sum over the group), and `mark(id)`, a phase boundary that a timing policy can record. The
+// `gsize(p)`, `gsum(x, G)` and `gsum4(a, b, c, d, G)` (groups of G consecutive threads that
+// share a secular root, and sums over the group), `nthreads()` (the workgroup size, at most kDcThreads; small matrices
+// run on fewer threads, where a barrier costs less), and `mark(id)`, a phase boundary that a
+// timing policy can record. The
 // device policy (in the .hip file) maps these to threads, barriers and lane shuffles; a host
 // policy that runs the loops sequentially (G = 1) compiles the same code for tests on the
 // CPU. The merge products are the one place with a device-only path (matrix-core tiles);
@@ -38,7 +40,7 @@ namespace tnh {
 namespace dc {
 
 constexpr int kDcPMax = 512;      // largest matrix side
-constexpr int kDcThreads = 1024;  // workgroup size the partial-sum layout is sized for
+constexpr int kDcThreads = 1024;  // largest workgroup; the partial-sum layout is sized for it
 constexpr int kDcLdsMat = 13800;  // doubles of LDS for the active matrix of the reduction
 constexpr double kEps = 2.220446049250313e-16;
 
@@ -145,10 +147,7 @@ TNH_HD double secular_root(const Ex &ex, const double *dl, const double *zk, dou
         dphi += q * r;
       }
     }
-    psi = ex.gsum(psi, G);
-    dpsi = ex.gsum(dpsi, G);
-    phi = ex.gsum(phi, G);
-    dphi = ex.gsum(dphi, G);
+    ex.gsum4(psi, dpsi, phi, dphi, G);
     const double f = 1.0 + psi + phi;
     // f is zero to within the rounding of its own evaluation: t is a root
     if (fabs(f) <= 8.0 * kEps * (1.0 + fabs(psi) + fabs(phi))) break;
@@ -209,7 +208,7 @@ TNH_HD inline void merge_product(DcShared &sh, int s, int p, const double *Q2, c
 #if defined(__HIP_DEVICE_COMPILE__)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, rr = lane & 15;
   const int tpd = (2 * s + 31) / 32, tpm = tpd * tpd;
-  for (int tile = wave; tile < nm * tpm; tile += kDcThreads / 64) {
+  for (int tile = wave; tile < nm * tpm; tile += int(blockDim.x >> 6)) {
     const int m = tile / tpm, loc = tile % tpm;
     const int b = 2 * s * m, bm = b + s < p ? b + s : p, b2 = b + 2 * s < p ? b + 2 * s : p;
     if (bm >= b2) continue;
@@ -382,7 +381,7 @@ TNH_HD void dc_eig(const Ex &ex, DcShared &sh, double *Gr, double *Gi, int p, do
       }
     };
     // y = B v: nc (<= 16) column chunks per row, partial sums in LDS; v formed on the fly
-    int nc = kDcThreads / n > 0 ? kDcThreads / n : 1;
+    int nc = ex.nthreads() / n > 0 ? ex.nthreads() / n : 1;
     if (nc > 16) nc = 16;
     ex.par(n * nc, [&](int e) {
       const int i = e % n, cc = e / n;

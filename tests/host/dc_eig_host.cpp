@@ -26,7 +26,9 @@ struct HostEx {
     return s;
   }
   int gsize(int) const { return 1; }
+  int nthreads() const { return 1024; }
   double gsum(double x, int) const { return x; }
+  void gsum4(double &, double &, double &, double &, int) const {}
 };
 }  // namespace
 
