@@ -139,3 +139,83 @@ def test_applyop_and_norm(ctx, rng):
     d.movecenter(5)
     d.movecenter(1)
     assert abs(d.norm() - np.linalg.norm(v)) < 1e-12
+
+
+@pytest.mark.parametrize("trunc", [dict(cutoff=1e-10, maxdim=3, mindim=1), dict(cutoff=1e-4, maxdim=0, mindim=1)])
+def test_tebd_truncated_matches_oracle(ctx, trunc):
+    """Real-time TEBD with the truncation active (bond dimension capped at 3, or a cutoff of
+    1e-4 that discards weight at every step), N = 10 TFIM quench from a Neel state: the
+    device tebd against the oracle's applygates_mps (gatelist.jl:191-223) plus the same
+    normalisation per step, on the same Trotter rows. A truncated step keeps the top Schmidt
+    subspace, which does not depend on the SVD's gauge, so the bond dimensions agree exactly
+    and the local <sz_i> and the norm history to rounding."""
+    from tnhip import productMPS, spinhalf
+    from tnhip import OpList
+    from tnhip.tebd import TEBDOperators, tebd
+    from oracle.exact import OPS
+    from oracle.mps import normalize, product_mps
+    from oracle.tebd import applygates_mps, rows_from_layers, trotter_layers
+
+    N, dt, steps = 10, 0.05, 12
+    st = spinhalf()
+    H = tfim_oplist(N, J=1.0, h=0.9, g=0.3)
+    names = ["up" if i % 2 else "dn" for i in range(1, N + 1)]
+    zs = OpList(N)
+    for i in range(1, N + 1):
+        zs.add("z", i)
+    obs = TEBDOperators(st, zs)
+    psi, _ = tebd(st, productMPS(st, names), H, dt, dt * steps, dt, [obs], evol="real", verbose=False, **trunc)
+    rows = rows_from_layers(trotter_layers(terms_of(H), N, dt, order=2, evol="real"))
+    up, dn = np.array([1.0, 0.0]), np.array([0.0, 1.0])
+    ref = product_mps([up if n == "up" else dn for n in names])
+    zref = []
+    for _ in range(steps):
+        applygates_mps(ref, rows, **trunc)
+        normalize(ref)
+        zref.append([_expect_z(ref, i, OPS["z"]) for i in range(1, N + 1)])
+    assert [np.asarray(t).shape[2] for t in psi.tensors[:-1]] == ref.bonddims()
+    assert max(ref.bonddims()) > 1
+    got = np.real(np.array(obs.measurements[1:]))
+    assert got.shape == (steps, N)
+    assert np.allclose(got, np.array(zref), atol=1e-9)
+
+
+def _expect_z(psi, site, op):
+    """<psi|op_site|psi> of a normalised oracle MPS by a full contraction."""
+    from oracle.mps import to_dense
+
+    v = to_dense(psi)
+    N = len(psi)
+    v = v.reshape([2] * N)
+    w = np.tensordot(op, v, axes=([1], [site - 1]))
+    w = np.moveaxis(w, 0, site - 1)
+    return float(np.real(np.vdot(v.ravel(), w.ravel())))
+
+
+def test_random_mps_matches_oracle_construction(ctx):
+    """randomMPS(d, N, chi) (gmps.jl:276-291) against the oracle's random_mps on the same
+    generator: same draws in the same order, the centre moved to N and back to 1, site 1
+    redrawn, normalised. The two SVD implementations may choose opposite signs for the bond-1
+    basis vectors (the only gauge that reaches the state, since site 1 is redrawn after the
+    move), so the device state must equal one of the oracle state's sign variants of site 1's
+    columns to 1e-12; every site 2..N is right-canonical and the bond dimensions agree."""
+    import itertools
+    from tnhip import randomMPS
+    from oracle.mps import MPS, random_mps, to_dense
+
+    N, d, chi = 8, 2, 4
+    psi = randomMPS(d, N, chi, rng=np.random.default_rng(11))
+    ref = random_mps(N, d, chi, np.random.default_rng(11))
+    assert [np.asarray(t).shape[2] for t in psi.tensors[:-1]] == ref.bonddims() and psi.center == ref.center == 1
+    got = to_dense(MPS([np.asarray(t) for t in psi.tensors], 1))
+    assert abs(np.linalg.norm(got) - 1.0) < 1e-12
+    best = np.inf
+    k = ref[1].shape[2]
+    for signs in itertools.product([1.0, -1.0], repeat=k):
+        var = MPS([t.copy() for t in ref.tensors], 1)
+        var[1] = var[1] * np.array(signs)[None, None, :]
+        best = min(best, np.linalg.norm(to_dense(var) - got))
+    assert best < 1e-12
+    for t in psi.tensors[1:]:
+        B = np.asarray(t).reshape(t.shape[0], -1, order="F")
+        assert np.allclose(B @ B.conj().T, np.eye(B.shape[0]), atol=1e-12)
