@@ -381,7 +381,7 @@ def gradient_tfim_n64(chi=512, block=8, cpu=True):
     return out
 
 
-def qjmc_tfim_n40(ntraj=512, steps=200, sample_traj=16, sample_steps=20, nworkers=8, chi_max=256):
+def qjmc_tfim_n40(ntraj=512, steps=200, sample_traj=16, sample_steps=20, nworkers=8, chi_max=256, cpu_traj=2):
     """Quantum-jump trajectories of the dissipative TFIM, N = 40, g = 0.5, jump operators
     s-_i at rate 0.1, dt = 0.01, second-order Trotter, cutoff 1e-8, chi_max 256, from all
     spins up, on fixed uniforms (seed 0), <sz_i> saved every step. The full workload
@@ -389,6 +389,8 @@ def qjmc_tfim_n40(ntraj=512, steps=200, sample_traj=16, sample_steps=20, nworker
     sample (the first `sample_traj` trajectories, `sample_steps` steps) also runs on the
     per-trajectory workers (`nworkers` streams), whose jump records must equal the batched
     engine's on the same uniforms, and one trajectory through the Python host loop.
+    CPU baseline: the first `cpu_traj` trajectories of the workload through the oracle's MPS
+    trajectory (numpy on the host cores), checked against the batched engine's rows.
     Reports trajectory steps per second for each."""
     from tnhip import productMPS, spinhalf
     from tnhip.models import OpList
@@ -437,6 +439,25 @@ def qjmc_tfim_n40(ntraj=512, steps=200, sample_traj=16, sample_steps=20, nworker
     psi.ctx.synchronize()
     host_el = time.perf_counter() - t0
     sz = full["measurements"].real.mean(axis=(0, 2))
+    cpu = None
+    if cpu_traj > 0:  # the oracle's MPS trajectory (qjmc.jl:28-167) on the same uniforms
+        from oracle.qjmc import qjmc_mps
+
+        def terms(ol):
+            return [(list(o), list(s_), c) for o, s_, c in zip(ol.ops, ol.sites, ol.coeffs)]
+
+        up = np.array([1.0, 0.0])
+        t0 = time.perf_counter()
+        same = True
+        for t in range(cpu_traj):
+            j, _, _, ms, _ = qjmc_mps([up] * N, terms(H), terms(Jo), N, dt, steps * dt, U[t],
+                                     obs=[("z", i) for i in range(1, N + 1)], cutoff=1e-8, maxdim=chi_max)
+            same = same and j == full["jumps"][t] and np.abs(ms.real - full["measurements"][t].real).max() <= 1e-8
+        el_cpu = time.perf_counter() - t0
+        cpu = {"seconds": el_cpu, "trajectories": cpu_traj, "steps": steps, "threads": os.cpu_count(),
+               "trajectory_steps_per_s": cpu_traj * steps / el_cpu,
+               "jump_records_and_sz_equal_batched": bool(same),
+               "sample": "oracle/qjmc.py qjmc_mps, the first trajectories of the workload"}
     return {"batched": {"seconds": el_full, "trajectories": ntraj, "steps": steps,
                         "trajectory_steps_per_s": ntraj * steps / el_full, "jumps": int(sum(full["njumps"])),
                         "maxbonddim": int(max(full["maxbonddim"])), "sz_avg_t0": float(sz[0]),
@@ -444,7 +465,9 @@ def qjmc_tfim_n40(ntraj=512, steps=200, sample_traj=16, sample_steps=20, nworker
             "workers_sample": res_w,
             "host_loop": {"seconds": host_el, "steps": sample_steps,
                           "trajectory_steps_per_s": sample_steps / host_el, "jumps": len(jumps),
-                          "jump_records_equal_batched": [j for j in jumps] == a["jumps"][0]}}
+                          "jump_records_equal_batched": [j for j in jumps] == a["jumps"][0]},
+            "cpu_port": cpu,
+            "batched_speedup_vs_cpu_port": (ntraj * steps / el_full) / cpu["trajectory_steps_per_s"] if cpu else None}
 
 
 def tebd_tfim_n40(steps=80, dt=0.05, chi_max=256, cutoff=1e-10, cpu_steps=0):
