@@ -1292,7 +1292,12 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
     }
     return true;
   };
-  if (!force_fallback) {
+  const int dtp = cplx ? TNH_C128 : TNH_F64;
+  // a usable warm-start basis of a block-Jacobi split: Q^H G Q is then formed from M
+  // directly, and G itself is never needed
+  const bool use_warm = !force_fallback && p > dc::kDcPMax && warm && *warm && (*warm)->ndim == 2 &&
+                        (*warm)->dims[0] == p && (*warm)->dims[1] == p && (*warm)->dtype == dtp;
+  if (!force_fallback && !use_warm) {
     form_gram();
     if (p > 8 && p <= dc::kDcPMax) {
       // reduction to tridiagonal form and divide and conquer in one workgroup, then the
@@ -1328,6 +1333,74 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
     }
   }
   if (!force_fallback && !ok && p > JP) {
+    if (use_warm) {
+      // Warm start: with an orthonormal basis Q of this Gram's space from an earlier split of
+      // the same bond (its eigenvectors then), the Jacobi runs on Q^H G Q, nearly diagonal
+      // when the matrix has changed little, and accumulates its rotations onto V = Q, so that
+      // V ends as the eigenvectors of G itself. Same convergence test, so the same accuracy.
+      // Q^H G Q = X X^H with X = Q^H M (side 0), or X^H X with X = M Q (side 1): two products
+      // instead of the Gram matrix and two more.
+      const double *Q = static_cast<const double *>((*warm)->ptr);
+      const int q = side == 0 ? n : m;  // the other side of M
+      const int64_t xq = int64_t(p) * q;
+      double *X = static_cast<double *>(ctx->workspace(618, sizeof(double) * xq * planes));
+      COperand Mo, Qo, Xo;
+      Mo.re = Mp;
+      Mo.im = cplx ? Mp + int64_t(m) * n : nullptr;
+      Qo.re = Q;
+      Qo.im = cplx ? Q + pp : nullptr;
+      Xo.re = X;
+      Xo.im = cplx ? X + xq : nullptr;
+      Gemm g1, g2;
+      COperand A1, B1, A2, B2;
+      if (side == 0) {
+        g1.M = p;  // X (p x n) = Q^H M
+        g1.N = n;
+        g1.K = p;
+        g1.transA = true;
+        g1.lda = p;
+        g1.ldb = m;
+        g1.ldc = p;
+        A1 = Qo;
+        A1.conj = true;
+        B1 = Mo;
+        g2.M = p;  // G = X X^H
+        g2.N = p;
+        g2.K = n;
+        g2.transB = true;
+        g2.lda = p;
+        g2.ldb = p;
+        g2.ldc = p;
+        A2 = Xo;
+        B2 = Xo;
+        B2.conj = true;
+      } else {
+        g1.M = m;  // X (m x p) = M Q
+        g1.N = p;
+        g1.K = p;
+        g1.lda = m;
+        g1.ldb = p;
+        g1.ldc = m;
+        A1 = Mo;
+        B1 = Qo;
+        g2.M = p;  // G = X^H X
+        g2.N = p;
+        g2.K = m;
+        g2.transA = true;
+        g2.lda = m;
+        g2.ldb = m;
+        g2.ldc = p;
+        A2 = Xo;
+        A2.conj = true;
+        B2 = Xo;
+      }
+      cgemm(ctx, g1, A1, B1, X, cplx ? X + xq : nullptr, TNH_F64, TNH_PREC_F64);
+      cgemm(ctx, g2, A2, B2, G, cplx ? G + pp : nullptr, TNH_F64, TNH_PREC_F64);
+      copy_device(ctx, V, Q, sizeof(double) * pp * planes);
+    } else {
+      hipLaunchKernelGGL(eye_kernel, dim3(unsigned((pp + 255) / 256)), dim3(256), 0, ctx->stream, V, p,
+                         pp, cplx);
+    }
     hipLaunchKernelGGL(hermitize_kernel, dim3(unsigned((pp + 255) / 256)), dim3(256), 0, ctx->stream,
                        G, p, pp, cplx);
     // Entries below 4 eps sqrt(p) max_i G_ii are rounding noise: each rotation of a
@@ -1348,50 +1421,6 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
       scale(ctx, G, TNH_F64, pp * planes, std::ldexp(1.0, -ex));
       dmax = std::ldexp(dmax, -ex);
     }
-    // Warm start: with an orthonormal basis Q of this Gram's space from an earlier split of
-    // the same bond (its eigenvectors then), the Jacobi runs on Q^H G Q, nearly diagonal when
-    // the matrix has changed little, and accumulates its rotations onto V = Q, so that V ends
-    // as the eigenvectors of G itself. Same convergence test, so the same accuracy.
-    const int dtp = cplx ? TNH_C128 : TNH_F64;
-    const bool use_warm = warm && *warm && (*warm)->ndim == 2 && (*warm)->dims[0] == p &&
-                          (*warm)->dims[1] == p && (*warm)->dtype == dtp;
-    if (use_warm) {
-      const double *Q = static_cast<const double *>((*warm)->ptr);
-      double *T = static_cast<double *>(ctx->workspace(618, sizeof(double) * pp * planes));
-      Gemm g;
-      g.M = p;
-      g.N = p;
-      g.K = p;
-      g.lda = p;
-      g.ldb = p;
-      g.ldc = p;
-      COperand Go, Qo, To;
-      Go.re = G;
-      Go.im = cplx ? G + pp : nullptr;
-      Qo.re = Q;
-      Qo.im = cplx ? Q + pp : nullptr;
-      To.re = T;
-      To.im = cplx ? T + pp : nullptr;
-      cgemm(ctx, g, Go, Qo, T, cplx ? T + pp : nullptr, TNH_F64, TNH_PREC_F64);  // T = G Q
-      Gemm h = g;
-      h.transA = true;
-      COperand Qh = Qo;
-      Qh.conj = true;
-      cgemm(ctx, h, Qh, To, G, cplx ? G + pp : nullptr, TNH_F64, TNH_PREC_F64);  // G = Q^H T
-      hipLaunchKernelGGL(hermitize_kernel, dim3(unsigned((pp + 255) / 256)), dim3(256), 0, ctx->stream,
-                         G, p, pp, cplx);
-      copy_device(ctx, V, Q, sizeof(double) * pp * planes);
-      // the noise floor below is relative to max_i G_ii of the matrix the Jacobi sees
-      double *dv = static_cast<double *>(ctx->workspace(612, sizeof(double) * p));
-      hipLaunchKernelGGL(diag_kernel, dim3((p + 255) / 256), dim3(256), 0, ctx->stream, G, p, dv);
-      TNH_HIP(hipGetLastError());
-      to_host(ctx, val.data(), dv, sizeof(double) * p);
-      dmax = 0;
-      for (double v : val) dmax = std::max(dmax, std::fabs(v));
-    } else {
-      hipLaunchKernelGGL(eye_kernel, dim3(unsigned((pp + 255) / 256)), dim3(256), 0, ctx->stream, V, p,
-                         pp, cplx);
-    }
     const double abstol = 4.0 * 2.220446049250313e-16 * std::sqrt(double(p)) * dmax;
     hipEvent_t pj = ctx->prof_begin();
     JacobiResult jr = jacobi(ctx, true, G, p, p, cplx, V, tol, abstol, 40);
@@ -1407,7 +1436,7 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
         v = std::sqrt(std::max(std::ldexp(v, ex), 0.0));
       }
     }
-    if (ok && warm) {
+    if (ok && warm && p > dc::kDcPMax) {
       // The eigenvectors become the next split's warm start, ordered by descending
       // eigenvalue: the next Q^H G Q then holds its large eigenvalues in the leading column
       // blocks, so pairs of two trailing (noise-level) blocks are below the floor from the
