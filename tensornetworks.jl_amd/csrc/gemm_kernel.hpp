@@ -207,10 +207,18 @@ struct Tile {
 // One pre-split operand tile (ROWS x 32 bf16, hi and lo planes) from [row][k] planes with
 // row stride ld (k padded, so every 16-byte vector lies inside a row); rows beyond nrows
 // load the tile base and are zeroed (branch-free).
+// LDS layout: unpadded 64-byte rows whose four 16-byte k-chunks are permuted per row,
+// chunk c of row r at position c ^ swz(r), swz(r) = ((r >> 1) ^ (r >> 3)) & 3. A fragment
+// read (16 rows, one chunk) and a staging store (4 chunks of consecutive rows) then touch
+// distinct banks whether the LDS serves 8 or 16 lanes of a 16-byte access per cycle; the
+// padded 80-byte rows this replaces stalled ~28% of the kernel on bank conflicts
+// (SQ_LDS_BANK_CONFLICT).
+__device__ __forceinline__ int bf16_swz(int r) { return ((r >> 1) ^ (r >> 3)) & 3; }
+
 template <int ROWS>
 struct TileP {
   static constexpr int BK = 32;
-  static constexpr int LDK = BK + 8;  // [row][k] stride: 80 bytes keeps 16-byte alignment
+  static constexpr int LDK = BK;  // [row][k] stride (64 bytes), chunks permuted by bf16_swz
   static constexpr int ELEMS = 2 * ROWS * LDK;
   static constexpr int NV = ROWS * BK / 8 / 256;  // 16-byte vectors per plane per thread
   static_assert(NV >= 1, "tile too small");
@@ -242,8 +250,9 @@ struct TileP {
     for (int e = 0; e < NV; ++e) {
       int r, k;
       pos(e, r, k);
-      *reinterpret_cast<v8us *>(&lds[r * LDK + k]) = hi[e];
-      *reinterpret_cast<v8us *>(&lds[ROWS * LDK + r * LDK + k]) = lo[e];
+      const int o = r * LDK + ((((k >> 3) ^ bf16_swz(r)) & 3) << 3);
+      *reinterpret_cast<v8us *>(&lds[o]) = hi[e];
+      *reinterpret_cast<v8us *>(&lds[ROWS * LDK + o]) = lo[e];
     }
   }
 };
@@ -363,13 +372,15 @@ __global__ __launch_bounds__(256) void gemm_kernel(Params p) {
       v8us ah[FM], al[FM], bh[FN], bl[FN];
 #pragma unroll
       for (int i = 0; i < FM; ++i) {
-        const int o = (wm0 + 16 * i + rr) * LD + 8 * g;
+        const int row = wm0 + 16 * i + rr;
+        const int o = row * LD + (((g ^ bf16_swz(row)) & 3) << 3);
         ah[i] = *reinterpret_cast<const v8us *>(&As[o]);
         al[i] = *reinterpret_cast<const v8us *>(&As[BM * LD + o]);
       }
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
-        const int o = (wn0 + 16 * j + rr) * LD + 8 * g;
+        const int row = wn0 + 16 * j + rr;
+        const int o = row * LD + (((g ^ bf16_swz(row)) & 3) << 3);
         bh[j] = *reinterpret_cast<const v8us *>(&Bs[o]);
         bl[j] = *reinterpret_cast<const v8us *>(&Bs[BN * LD + o]);
       }
