@@ -470,6 +470,83 @@ def qjmc_tfim_n40(ntraj=512, steps=200, sample_traj=16, sample_steps=20, nworker
             "batched_speedup_vs_cpu_port": (ntraj * steps / el_full) / cpu["trajectory_steps_per_s"] if cpu else None}
 
 
+def qjmc_tfim_n40_critical(ntraj=256, steps=150, sample_traj=4, nworkers=4, chi_max=256, cpu_traj=1):
+    """Quantum-jump trajectories in the regime of uneven, growing bond dimensions: the
+    critical dissipative TFIM, N = 40, g = 1, s-_i at rate 0.02, dt = 0.02, 150 second-order
+    Trotter steps, cutoff 1e-10, chi_max 256, from all spins up, uniforms from seed 1. Bonds
+    pass 16 near t = 2 and reach 30-40 by t = 3 (fewer in trajectories that jump), so every
+    trajectory outgrows the narrow batched slots and is rerun on the wide ones (one
+    1024-thread workgroup per trajectory, divide-and-conquer splits); the time includes the
+    narrow run. The first `sample_traj` trajectories also run on the per-trajectory workers
+    (`nworkers` streams, one trajectory per stream), and the first `cpu_traj` through the
+    oracle's MPS trajectory; both must reproduce the batched rows (jump records, <sz_i> to
+    1e-8). Reports trajectory steps per second for each."""
+    from tnhip import spinhalf
+    from tnhip.models import OpList
+    from tnhip.qjmc import qjmc_trajectories
+
+    N, g, gamma, dt, cutoff = 40, 1.0, 0.02, 0.02, 1e-10
+    H = OpList(N)
+    for i in range(1, N):
+        H.add(["z", "z"], [i, i + 1], -1.0)
+    for i in range(1, N + 1):
+        H.add("x", i, -g)
+    Jo = OpList(N)
+    for i in range(1, N + 1):
+        Jo.add("s-", i, gamma ** 0.5)
+    zs = OpList(N)
+    for i in range(1, N + 1):
+        zs.add("z", i)
+    st = spinhalf()
+    U = np.random.default_rng(1).random((ntraj, 4 * steps + 16))
+    kw = dict(observables=zs, cutoff=cutoff, maxdim=chi_max)
+    qjmc_trajectories(st, ["up"] * N, H, Jo, 2 * dt, dt, U[:8], engine="batched", **kw)  # warm-up
+    t0 = time.perf_counter()
+    full = qjmc_trajectories(st, ["up"] * N, H, Jo, steps * dt, dt, U, engine="batched", **kw)
+    el_full = time.perf_counter() - t0
+    print(f"qjmc critical: batched {ntraj} x {steps} in {el_full:.2f} s", file=sys.stderr, flush=True)
+    res_w = None
+    if sample_traj > 0:
+        t0 = time.perf_counter()
+        w = qjmc_trajectories(st, ["up"] * N, H, Jo, steps * dt, dt, U[:sample_traj], engine="workers",
+                              nworkers=nworkers, **kw)
+        el_w = time.perf_counter() - t0
+        res_w = {"seconds": el_w, "trajectories": sample_traj, "steps": steps, "workers": nworkers,
+                 "trajectory_steps_per_s": sample_traj * steps / el_w,
+                 "jump_records_equal_batched": w["jumps"] == full["jumps"][:sample_traj],
+                 "max_abs_meas_diff_vs_batched": float(np.abs(w["measurements"] -
+                                                              full["measurements"][:sample_traj]).max())}
+    cpu = None
+    if cpu_traj > 0:
+        from oracle.qjmc import qjmc_mps
+
+        def terms(ol):
+            return [(list(o), list(s_), c) for o, s_, c in zip(ol.ops, ol.sites, ol.coeffs)]
+
+        up = np.array([1.0, 0.0])
+        t0 = time.perf_counter()
+        same = True
+        for t in range(cpu_traj):
+            j, _, _, ms, chi = qjmc_mps([up] * N, terms(H), terms(Jo), N, dt, steps * dt, U[t],
+                                       obs=[("z", i) for i in range(1, N + 1)], cutoff=cutoff, maxdim=chi_max)
+            same = same and j == full["jumps"][t] and chi == full["maxbonddim"][t] and \
+                np.abs(ms.real - full["measurements"][t].real).max() <= 1e-8
+        el_cpu = time.perf_counter() - t0
+        cpu = {"seconds": el_cpu, "trajectories": cpu_traj, "steps": steps, "threads": os.cpu_count(),
+               "trajectory_steps_per_s": cpu_traj * steps / el_cpu,
+               "jump_records_bonds_and_sz_equal_batched": bool(same),
+               "sample": "oracle/qjmc.py qjmc_mps, the first trajectories of the workload"}
+    bonds = np.array(full["maxbonddim"])
+    rate = ntraj * steps / el_full
+    return {"batched": {"seconds": el_full, "trajectories": ntraj, "steps": steps, "engine": full["engine"],
+                        "trajectory_steps_per_s": rate, "jumps": int(sum(full["njumps"])),
+                        "maxbonddim_min_median_max": [int(bonds.min()), int(np.median(bonds)), int(bonds.max())]},
+            "workers_sample": res_w,
+            "cpu_port": cpu,
+            "batched_speedup_vs_workers": rate / res_w["trajectory_steps_per_s"] if res_w else None,
+            "batched_speedup_vs_cpu_port": rate / cpu["trajectory_steps_per_s"] if cpu else None}
+
+
 def tebd_tfim_n40(steps=80, dt=0.05, chi_max=256, cutoff=1e-10, cpu_steps=0):
     """Real-time TEBD of the critical TFIM quench, N = 40, g = 1, from all spins up:
     second-order Trotter (exp(-i dt H), 60 two-site gates per step), cutoff 1e-10, chi_max
@@ -693,6 +770,8 @@ def main(argv=None):
                     help="with --sweep2048-only: kernel-family times per pass (HIP events)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--tebd-only", action="store_true", help="run only the real-time TEBD leg")
+    ap.add_argument("--qjmc-critical-only", action="store_true",
+                    help="run only the critical-chain QJMC leg (bonds above 16, wide batched slots)")
     ap.add_argument("--qjmc-only", action="store_true",
                     help="run only the QJMC leg on the batched engine and the host loop (one host thread)")
     ap.add_argument("--bond-only", action="store_true",
@@ -744,11 +823,15 @@ def main(argv=None):
     if a.tebd_only:
         print(json.dumps({"tebd_tfim_N40": tebd_tfim_n40(cpu_steps=0 if a.no_cpu else 2)}))
         return 0
+    if a.qjmc_critical_only:
+        print(json.dumps({"qjmc_tfim_N40_critical": qjmc_tfim_n40_critical()}))
+        return
     if a.qjmc_only:
         print(json.dumps({"qjmc_tfim_N40": qjmc_tfim_n40(sample_traj=0)}))
         return 0
     if a.legs_only:
-        print(json.dumps({"gradient_tfim_N64": gradient_tfim_n64(), "qjmc_tfim_N40": qjmc_tfim_n40()}))
+        print(json.dumps({"gradient_tfim_N64": gradient_tfim_n64(), "qjmc_tfim_N40": qjmc_tfim_n40(),
+                          "qjmc_tfim_N40_critical": qjmc_tfim_n40_critical()}))
         return 0
     if a.sweep_only:
         sw = matvec_chi_sweep(ctx, ops, Tensor, torch, seed=a.seed)
@@ -856,6 +939,7 @@ def main(argv=None):
     if not a.no_extra_legs:
         extra["gradient_tfim_N64"] = gradient_tfim_n64()
         extra["qjmc_tfim_N40"] = qjmc_tfim_n40()
+        extra["qjmc_tfim_N40_critical"] = qjmc_tfim_n40_critical()
         extra["tebd_tfim_N40"] = tebd_tfim_n40(cpu_steps=0 if a.no_cpu else 2)
     cpu = None
     if not a.no_cpu:

@@ -333,14 +333,19 @@ int tnh_qjmc_run(tnh_ctx *ctx, const tnh_qjmc_problem *problem, int ntraj, const
 
 /* The same trajectories and outputs as tnh_qjmc_run (same algorithm and uniform order),
  * advanced together on ctx's stream: all site tensors of all trajectories stay resident in
- * fixed-capacity slots (bond dimension at most min(16, 32 / d)), and every launch applies
- * one primitive (centre move, two-site gate split, one-site operator, normalisation) to
- * every trajectory, one workgroup each, with the Gram eigensolve of each split in LDS.
- * Replaces the per-trajectory loop over qjmc_simulation (qjmc.jl:28-167) for the small bond
- * dimensions of dissipative chains. A trajectory whose split would keep more than the slot
- * capacity stops there: its maxbond entry is set to -1 and its other outputs are undefined,
- * while the other trajectories complete (the call returns TNH_OK); rerun the marked ones
- * with tnh_qjmc_run, which has no such limit, on their own rows of uniforms. */
+ * fixed-capacity slots, and every launch gives each trajectory one workgroup that applies its
+ * list of primitives (centre moves, two-site gate splits, one-site operators,
+ * normalisation). Narrow slots (bond dimension at most min(16, 32 / d)) keep each split's
+ * Gram eigensolve in LDS; trajectories that outgrow them are rerun inside this call, from the
+ * start on their own uniforms, on wide slots (bond dimension 64, then up to 512 / d, capped by
+ * maxdim and the chain's largest Schmidt rank): a 1024-thread workgroup per trajectory with
+ * the divide-and-conquer Gram eigensolve. Replaces the per-trajectory loop over
+ * qjmc_simulation (qjmc.jl:28-167) for dissipative chains with uneven bond dimensions. A
+ * trajectory whose split would keep more than the largest slot capacity (or whose
+ * divide and conquer fails) stops there: its maxbond entry is set to -1 and its other
+ * outputs are undefined, while the other trajectories complete (the call returns TNH_OK);
+ * rerun the marked ones with tnh_qjmc_run, which has no such limit (d <= 8), on their own rows
+ * of uniforms. */
 int tnh_qjmc_run_batched(tnh_ctx *ctx, const tnh_qjmc_problem *problem, int ntraj,
                          const double *uniforms, int64_t nuniform, double *meas, int *njumps,
                          int *jumps, int *jump_steps, int maxjumps, int *maxbond);

@@ -18,7 +18,8 @@
 // share a secular root, and sums over the group), `nthreads()` (the workgroup size, at most kDcThreads; small matrices
 // run on fewer threads, where a barrier costs less), and `mark(id)`, a phase boundary that a
 // timing policy can record. The
-// device policy (in the .hip file) maps these to threads, barriers and lane shuffles; a host
+// device policy (DcDeviceEx, at the end of this header in HIP builds) maps these to threads,
+// barriers and lane shuffles; a host
 // policy that runs the loops sequentially (G = 1) compiles the same code for tests on the
 // CPU. The merge products are the one place with a device-only path (matrix-core tiles);
 // the host runs the same sums as plain loops.
@@ -748,6 +749,69 @@ TNH_HD void dc_eig(const Ex &ex, DcShared &sh, double *Gr, double *Gi, int p, do
   ex.par(1, [&](int) { out[p] = sh.status ? 1.0 : 0.0; });
   ex.mark(63);
 }
+
+#ifdef __HIPCC__
+// Execution policy of dc_eig on one workgroup of NT threads. The reductions take one barrier
+// (the caller alternates two scratch buffers, so a fast thread's next write cannot overtake a
+// slow thread's read); secular-root groups are G consecutive lanes of one wave.
+template <int NT>
+struct DcDeviceEx {
+  template <class F> __device__ void par(int n, F &&f) const {
+#pragma unroll 4
+    for (int t = threadIdx.x; t < n; t += NT) f(t);
+  }
+  __device__ void sync() const { __syncthreads(); }
+  __device__ void mark(int) const {}
+  __device__ int nthreads() const { return NT; }
+  template <class F> __device__ double sum(int n, F &&f, double *red) const {
+    double s = 0;
+#pragma unroll 4
+    for (int t = threadIdx.x; t < n; t += NT) s += f(t);
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (NT == 64) return s;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    double r = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) r += red[w];
+    return r;
+  }
+  template <class F> __device__ double vmax(int n, F &&f, double *red) const {
+    double s = 0;
+    for (int t = threadIdx.x; t < n; t += NT) s = fmax(s, f(t));
+    for (int o = 32; o > 0; o >>= 1) s = fmax(s, __shfl_xor(s, o, 64));
+    if (NT == 64) return s;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    double r = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) r = fmax(r, red[w]);
+    return r;
+  }
+  // lanes per secular root: the largest power of two with p G <= NT, at most a wave
+  __device__ int gsize(int p) const {
+    int g = 1;
+    while (g < 64 && 2 * g * p <= NT) g *= 2;
+    return g;
+  }
+  __device__ double gsum(double x, int G) const {
+    for (int o = G >> 1; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+  }
+  // four sums over the group, their shuffle chains interleaved
+  __device__ void gsum4(double &a, double &b, double &c, double &d, int G) const {
+    for (int o = G >> 1; o > 0; o >>= 1) {
+      const double ta = __shfl_xor(a, o, 64), tb = __shfl_xor(b, o, 64);
+      const double tc = __shfl_xor(c, o, 64), td = __shfl_xor(d, o, 64);
+      a += ta;
+      b += tb;
+      c += tc;
+      d += td;
+    }
+  }
+};
+
+#endif  // __HIPCC__
 
 }  // namespace dc
 }  // namespace tnh

@@ -1,23 +1,30 @@
 // Batched quantum-jump trajectories: every trajectory of qjmc_simulation
 // (src/algorithms/mps/qjmc.jl:28-167, classical branch, single-site jump operators) advances
 // together, one workgroup per trajectory in each launch. The site tensors of all trajectories
-// live in fixed-capacity slots in HBM (bond dimension at most CH = 32 / d, so every Gram
-// matrix of a split is at most 32 x 32 and is diagonalised in LDS by the cyclic Jacobi of
-// jacobi_lds.hpp). The host keeps each trajectory's orthogonality centre and turns the
-// reference's step (applygates!, gatelist.jl:191-223; normalize!; the emission rates of
-// :170-220; the jump of :103-107) into a queue of primitives per trajectory: centre moves
-// (gmps.jl:90-112), two-site gates (gatelist.jl:137-171) and one-site operators. A round
-// launches one kernel per primitive kind over the trajectories whose queue front is of that
-// kind, so trajectories with different centres still advance in the same launches. Uniforms
-// are consumed per trajectory in the reference's order (one discarded draw, the jump test,
-// the channel). A split whose kept rank would exceed CH, or whose Jacobi does not converge,
-// stops the run with an error instead of truncating differently from the reference.
+// live in fixed-capacity slots in HBM. Narrow slots hold bond dimension CH <= 32 / d, so every
+// Gram matrix of a split is at most 32 x 32 and is diagonalised in LDS by the cyclic Jacobi
+// of jacobi_lds.hpp. Wide slots (CH <= 512 / d) give each trajectory a 1024-thread
+// workgroup, work matrices in HBM and the divide-and-conquer eigensolver of dc_eig.hpp; a
+// call runs every trajectory on narrow slots and reruns the ones that outgrew them, from the
+// start and on their own uniforms, on wide slots of bond dimension 64 and then of the largest
+// the chain and the truncation allow. The host keeps each trajectory's orthogonality centre
+// and turns the reference's step (applygates!, gatelist.jl:191-223; normalize!; the emission
+// rates of :170-220; the jump of :103-107) into a list of primitives per trajectory: centre
+// moves (gmps.jl:90-112), two-site gates (gatelist.jl:137-171) and one-site operators. A
+// launch gives every trajectory with work a workgroup that walks its own list, so
+// trajectories with different centres and bond dimensions advance in the same launch.
+// Uniforms are consumed per trajectory in the reference's order (one discarded draw, the
+// jump test, the channel). A split whose kept rank would exceed CH stops that trajectory
+// (rerun on the next slot size, or marked for the caller) instead of truncating differently
+// from the reference.
 #include <algorithm>
 #include <cmath>
 #include <complex>
 #include <deque>
+#include <numeric>
 #include <vector>
 
+#include "dc_eig.hpp"
 #include "jacobi_lds.hpp"
 #include "tnh_internal.hpp"
 
@@ -34,7 +41,9 @@ constexpr int QM = JP;     // largest matrix side of a split (32)
 constexpr int QMM = QM * QM;
 
 enum Prim { P_RIGHT = 0, P_LEFT = 1, P_GATE = 2, P_SITEOP = 3, P_NORM = 4 };
-enum Flag { F_OVERFLOW = 1, F_NOCONV = 2, F_NUMERIC = 4 };
+// F_FALLBACK: a wide-slot split whose divide and conquer failed; the trajectory is rerun on
+// the per-trajectory path, whose splits fall back to the Jacobi solvers.
+enum Flag { F_OVERFLOW = 1, F_NOCONV = 2, F_NUMERIC = 4, F_FALLBACK = 8 };
 
 struct Batch {
   double *S;        // site slots: [traj][site][re CAP | im CAP]
@@ -45,6 +54,9 @@ struct Batch {
   int64_t cap;      // CH * d * CH
   double cutoff;
   int maxdim, mindim;
+  double *scratch;  // wide slots: per-workgroup work matrices, sstride doubles each
+  int64_t sstride;
+  int P;            // wide slots: d * CH, the largest side of a split
 };
 
 __device__ inline double *slot(const Batch &b, int t, int site) {
@@ -661,6 +673,595 @@ __global__ void qb_init_kernel(Batch b, const double *__restrict__ init, int ntr
   if (threadIdx.x == 0) b.flags[tr] = 0;
 }
 
+// ---- wide slots: bond dimension up to kDcPMax / d, one 1024-thread workgroup per trajectory
+//
+// The same primitives as above with every matrix in HBM (per-workgroup work matrices of side
+// P = d CH) instead of LDS: products on the matrix cores (wg_zgemm), and the Gram eigensolve
+// of each split by the reduction and divide and conquer of dc_eig.hpp, which needs the whole
+// LDS of the workgroup. Above a bond dimension of 16 the eigensolve is most of a step, and it
+// is a one-workgroup algorithm either way: running one trajectory per workgroup fills the
+// GPU, where the per-trajectory path keeps one workgroup busy per host thread.
+
+constexpr int WT = dc::kDcThreads;  // threads per trajectory workgroup
+constexpr int WW = WT / 64;         // waves
+
+typedef double wv4d __attribute__((ext_vector_type(4)));  // an FP64 MFMA accumulator
+
+// work matrices of one workgroup: complex regions of 2 P^2 doubles (planes P^2 apart), real
+// regions of P^2
+struct WScr {
+  double *M, *T, *G;  // the matrix being split; theta, then eigenvectors; Gram, then reflectors
+  double *Q, *Q2;     // (real) divide-and-conquer work, then the basis and U^H U
+  double *U;          // divide-and-conquer work, then U X, then the right factor (complex)
+  double *taus, *dout, *val;
+  int *perm;
+  int64_t P2;
+};
+
+// doubles per workgroup for side P (a multiple of 32, so every region stays 256-byte aligned)
+__host__ __device__ inline int64_t wide_stride(int P) {
+  const int64_t P2 = int64_t(P) * P, P32 = (int64_t(P) + 32) & ~int64_t(31);
+  return ((10 * P2 + 31) & ~int64_t(31)) + 5 * P32;
+}
+
+__device__ inline WScr wscr(const Batch &b, int w) {
+  WScr s;
+  const int64_t P2 = int64_t(b.P) * b.P, P32 = (int64_t(b.P) + 32) & ~int64_t(31);
+  double *x = b.scratch + int64_t(w) * b.sstride;
+  s.P2 = P2;
+  s.M = x;
+  s.T = x + 2 * P2;
+  s.G = x + 4 * P2;
+  s.Q = x + 6 * P2;
+  s.Q2 = x + 7 * P2;
+  s.U = x + 8 * P2;
+  double *t = x + ((10 * P2 + 31) & ~int64_t(31));
+  s.taus = t;                // 2 P (real, then imaginary factors)
+  s.dout = t + 2 * P32;      // P + 1
+  s.val = t + 3 * P32;       // P
+  s.perm = reinterpret_cast<int *>(t + 4 * P32);  // P ints
+  return s;
+}
+
+// Operand of a workgroup product: element (i, l) at re[i * rs + l * cs] and im[...],
+// conjugated when conj is set.
+struct ZOp {
+  const double *re, *im;
+  int64_t rs, cs;
+  bool conj;
+};
+
+// C (M x N, column-major, leading dimension ldc, planes Cr and Ci) = A (M x K) B (K x N) by
+// the whole workgroup on the matrix cores: each wave takes 32 x 16 output tiles in turn; per
+// 16 x 16 half and k-step four v_mfma_f64_16x16x4f64 (re += ar br - ai bi, im += ar bi + ai br),
+// the operands read per lane from memory (L1/L2). Out-of-range rows, columns and k read the
+// operand base and are zeroed. Ends with a barrier.
+__device__ void wg_zgemm(int M, int N, int K, const ZOp &A, const ZOp &B, double *Cr, double *Ci, int64_t ldc) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, rr = lane & 15;
+  const int tm = (M + 31) / 32, tn = (N + 15) / 16;
+  const double sa = A.conj ? -1.0 : 1.0, sb = B.conj ? -1.0 : 1.0;
+  for (int tile = wave; tile < tm * tn; tile += WW) {
+    const int row0 = 32 * (tile % tm), col0 = 16 * (tile / tm);
+    wv4d accr[2], acci[2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+      accr[x] = wv4d{0.0, 0.0, 0.0, 0.0};
+      acci[x] = wv4d{0.0, 0.0, 0.0, 0.0};
+    }
+    const int c = col0 + rr;
+    for (int k0 = 0; k0 < K; k0 += 4) {
+      const int k = k0 + g;
+      const bool bon = k < K && c < N;
+      const int64_t boff = bon ? int64_t(k) * B.rs + int64_t(c) * B.cs : 0;
+      const double bvr = B.re[boff], bvi = B.im[boff];
+      const double br = bon ? bvr : 0.0, bi = bon ? sb * bvi : 0.0;
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        const int r = row0 + 16 * x + rr;
+        const bool on = k < K && r < M;
+        const int64_t off = on ? int64_t(r) * A.rs + int64_t(k) * A.cs : 0;
+        const double vr = A.re[off], vi = A.im[off];
+        const double ar = on ? vr : 0.0, ai = on ? sa * vi : 0.0;
+        accr[x] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, accr[x], 0, 0, 0);
+        accr[x] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, accr[x], 0, 0, 0);
+        acci[x] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, acci[x], 0, 0, 0);
+        acci[x] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, acci[x], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = row0 + 16 * x + g + 4 * i;
+        if (r < M && c < N) {
+          Cr[r + c * ldc] = accr[x][i];
+          Ci[r + c * ldc] = acci[x][i];
+        }
+      }
+  }
+  __syncthreads();
+}
+
+// V = H_0 ... H_{p-2} Z (as dc_backtransform_kernel of the SVD module, inside the
+// workgroup): one wave per column, lane l holding rows l, l + 64, ... (R of them) of it; each
+// reflector is a wave-wide shuffle reduction and an update. Ends with a barrier.
+template <int R>
+__device__ void wg_backtransform_r(const double *Gr, const double *Gi, int p, const double *taus, const double *Z,
+                                   double *Vr, double *Vi) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t P = p;
+  for (int c = wave; c < p; c += WW) {
+    double xr[R], xi[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int i = lane + 64 * r;
+      xr[r] = i < p ? Z[i + c * P] : 0.0;
+      xi[r] = 0.0;
+    }
+    for (int k = p - 2; k >= 0; --k) {
+      const double tr = taus[k], ti = taus[p + k];
+      if (tr == 0.0 && ti == 0.0) continue;
+      double sr = 0, si = 0;  // s = v^H x
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int i = lane + 64 * r;
+        const bool on = i > k && i < p;
+        const int64_t off = on ? i + k * P : 0;
+        const double a = Gr[off], b = Gi[off];
+        const double vr = on ? a : 0.0, vi = on ? b : 0.0;
+        sr += vr * xr[r] + vi * xi[r];
+        si += vr * xi[r] - vi * xr[r];
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        sr += __shfl_xor(sr, o, 64);
+        si += __shfl_xor(si, o, 64);
+      }
+      const double fr = tr * sr - ti * si, fi = tr * si + ti * sr;  // tau s
+#pragma unroll
+      for (int r = 0; r < R; ++r) {  // the reflector again (L1), not kept in registers
+        const int i = lane + 64 * r;
+        const bool on = i > k && i < p;
+        const int64_t off = on ? i + k * P : 0;
+        const double a = Gr[off], b = Gi[off];
+        const double vr = on ? a : 0.0, vi = on ? b : 0.0;
+        xr[r] -= vr * fr - vi * fi;
+        xi[r] -= vr * fi + vi * fr;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int i = lane + 64 * r;
+      if (i < p) {
+        Vr[i + c * P] = xr[r];
+        Vi[i + c * P] = xi[r];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+__device__ void wg_backtransform(const double *Gr, const double *Gi, int p, const double *taus, const double *Z,
+                                 double *Vr, double *Vi) {
+  if (p <= 128)
+    wg_backtransform_r<2>(Gr, Gi, p, taus, Z, Vr, Vi);
+  else
+    wg_backtransform_r<8>(Gr, Gi, p, taus, Z, Vr, Vi);
+}
+
+// workgroup sum of (a, b), returned to every thread (two barriers)
+__device__ void wg_sum2(double &a, double &b, double (*red)[WW]) {
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, 64);
+    b += __shfl_xor(b, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = a;
+    red[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  a = 0;
+  b = 0;
+  for (int w = 0; w < WW; ++w) {
+    a += red[0][w];
+    b += red[1][w];
+  }
+  __syncthreads();
+}
+
+struct WShared {
+  dc::DcShared dc;
+  double red[2][WW];
+  int k, flag;
+};
+
+// Truncated split of M (m x n, column-major, leading dimension m, planes Mr / Mi) with the
+// same steps and outcome as split_lds_w: Gram matrix of the basis side p, eigenvalues and
+// eigenvectors by dc_eig (reduction, divide and conquer, back-transformation), stable
+// descending order, the truncation rule, one Newton-Schulz step on the kept basis. On return
+// the left factor (m x k, leading dimension m) is in w.G and the right factor (k x n, leading
+// dimension k) in w.U, planes P^2 apart; returns k (0 on failure, with sh.flag set).
+__device__ int split_w(const Batch &b, WShared &sh, const WScr &w, const double *Mr, const double *Mi, int m, int n,
+                       int side, double cutoff, int maxdim, int mindim) {
+  const int t = threadIdx.x;
+  const int p = side == 0 ? m : n;
+  const int64_t P2 = w.P2;
+  // G = M M^H (side 0) or M^H M (side 1)
+  {
+    ZOp A, B;
+    if (side == 0) {
+      A = ZOp{Mr, Mi, 1, m, false};
+      B = ZOp{Mr, Mi, m, 1, true};
+    } else {
+      A = ZOp{Mr, Mi, m, 1, true};
+      B = ZOp{Mr, Mi, 1, m, false};
+    }
+    wg_zgemm(p, p, side == 0 ? n : m, A, B, w.G, w.G + P2, p);
+  }
+  dc::dc_eig<true>(dc::DcDeviceEx<WT>{}, sh.dc, w.G, w.G + P2, p, w.taus, w.taus + p, w.Q, w.Q2, w.U, w.dout);
+  __syncthreads();
+  if (w.dout[p] != 1.0) {
+    if (t == 0) sh.flag |= F_FALLBACK;
+    __syncthreads();
+    return 0;
+  }
+  // eigenvectors of G into w.T (the reflectors in w.G, the tridiagonal eigenvectors in w.Q)
+  wg_backtransform(w.G, w.G + P2, p, w.taus, w.Q, w.T, w.T + P2);
+  for (int i = t; i < p; i += WT) w.val[i] = sqrt(fmax(w.dout[i], 0.0));
+  __syncthreads();
+  for (int i = t; i < p; i += WT) {  // stable descending rank
+    const double v = w.val[i];
+    int rk = 0;
+    for (int j = 0; j < p; ++j) rk += (w.val[j] > v || (w.val[j] == v && j < i)) ? 1 : 0;
+    w.perm[rk] = i;
+  }
+  __syncthreads();
+  if (t == 0) {
+    // descending values into w.dout (no longer needed), then the rule
+    int ok = 1;
+    for (int i = 0; i < p; ++i) {
+      w.dout[i] = w.val[w.perm[i]];
+      if (!isfinite(w.dout[i])) ok = 0;
+    }
+    double disc = 0;
+    int k = ok ? trunc_rank(w.dout, min(m, n), cutoff, maxdim, mindim, &disc) : 0;
+    if (!ok) sh.flag |= F_NUMERIC;
+    if (ok && k > b.CH) {
+      sh.flag |= F_OVERFLOW;
+      k = 0;
+    }
+    sh.k = k;
+  }
+  __syncthreads();
+  const int k = sh.k;
+  if (k == 0) return 0;
+  // basis U (p x k) = columns perm[0..k) of V, into w.Q
+  double *Ur = w.Q, *Ui = w.Q + int64_t(p) * k;
+  for (int e = t; e < p * k; e += WT) {
+    const int r = e % p, j = e / p, c = w.perm[j];
+    Ur[e] = w.T[r + int64_t(p) * c];
+    Ui[e] = w.T[P2 + r + int64_t(p) * c];
+  }
+  __syncthreads();
+  // Newton-Schulz: X = U^H U (k x k, w.Q2), W = U X (p x k, w.U), U <- 1.5 U - 0.5 W
+  double *Xr = w.Q2, *Xi = w.Q2 + int64_t(k) * k;
+  wg_zgemm(k, k, p, ZOp{Ur, Ui, p, 1, true}, ZOp{Ur, Ui, 1, p, false}, Xr, Xi, k);
+  wg_zgemm(p, k, k, ZOp{Ur, Ui, 1, p, false}, ZOp{Xr, Xi, 1, k, false}, w.U, w.U + P2, p);
+  for (int e = t; e < p * k; e += WT) {
+    Ur[e] = 1.5 * Ur[e] - 0.5 * w.U[e];
+    Ui[e] = 1.5 * Ui[e] - 0.5 * w.U[P2 + e];
+  }
+  __syncthreads();
+  if (side == 0) {
+    // left factor U (m x k); right factor U^H M (k x n)
+    for (int e = t; e < m * k; e += WT) {
+      w.G[e] = Ur[e];
+      w.G[P2 + e] = Ui[e];
+    }
+    wg_zgemm(k, n, m, ZOp{Ur, Ui, m, 1, true}, ZOp{Mr, Mi, 1, m, false}, w.U, w.U + P2, k);
+  } else {
+    // right factor V_k^H (k x n); left factor M V_k (m x k)
+    for (int e = t; e < k * n; e += WT) {
+      const int j = e % k, c = e / k;
+      w.U[e] = Ur[c + int64_t(n) * j];
+      w.U[P2 + e] = -Ui[c + int64_t(n) * j];
+    }
+    wg_zgemm(m, k, n, ZOp{Mr, Mi, 1, m, false}, ZOp{Ur, Ui, 1, n, false}, w.G, w.G + P2, m);
+  }
+  return k;
+}
+
+__device__ inline void copy_to_slot(double *dst, int64_t cap, int numel, const double *src, int64_t plane) {
+  for (int e = threadIdx.x; e < numel; e += WT) {
+    dst[e] = src[e];
+    dst[cap + e] = src[plane + e];
+  }
+}
+
+// The split primitives (two-site gate, centre moves) in three parts around one split_w call
+// site, so the kernel holds a single copy of the eigensolver. split_setup prepares the matrix
+// to split and returns its description; split_finish writes the factors back.
+struct WSplit {
+  const double *Mr, *Mi;
+  int m, n, side;
+  double cutoff;
+  int maxdim, mindim;
+};
+
+// two-site gate at (s, s + 1), as prim_gate: theta = A_s A_{s+1}, the gate applied into w.M
+__device__ WSplit gate_setup(const Batch &b, const WScr &w, int tr, int s, const double *Gop, int dir) {
+  const int d = b.d;
+  const int *bd = b.bond + int64_t(tr) * (b.N + 1);
+  const int Dl = bd[s - 1], Dm = bd[s], Dr = bd[s + 1];
+  const double *sa = slot(b, tr, s), *sb = slot(b, tr, s + 1);
+  const int m = Dl * d, n = d * Dr;
+  const int64_t P2 = w.P2;
+  // theta = A_s (m x Dm) A_{s+1} (Dm x n), into w.T
+  wg_zgemm(m, n, Dm, ZOp{sa, sa + b.cap, 1, m, false}, ZOp{sb, sb + b.cap, 1, Dm, false}, w.T, w.T + P2, m);
+  // M[(a,o1),(o2,b)] = sum_{i1,i2} G[o1,i1,o2,i2] theta[(a,i1),(i2,b)]
+  for (int e = threadIdx.x; e < m * n; e += WT) {
+    const int r = e % m, c = e / m;
+    const int a = r % Dl, o1 = r / Dl, o2 = c % d, bb = c / d;
+    double sr = 0, si = 0;
+    for (int i2 = 0; i2 < d; ++i2)
+      for (int i1 = 0; i1 < d; ++i1) {
+        const int64_t q = o1 + int64_t(d) * (i1 + int64_t(d) * (o2 + int64_t(d) * i2));
+        const double gr_ = Gop[2 * q], gi_ = Gop[2 * q + 1];
+        const int64_t te = (a + Dl * i1) + int64_t(m) * (i2 + d * bb);
+        const double xr = w.T[te], xi = w.T[P2 + te];
+        sr += gr_ * xr - gi_ * xi;
+        si += gr_ * xi + gi_ * xr;
+      }
+    w.M[e] = sr;
+    w.M[P2 + e] = si;
+  }
+  __syncthreads();
+  return WSplit{w.M, w.M + P2, m, n, dir ? 1 : 0, b.cutoff, b.maxdim, b.mindim};
+}
+
+// centre move by one site (dir 0: s -> s + 1, dir 1: s -> s - 1), as prim_move: A_s is split
+// in place
+__device__ WSplit move_setup(const Batch &b, int tr, int s, int dir, int trunc) {
+  const int d = b.d;
+  const int *bd = b.bond + int64_t(tr) * (b.N + 1);
+  const int Dl = bd[s - 1], Dr = bd[s];
+  const double *sa = slot(b, tr, s);
+  const int m = dir == 0 ? Dl * d : Dl, n = dir == 0 ? Dr : d * Dr;
+  return WSplit{sa, sa + b.cap, m, n, dir, trunc ? b.cutoff : 0.0, trunc ? b.maxdim : 0, trunc ? b.mindim : 1};
+}
+
+// the kept factors (k > 0) back into the slots: left factor in w.G, right factor in w.U
+__device__ void split_finish(const Batch &b, const WScr &w, int tr, int kind, int s, const WSplit &q, int k) {
+  const int d = b.d;
+  int *bd = b.bond + int64_t(tr) * (b.N + 1);
+  double *sa = slot(b, tr, s);
+  const int64_t P2 = w.P2;
+  const int m = q.m, n = q.n;
+  if (kind == P_GATE) {
+    double *sb = slot(b, tr, s + 1);
+    copy_to_slot(sa, b.cap, m * k, w.G, P2);
+    copy_to_slot(sb, b.cap, k * n, w.U, P2);
+    if (threadIdx.x == 0) bd[s] = k;
+  } else if (kind == P_RIGHT) {
+    // A_{s+1} <- (U^H A_s) (k x Dr) A_{s+1} (Dr x d Dr2)
+    const int Dr = bd[s], Dr2 = bd[s + 1];
+    double *sn = slot(b, tr, s + 1);
+    const int nn = d * Dr2;
+    wg_zgemm(k, nn, Dr, ZOp{w.U, w.U + P2, 1, k, false}, ZOp{sn, sn + b.cap, 1, Dr, false}, w.T, w.T + P2, k);
+    copy_to_slot(sa, b.cap, m * k, w.G, P2);
+    copy_to_slot(sn, b.cap, k * nn, w.T, P2);
+    if (threadIdx.x == 0) bd[s] = k;
+  } else {
+    // A_{s-1} <- A_{s-1} (Dl2 d x Dl) (A_s V_k) (Dl x k)
+    const int Dl = bd[s - 1], Dl2 = bd[s - 2];
+    double *sp = slot(b, tr, s - 1);
+    const int mm = Dl2 * d;
+    wg_zgemm(mm, k, Dl, ZOp{sp, sp + b.cap, 1, mm, false}, ZOp{w.G, w.G + P2, 1, m, false}, w.T, w.T + P2, mm);
+    copy_to_slot(sa, b.cap, k * n, w.U, P2);
+    copy_to_slot(sp, b.cap, mm * k, w.T, P2);
+    if (threadIdx.x == 0) bd[s - 1] = k;
+  }
+  __syncthreads();
+}
+
+// A[a,o,b] <- sum_i O[o,i] A[a,i,b], in place: each thread owns the d entries of its (a, b)
+__device__ void prim_siteop_w(const Batch &b, int tr, int s, const double *O) {
+  const int d = b.d;
+  const int *bd = b.bond + int64_t(tr) * (b.N + 1);
+  const int Dl = bd[s - 1], Dr = bd[s];
+  double *sa = slot(b, tr, s);
+  for (int e = threadIdx.x; e < Dl * Dr; e += WT) {
+    const int a = e % Dl, bb = e / Dl;
+    double xr[QM / 2], xi[QM / 2];
+    for (int i = 0; i < d; ++i) {
+      const int64_t x = a + int64_t(Dl) * (i + int64_t(d) * bb);
+      xr[i] = sa[x];
+      xi[i] = sa[b.cap + x];
+    }
+    for (int o = 0; o < d; ++o) {
+      double sr = 0, si = 0;
+      for (int i = 0; i < d; ++i) {
+        const double gr_ = O[2 * (o + d * i)], gi_ = O[2 * (o + d * i) + 1];
+        sr += gr_ * xr[i] - gi_ * xi[i];
+        si += gr_ * xi[i] + gi_ * xr[i];
+      }
+      const int64_t x = a + int64_t(Dl) * (o + int64_t(d) * bb);
+      sa[x] = sr;
+      sa[b.cap + x] = si;
+    }
+  }
+  __syncthreads();
+}
+
+// A_c <- A_c / ||A_c||; false on a zero or non-finite norm
+__device__ bool prim_normalize_w(const Batch &b, WShared &sh, int tr, int s) {
+  const int *bd = b.bond + int64_t(tr) * (b.N + 1);
+  const int ne = bd[s - 1] * b.d * bd[s];
+  double *sa = slot(b, tr, s);
+  double acc = 0, dummy = 0;
+  for (int e = threadIdx.x; e < ne; e += WT) acc += sa[e] * sa[e] + sa[b.cap + e] * sa[b.cap + e];
+  wg_sum2(acc, dummy, sh.red);
+  const double nr = sqrt(acc);
+  if (!(nr > 0) || !isfinite(nr)) return false;
+  const double inv = 1.0 / nr;
+  for (int e = threadIdx.x; e < ne; e += WT) {
+    sa[e] *= inv;
+    sa[b.cap + e] *= inv;
+  }
+  __syncthreads();
+  return true;
+}
+
+// qb_run_kernel on wide slots: one 1024-thread workgroup per listed trajectory, work matrices
+// at b.scratch + (list index) * b.sstride
+__global__ __launch_bounds__(WT) void qw_run_kernel(Batch b, const int *__restrict__ trajs,
+                                                    const int *__restrict__ start, const int *__restrict__ items) {
+  __shared__ WShared sh;
+  const int w = blockIdx.x, tr = trajs[w], t = threadIdx.x;
+  const WScr ws = wscr(b, w);
+  if (t == 0) sh.flag = b.flags[tr];
+  __syncthreads();
+  if (sh.flag) return;
+  for (int i = start[w]; i < start[w + 1]; ++i) {
+    const int *it = items + 4 * i;
+    const int kind = it[0], s = it[1], arg = it[2], fl = it[3];
+    if (kind == P_SITEOP) {
+      prim_siteop_w(b, tr, s, b.op + 2 * int64_t(arg));
+    } else if (kind == P_NORM) {
+      if (!prim_normalize_w(b, sh, tr, s) && t == 0) sh.flag |= F_NUMERIC;
+    } else {
+      const WSplit q = kind == P_GATE ? gate_setup(b, ws, tr, s, b.op + 2 * int64_t(arg), fl)
+                                      : move_setup(b, tr, s, kind == P_RIGHT ? 0 : 1, fl);
+      const int k = split_w(b, sh, ws, q.Mr, q.Mi, q.m, q.n, q.side, q.cutoff, q.maxdim, q.mindim);
+      if (k > 0) split_finish(b, ws, tr, kind, s, q, k);
+    }
+    __syncthreads();
+    if (sh.flag) {
+      if (t == 0) b.flags[tr] = sh.flag;
+      return;
+    }
+  }
+}
+
+// sum over the centre-form tensor of the term at site j: val = sum_{x,sg,b} conj(A[x,sg,b])
+// sum_tau O[sg,tau] W[x,tau,b], A and W of shape Dl x d x Dr
+__device__ void term_value(const double *A, int64_t capA, const double *W, int64_t capW, int Dl, int d, int Dr,
+                           const double *O, double (*red)[WW], double *out) {
+  const int ne = Dl * d * Dr;
+  double vr = 0, vi = 0;
+  for (int e = threadIdx.x; e < ne; e += WT) {
+    const int x = e % Dl, sg = (e / Dl) % d, bb = e / (Dl * d);
+    double pr = 0, pi = 0;
+    for (int tau = 0; tau < d; ++tau) {
+      const double gr_ = O[2 * (sg + d * tau)], gi_ = O[2 * (sg + d * tau) + 1];
+      const int64_t it = x + int64_t(Dl) * (tau + int64_t(d) * bb);
+      pr += gr_ * W[it] - gi_ * W[capW + it];
+      pi += gr_ * W[capW + it] + gi_ * W[it];
+    }
+    const double ar = A[e], ai = -A[capA + e];
+    vr += ar * pr - ai * pi;
+    vi += ar * pi + ai * pr;
+  }
+  wg_sum2(vr, vi, red);
+  if (threadIdx.x == 0) {
+    out[0] = vr;
+    out[1] = vi;
+  }
+}
+
+// <psi|O|psi> for every (site, operator) term of every listed trajectory, from the mixed
+// canonical form that the step loop keeps (centre c = centre[list index], sites left of it
+// left-orthonormal, right of it right-orthonormal): the terms at c need only the centre
+// tensor, those right of c the chain L_j = A_j^H (L_{j-1} A_j) started from L_c = A_c^H A_c,
+// those left of c the chain R_j = (A_j R_{j+1}) A_j^H started from R_c = A_c A_c^H. One
+// environment is live at a time. Centre 0 (the initial product state, every bond 1) takes
+// the product of the site norms. out as qb_expect_kernel.
+__global__ __launch_bounds__(WT) void qw_expect_kernel(Batch b, const int *__restrict__ trajs,
+                                                       const int *__restrict__ centre, int nterms,
+                                                       const int *__restrict__ term_site,
+                                                       const int64_t *__restrict__ term_off, double *__restrict__ out) {
+  __shared__ double red[2][WW];
+  const int w = blockIdx.x, tr = trajs[w], d = b.d, N = b.N, c = centre[w];
+  const int *bd = b.bond + int64_t(tr) * (N + 1);
+  const WScr ws = wscr(b, w);
+  const int64_t P2 = ws.P2;
+  double *o = out + 2 * int64_t(w) * nterms;
+  if (c == 0) {
+    if (threadIdx.x == 0) {
+      bool prod = true;
+      for (int j = 0; j <= N; ++j) prod = prod && bd[j] == 1;
+      for (int q = 0; q < nterms; ++q) {
+        const int s = term_site[q];
+        const double *O = b.op + 2 * term_off[q];
+        double nr = 1.0;
+        for (int j = 1; j <= N; ++j) {
+          if (j == s) continue;
+          const double *a = slot(b, tr, j);
+          double x = 0;
+          for (int i = 0; i < d; ++i) x += a[i] * a[i] + a[b.cap + i] * a[b.cap + i];
+          nr *= x;
+        }
+        const double *a = slot(b, tr, s);
+        double vr = 0, vi = 0;
+        for (int sg = 0; sg < d; ++sg)
+          for (int tau = 0; tau < d; ++tau) {
+            const double gr_ = O[2 * (sg + d * tau)], gi_ = O[2 * (sg + d * tau) + 1];
+            const double xr = a[sg], xi = -a[b.cap + sg], yr = a[tau], yi = a[b.cap + tau];
+            const double pr = gr_ * yr - gi_ * yi, pi = gr_ * yi + gi_ * yr;
+            vr += xr * pr - xi * pi;
+            vi += xr * pi + xi * pr;
+          }
+        o[2 * q] = prod ? nr * vr : NAN;
+        o[2 * q + 1] = prod ? nr * vi : NAN;
+      }
+    }
+    return;
+  }
+  int smin = c, smax = c;
+  for (int q = 0; q < nterms; ++q) {
+    smin = min(smin, term_site[q]);
+    smax = max(smax, term_site[q]);
+  }
+  double *Er = ws.M, *Ei = ws.M + P2;  // the live environment
+  double *Wr = ws.T, *Wi = ws.T + P2;
+  // sites c .. smax
+  for (int j = c; j <= smax; ++j) {
+    const int Dl = bd[j - 1], Dr = bd[j], m = Dl * d;
+    const double *A = slot(b, tr, j);
+    const double *W = A;
+    int64_t capW = b.cap;
+    if (j > c) {  // W = L_{j-1} (Dl x Dl) A_j (Dl x d Dr)
+      wg_zgemm(Dl, d * Dr, Dl, ZOp{Er, Ei, 1, Dl, false}, ZOp{A, A + b.cap, 1, Dl, false}, Wr, Wi, Dl);
+      W = Wr;
+      capW = P2;
+    }
+    for (int q = 0; q < nterms; ++q)
+      if (term_site[q] == j)
+        term_value(A, b.cap, W, capW, Dl, d, Dr, b.op + 2 * term_off[q], red, o + 2 * q);
+    if (j < smax)  // L_j = A_j^H W (both (Dl d) x Dr)
+      wg_zgemm(Dr, Dr, m, ZOp{A, A + b.cap, m, 1, true}, ZOp{W, W + capW, 1, m, false}, Er, Ei, Dr);
+  }
+  // sites c - 1 .. smin
+  if (smin < c) {
+    {  // R_c = A_c (Dl x d Dr) A_c^H
+      const int Dl = bd[c - 1], Dr = bd[c];
+      const double *A = slot(b, tr, c);
+      wg_zgemm(Dl, Dl, d * Dr, ZOp{A, A + b.cap, 1, Dl, false}, ZOp{A, A + b.cap, Dl, 1, true}, Er, Ei, Dl);
+    }
+    for (int j = c - 1; j >= smin; --j) {
+      const int Dl = bd[j - 1], Dr = bd[j], m = Dl * d;
+      const double *A = slot(b, tr, j);
+      // W = A_j ((Dl d) x Dr) R_{j+1} (Dr x Dr)
+      wg_zgemm(m, Dr, Dr, ZOp{A, A + b.cap, 1, m, false}, ZOp{Er, Ei, 1, Dr, false}, Wr, Wi, m);
+      for (int q = 0; q < nterms; ++q)
+        if (term_site[q] == j)
+          term_value(A, b.cap, Wr, P2, Dl, d, Dr, b.op + 2 * term_off[q], red, o + 2 * q);
+      if (j > smin)  // R_j = W (Dl x d Dr) A_j^H
+        wg_zgemm(Dl, Dl, d * Dr, ZOp{Wr, Wi, 1, Dl, false}, ZOp{A, A + b.cap, Dl, 1, true}, Er, Ei, Dl);
+    }
+  }
+}
+
 struct Item {
   int kind, site, arg, flag;
 };
@@ -675,6 +1276,7 @@ struct Engine {
   tnh_ctx *ctx;
   const tnh_qjmc_problem *p;
   int T;
+  bool wide = false;  // wide slots: qw_* kernels
   Batch b{};
   std::vector<TrajQ> tq;
   std::vector<int64_t> gate_off;  // operator-store offsets (complex elements) of the gates
@@ -756,7 +1358,10 @@ struct Engine {
     }
     to_device(ctx, ops_dev, buf.data(), bytes);
     const int *od = static_cast<const int *>(ops_dev);
-    hipLaunchKernelGGL(qb_run_kernel, dim3(nw), dim3(QT), 0, ctx->stream, b, od, od + nw, od + 2 * nw + 1);
+    if (wide)
+      hipLaunchKernelGGL(qw_run_kernel, dim3(nw), dim3(WT), 0, ctx->stream, b, od, od + nw, od + 2 * nw + 1);
+    else
+      hipLaunchKernelGGL(qb_run_kernel, dim3(nw), dim3(QT), 0, ctx->stream, b, od, od + nw, od + 2 * nw + 1);
     TNH_HIP(hipGetLastError());
   }
 };
@@ -767,7 +1372,7 @@ void check_flags(tnh_ctx *ctx, const Engine &E, std::vector<char> &dead) {
   std::vector<int> f(E.T);
   to_host(ctx, f.data(), E.b.flags, sizeof(int) * E.T);
   for (int t = 0; t < E.T; ++t) {
-    if (f[t] & F_OVERFLOW) dead[t] = 1;
+    if (f[t] & (F_OVERFLOW | F_FALLBACK)) dead[t] = 1;
     TNH_REQUIRE(!(f[t] & F_NOCONV), TNH_ERR_NOCONV,
                 "qjmc batched: Gram eigensolve did not converge (trajectory " + std::to_string(t) + ")");
     TNH_REQUIRE(!(f[t] & F_NUMERIC), TNH_ERR_NUMERIC,
@@ -775,20 +1380,256 @@ void check_flags(tnh_ctx *ctx, const Engine &E, std::vector<char> &dead) {
   }
 }
 
+
+// Device buffers of one engine run, returned to the context's stream-ordered cache at the end
+// (also when the run throws): later work on the stream reuses them only after this run's
+// kernels.
+struct Owned {
+  tnh_ctx *ctx;
+  std::vector<std::pair<void *, size_t>> v;
+  void *take(size_t bytes) {
+    size_t cap = 0;
+    void *q = ctx->alloc(bytes, &cap);
+    v.emplace_back(q, cap);
+    return q;
+  }
+  ~Owned() {
+    for (auto &x : v) ctx->release(x.first, x.second);
+  }
+};
+
+// The step loop of qjmc_simulation over the trajectories `rows` (their indices into the
+// caller's uniform and output arrays) on slots of bond dimension CH: narrow slots (the whole
+// state of a split in LDS, CH <= 32 / d) or wide ones (work matrices in HBM, d CH <= 512). A
+// trajectory that outgrows its slots (or whose wide split needs the general solvers) stops
+// with maxbond[row] = -1; the others complete.
+void run_engine(tnh_ctx *ctx, const tnh_qjmc_problem *p, const std::vector<int> &rows, const double *uniforms,
+                int64_t nuniform, double *meas, int *njumps, int *jumps, int *jump_steps, int maxjumps, int *maxbond,
+                int CH, bool wide) {
+  const int N = p->N, d = p->d, dd = d * d, ntraj = int(rows.size());
+  Owned own{ctx, {}};
+  Engine E;
+  E.ctx = ctx;
+  E.p = p;
+  E.T = ntraj;
+  E.wide = wide;
+  E.tq.resize(ntraj);
+  struct OpsGuard {
+    Engine &e;
+    ~OpsGuard() {
+      if (e.ops_dev) e.ctx->release(e.ops_dev, e.ops_cap);
+    }
+  } ops_guard{E};
+  // operator store: gates, jump operators, escape operators c-free L^H L, observables
+  int ng = 0;
+  for (int l = 0; l < p->nlayers; ++l) ng += p->layer_count[l];
+  std::vector<double> ops;
+  auto add_op = [&](const double *src, int64_t n) {
+    const int64_t off = int64_t(ops.size() / 2);
+    ops.insert(ops.end(), src, src + 2 * n);
+    return off;
+  };
+  for (int g = 0; g < ng; ++g)
+    E.gate_off.push_back(add_op(p->gate_data + 2 * p->gate_offset[g], p->gate_range[g] == 2 ? dd * dd : dd));
+  std::vector<double> esc(2 * dd);
+  for (int t = 0; t < p->njump; ++t) {
+    const double *L = p->jump_op + 2 * size_t(dd) * t;
+    E.jump_off.push_back(add_op(L, dd));
+    for (int j = 0; j < d; ++j)
+      for (int i = 0; i < d; ++i) {  // (L^H L)[i, j] = sum_k conj(L[k, i]) L[k, j]
+        std::complex<double> s = 0;
+        for (int k = 0; k < d; ++k)
+          s += std::conj(std::complex<double>(L[2 * (k + d * i)], L[2 * (k + d * i) + 1])) *
+               std::complex<double>(L[2 * (k + d * j)], L[2 * (k + d * j) + 1]);
+        esc[2 * (i + d * j)] = s.real();
+        esc[2 * (i + d * j) + 1] = s.imag();
+      }
+    E.esc_off.push_back(add_op(esc.data(), dd));
+  }
+  for (int o = 0; o < p->nobs; ++o) E.obs_off.push_back(add_op(p->obs_op + 2 * size_t(dd) * o, dd));
+  TNH_REQUIRE(ops.size() / 2 < size_t(1) << 30, TNH_ERR_ARG, "qjmc batched: operator store too large");
+  double *op_dev = static_cast<double *>(own.take(sizeof(double) * std::max<size_t>(ops.size(), 2)));
+  to_device(ctx, op_dev, ops.data(), sizeof(double) * ops.size());
+  // state
+  E.b.N = N;
+  E.b.d = d;
+  E.b.CH = CH;
+  E.b.cap = int64_t(CH) * d * CH;
+  E.b.cutoff = p->cutoff;
+  E.b.maxdim = p->maxdim;
+  E.b.mindim = p->mindim;
+  E.b.op = op_dev;
+  E.b.S = static_cast<double *>(own.take(sizeof(double) * size_t(ntraj) * N * 2 * E.b.cap));
+  E.b.bond = static_cast<int *>(own.take(sizeof(int) * size_t(ntraj) * (N + 1)));
+  E.b.flags = static_cast<int *>(own.take(sizeof(int) * size_t(ntraj)));
+  if (wide) {
+    E.b.P = d * CH;
+    E.b.sstride = wide_stride(E.b.P);
+    E.b.scratch = static_cast<double *>(own.take(sizeof(double) * size_t(ntraj) * E.b.sstride));
+  }
+  double *init_dev = static_cast<double *>(own.take(sizeof(double) * 2 * N * d));
+  to_device(ctx, init_dev, p->init_state, sizeof(double) * 2 * N * d);
+  hipLaunchKernelGGL(qb_init_kernel, dim3(ntraj), dim3(64), 0, ctx->stream, E.b, init_dev, ntraj);
+  TNH_HIP(hipGetLastError());
+  // expectation terms: escape operators, then observables
+  const int nesc = p->njump, nobs = p->nobs;
+  std::vector<int> tsite;
+  std::vector<int64_t> toff;
+  for (int t = 0; t < nesc; ++t) {
+    tsite.push_back(p->jump_site[t]);
+    toff.push_back(E.esc_off[t]);
+  }
+  for (int o = 0; o < nobs; ++o) {
+    tsite.push_back(p->obs_site[o]);
+    toff.push_back(E.obs_off[o]);
+  }
+  const int nterm = int(tsite.size());
+  int *tsite_dev = static_cast<int *>(own.take(sizeof(int) * std::max(1, nterm)));
+  int64_t *toff_dev = static_cast<int64_t *>(own.take(sizeof(int64_t) * std::max(1, nterm)));
+  if (nterm) {
+    to_device(ctx, tsite_dev, tsite.data(), sizeof(int) * nterm);
+    to_device(ctx, toff_dev, toff.data(), sizeof(int64_t) * nterm);
+  }
+  std::vector<int> all(ntraj);
+  for (int t = 0; t < ntraj; ++t) all[t] = t;
+  int *all_dev = static_cast<int *>(own.take(sizeof(int) * ntraj));
+  to_device(ctx, all_dev, all.data(), sizeof(int) * ntraj);
+  double *env = nullptr;
+  int *centre_dev = nullptr;
+  if (wide)
+    centre_dev = static_cast<int *>(own.take(sizeof(int) * ntraj));
+  else
+    env = static_cast<double *>(own.take(sizeof(double) * size_t(ntraj) * (N + 1) * 2 * CH * CH));
+  double *outd = static_cast<double *>(own.take(sizeof(double) * 2 * size_t(ntraj) * std::max(1, nterm)));
+  std::vector<double> outh(2 * size_t(ntraj) * std::max(1, nterm));
+  std::vector<int> centres(ntraj);
+  // all terms of all trajectories in one launch; terms [q0, q0 + nq) are read back
+  auto expect = [&](int q0, int nq) {
+    if (wide) {
+      for (int t = 0; t < ntraj; ++t) centres[t] = E.tq[t].center;
+      to_device(ctx, centre_dev, centres.data(), sizeof(int) * ntraj);
+      hipLaunchKernelGGL(qw_expect_kernel, dim3(ntraj), dim3(WT), 0, ctx->stream, E.b, all_dev, centre_dev, nq,
+                         tsite_dev + q0, toff_dev + q0, outd);
+    } else {
+      hipLaunchKernelGGL(qb_expect_kernel, dim3(ntraj), dim3(QT), 0, ctx->stream, E.b, all_dev, nq, tsite_dev + q0,
+                         toff_dev + q0, env, outd);
+    }
+    TNH_HIP(hipGetLastError());
+    to_host(ctx, outh.data(), outd, sizeof(double) * 2 * size_t(ntraj) * nq);
+  };
+  const int nsave = p->nsteps / p->savesteps + 1;
+  std::vector<int64_t> pos(ntraj, 0);
+  std::vector<int> save(ntraj, 0), nj(ntraj, 0);
+  auto draw = [&](int t) {
+    TNH_REQUIRE(pos[t] < nuniform, TNH_ERR_ARG, "qjmc: not enough uniforms for a trajectory");
+    return uniforms[size_t(rows[t]) * nuniform + pos[t]++];
+  };
+  auto measure = [&]() {
+    if (nobs > 0) {
+      expect(nesc, nobs);
+      for (int t = 0; t < ntraj; ++t)
+        for (int o = 0; o < nobs; ++o) {
+          double *m = meas + (size_t(rows[t]) * nsave + save[t]) * 2 * nobs;
+          m[2 * o] = outh[2 * (size_t(t) * nobs + o)];
+          m[2 * o + 1] = outh[2 * (size_t(t) * nobs + o) + 1];
+        }
+    }
+    for (int t = 0; t < ntraj; ++t) ++save[t];
+  };
+  auto normalize_all = [&](const std::vector<int> &which) {
+    for (int t : which) {
+      auto &x = E.tq[t];
+      if (x.center == 0) E.push_move(x, 1, 0);  // norm(): gmps.jl:29-37
+      x.q.push_back({P_NORM, x.center, 0, 0});
+    }
+    E.run_queues();
+  };
+  measure();
+  std::vector<double> rates(std::max(1, nesc));
+  std::vector<char> dead(ntraj, 0);  // trajectories that outgrew their slots
+  for (int step = 1; step <= p->nsteps; ++step) {
+    for (int t = 0; t < ntraj; ++t)
+      if (!dead[t]) E.push_applygates(E.tq[t]);
+    E.run_queues();
+    std::vector<int> live;
+    for (int t = 0; t < ntraj; ++t)
+      if (!dead[t]) live.push_back(t);
+    for (int t : live) (void)draw(t);  // qjmc.jl:64: drawn and unused
+    normalize_all(live);
+    check_flags(ctx, E, dead);
+    std::vector<int> jumpers;
+    if (nesc > 0) {
+      expect(0, nesc);
+      for (int t = 0; t < ntraj; ++t) {
+        if (dead[t]) continue;
+        double er = 0;
+        for (int q = 0; q < nesc; ++q) {
+          const double c = p->jump_coeff[q];
+          const double *v = &outh[2 * (size_t(t) * nesc + q)];
+          rates[q] = std::abs(std::complex<double>(c * c * v[0], c * c * v[1]));
+          er += rates[q];
+        }
+        const double prob = std::exp(-er * p->dt);
+        if (draw(t) > prob) {
+          const double r = draw(t);
+          int idx = -1;
+          double cum = 0;
+          for (int q = 0; q < nesc && idx < 0; ++q) {
+            cum += rates[q];
+            if (r < cum / er) idx = q;
+          }
+          TNH_REQUIRE(idx >= 0, TNH_ERR_NUMERIC, "qjmc: no jump channel selected");
+          auto &x = E.tq[t];
+          E.push_move(x, 1, 0);
+          x.q.push_back({P_SITEOP, p->jump_site[idx], int(E.jump_off[idx]), 0});
+          E.push_move(x, N, 0);
+          E.push_move(x, 1, 1);
+          jumpers.push_back(t);
+          const size_t row = size_t(rows[t]);
+          if (nj[t] < maxjumps) {
+            jumps[row * maxjumps + nj[t]] = idx + 1;
+            jump_steps[row * maxjumps + nj[t]] = step;
+          }
+          ++nj[t];
+        }
+      }
+    } else {
+      for (int t = 0; t < ntraj; ++t)
+        if (!dead[t]) (void)draw(t);  // the jump test draws even with no channels
+    }
+    if (!jumpers.empty()) {
+      E.run_queues();
+      normalize_all(jumpers);
+    }
+    if (step % p->savesteps == 0) measure();
+  }
+  check_flags(ctx, E, dead);
+  std::vector<int> bonds(size_t(ntraj) * (N + 1));
+  to_host(ctx, bonds.data(), E.b.bond, sizeof(int) * bonds.size());
+  for (int t = 0; t < ntraj; ++t) {
+    njumps[rows[t]] = dead[t] ? 0 : nj[t];
+    int mb = 0;
+    for (int i = 1; i < N; ++i) mb = std::max(mb, bonds[size_t(t) * (N + 1) + i]);
+    maxbond[rows[t]] = dead[t] ? -1 : mb;
+  }
+}
+
+// device memory of the wide reruns held at once (slots and work matrices); more overflowed
+// trajectories than fit run in consecutive groups
+constexpr double kWideBudgetBytes = 24.0 * (1 << 30);
+
 }  // namespace
 
 }  // namespace tnh
 
 using namespace tnh;
 
+
 extern "C" int tnh_qjmc_run_batched(tnh_ctx *ctx, const tnh_qjmc_problem *p, int ntraj,
                                     const double *uniforms, int64_t nuniform, double *meas, int *njumps,
                                     int *jumps, int *jump_steps, int maxjumps, int *maxbond) {
   if (!ctx) return TNH_ERR_ARG;
-  std::vector<void *> owned;
-  std::vector<size_t> owned_sz;
-  Engine eng;
-  int rc = tnh_guard(ctx, [&] {
+  return tnh_guard(ctx, [&] {
     TNH_REQUIRE(p && ntraj >= 1 && uniforms && njumps && maxbond, TNH_ERR_ARG, "qjmc: bad arguments");
     TNH_REQUIRE(p->N >= 2 && p->d >= 1 && p->nsteps >= 0 && p->savesteps >= 1 && p->dt > 0, TNH_ERR_ARG,
                 "qjmc: bad problem sizes");
@@ -805,193 +1646,35 @@ extern "C" int tnh_qjmc_run_batched(tnh_ctx *ctx, const tnh_qjmc_problem *p, int
       TNH_REQUIRE((p->gate_range[g] == 1 || p->gate_range[g] == 2) && p->gate_site[g] >= 1 &&
                       p->gate_site[g] + p->gate_range[g] - 1 <= p->N,
                   TNH_ERR_ARG, "qjmc: gate out of range");
-    const int N = p->N, d = p->d, dd = d * d;
-    Engine &E = eng;
-    E.ctx = ctx;
-    E.p = p;
-    E.T = ntraj;
-    E.tq.resize(ntraj);
-    auto alloc = [&](size_t bytes) {
-      size_t cap = 0;
-      void *q = ctx->alloc(bytes, &cap);
-      owned.push_back(q);
-      owned_sz.push_back(cap);
-      return q;
-    };
-    // operator store: gates, jump operators, escape operators c-free L^H L, observables
-    std::vector<double> ops;
-    auto add_op = [&](const double *src, int64_t n) {
-      const int64_t off = int64_t(ops.size() / 2);
-      ops.insert(ops.end(), src, src + 2 * n);
-      return off;
-    };
-    for (int g = 0; g < ng; ++g)
-      E.gate_off.push_back(add_op(p->gate_data + 2 * p->gate_offset[g], p->gate_range[g] == 2 ? dd * dd : dd));
-    std::vector<double> esc(2 * dd);
-    for (int t = 0; t < p->njump; ++t) {
-      const double *L = p->jump_op + 2 * size_t(dd) * t;
-      E.jump_off.push_back(add_op(L, dd));
-      for (int j = 0; j < d; ++j)
-        for (int i = 0; i < d; ++i) {  // (L^H L)[i, j] = sum_k conj(L[k, i]) L[k, j]
-          std::complex<double> s = 0;
-          for (int k = 0; k < d; ++k)
-            s += std::conj(std::complex<double>(L[2 * (k + d * i)], L[2 * (k + d * i) + 1])) *
-                 std::complex<double>(L[2 * (k + d * j)], L[2 * (k + d * j) + 1]);
-          esc[2 * (i + d * j)] = s.real();
-          esc[2 * (i + d * j) + 1] = s.imag();
-        }
-      E.esc_off.push_back(add_op(esc.data(), dd));
-    }
-    for (int o = 0; o < p->nobs; ++o) E.obs_off.push_back(add_op(p->obs_op + 2 * size_t(dd) * o, dd));
-    TNH_REQUIRE(ops.size() / 2 < size_t(1) << 30, TNH_ERR_ARG, "qjmc batched: operator store too large");
-    double *op_dev = static_cast<double *>(alloc(sizeof(double) * std::max<size_t>(ops.size(), 2)));
-    to_device(ctx, op_dev, ops.data(), sizeof(double) * ops.size());
-    // state
-    E.b.N = N;
-    E.b.d = d;
-    E.b.CH = std::min(16, QM / d);  // Gram side d * CH <= 32; environments CH x CH <= 256
-    E.b.cap = int64_t(E.b.CH) * d * E.b.CH;
-    E.b.cutoff = p->cutoff;
-    E.b.maxdim = p->maxdim;
-    E.b.mindim = p->mindim;
-    E.b.op = op_dev;
-    E.b.S = static_cast<double *>(alloc(sizeof(double) * size_t(ntraj) * N * 2 * E.b.cap));
-    E.b.bond = static_cast<int *>(alloc(sizeof(int) * size_t(ntraj) * (N + 1)));
-    E.b.flags = static_cast<int *>(alloc(sizeof(int) * size_t(ntraj)));
-    double *init_dev = static_cast<double *>(alloc(sizeof(double) * 2 * N * d));
-    to_device(ctx, init_dev, p->init_state, sizeof(double) * 2 * N * d);
-    hipLaunchKernelGGL(qb_init_kernel, dim3(ntraj), dim3(64), 0, ctx->stream, E.b, init_dev, ntraj);
-    TNH_HIP(hipGetLastError());
-    // expectation terms: escape operators, then observables
-    const int nesc = p->njump, nobs = p->nobs;
-    std::vector<int> tsite;
-    std::vector<int64_t> toff;
-    for (int t = 0; t < nesc; ++t) {
-      tsite.push_back(p->jump_site[t]);
-      toff.push_back(E.esc_off[t]);
-    }
-    for (int o = 0; o < nobs; ++o) {
-      tsite.push_back(p->obs_site[o]);
-      toff.push_back(E.obs_off[o]);
-    }
-    const int nterm = int(tsite.size());
-    int *tsite_dev = static_cast<int *>(alloc(sizeof(int) * std::max(1, nterm)));
-    int64_t *toff_dev = static_cast<int64_t *>(alloc(sizeof(int64_t) * std::max(1, nterm)));
-    if (nterm) {
-      to_device(ctx, tsite_dev, tsite.data(), sizeof(int) * nterm);
-      to_device(ctx, toff_dev, toff.data(), sizeof(int64_t) * nterm);
-    }
-    std::vector<int> all(ntraj);
-    for (int t = 0; t < ntraj; ++t) all[t] = t;
-    int *all_dev = static_cast<int *>(alloc(sizeof(int) * ntraj));
-    to_device(ctx, all_dev, all.data(), sizeof(int) * ntraj);
-    double *env = static_cast<double *>(alloc(sizeof(double) * size_t(ntraj) * (N + 1) * 2 * E.b.CH * E.b.CH));
-    double *outd = static_cast<double *>(alloc(sizeof(double) * 2 * size_t(ntraj) * std::max(1, nterm)));
-    std::vector<double> outh(2 * size_t(ntraj) * std::max(1, nterm));
-    // all terms of all trajectories in one launch; terms [q0, q0 + nq) are read back
-    auto expect = [&](int q0, int nq) {
-      hipLaunchKernelGGL(qb_expect_kernel, dim3(ntraj), dim3(QT), 0, ctx->stream, E.b, all_dev, nq,
-                         tsite_dev + q0, toff_dev + q0, env, outd);
-      TNH_HIP(hipGetLastError());
-      to_host(ctx, outh.data(), outd, sizeof(double) * 2 * size_t(ntraj) * nq);
-    };
-    const int nsave = p->nsteps / p->savesteps + 1;
-    std::vector<int64_t> pos(ntraj, 0);
-    std::vector<int> save(ntraj, 0), nj(ntraj, 0);
-    auto draw = [&](int t) {
-      TNH_REQUIRE(pos[t] < nuniform, TNH_ERR_ARG, "qjmc: not enough uniforms for a trajectory");
-      return uniforms[size_t(t) * nuniform + pos[t]++];
-    };
-    auto measure = [&]() {
-      if (nobs > 0) {
-        expect(nesc, nobs);
-        for (int t = 0; t < ntraj; ++t)
-          for (int o = 0; o < nobs; ++o) {
-            double *m = meas + (size_t(t) * nsave + save[t]) * 2 * nobs;
-            m[2 * o] = outh[2 * (size_t(t) * nobs + o)];
-            m[2 * o + 1] = outh[2 * (size_t(t) * nobs + o) + 1];
-          }
+    const int N = p->N, d = p->d;
+    std::vector<int> rows(ntraj);
+    std::iota(rows.begin(), rows.end(), 0);
+    // narrow slots: Gram side d CH <= 32 in LDS; environments CH x CH <= 256
+    const int chn = std::min(16, QM / d);
+    run_engine(ctx, p, rows, uniforms, nuniform, meas, njumps, jumps, jump_steps, maxjumps, maxbond, chn, false);
+    if (d < 2) return;  // every bond of a d = 1 chain is 1
+    // Wide slots for the trajectories that outgrew the narrow ones, rerun from the start on
+    // their own uniforms: bond dimension 64, then the largest the chain and the truncation
+    // allow (at most 512 / d, the divide and conquer's side).
+    int cmax = dc::kDcPMax / d;
+    int64_t full = 1;  // d^floor(N / 2), the largest Schmidt rank of the chain
+    for (int i = 0; i < N / 2 && full < cmax; ++i) full *= d;
+    cmax = int(std::min<int64_t>(cmax, full));
+    if (p->maxdim > 0) cmax = std::min(cmax, std::max(p->maxdim, p->mindim));
+    int prev = chn;
+    for (const int ch : {std::min(cmax, 64), cmax}) {
+      if (ch <= prev) continue;
+      prev = ch;
+      std::vector<int> dead;
+      for (int r = 0; r < ntraj; ++r)
+        if (maxbond[r] == -1) dead.push_back(r);
+      if (dead.empty()) break;
+      const double per = 8.0 * (double(N) * 2 * ch * d * ch + double(wide_stride(d * ch)) + 2.0 * (N + 1) + 64);
+      const size_t group = size_t(std::max(1.0, std::floor(kWideBudgetBytes / per)));
+      for (size_t g0 = 0; g0 < dead.size(); g0 += group) {
+        const std::vector<int> sub(dead.begin() + g0, dead.begin() + std::min(dead.size(), g0 + group));
+        run_engine(ctx, p, sub, uniforms, nuniform, meas, njumps, jumps, jump_steps, maxjumps, maxbond, ch, true);
       }
-      for (int t = 0; t < ntraj; ++t) ++save[t];
-    };
-    auto normalize_all = [&](const std::vector<int> &which) {
-      for (int t : which) {
-        auto &x = E.tq[t];
-        if (x.center == 0) E.push_move(x, 1, 0);  // norm(): gmps.jl:29-37
-        x.q.push_back({P_NORM, x.center, 0, 0});
-      }
-      E.run_queues();
-    };
-    measure();
-    std::vector<double> rates(std::max(1, nesc));
-    std::vector<char> dead(ntraj, 0);  // trajectories that outgrew their slots
-    for (int step = 1; step <= p->nsteps; ++step) {
-      for (int t = 0; t < ntraj; ++t)
-        if (!dead[t]) E.push_applygates(E.tq[t]);
-      E.run_queues();
-      std::vector<int> live;
-      for (int t = 0; t < ntraj; ++t)
-        if (!dead[t]) live.push_back(t);
-      for (int t : live) (void)draw(t);  // qjmc.jl:64: drawn and unused
-      normalize_all(live);
-      check_flags(ctx, E, dead);
-      std::vector<int> jumpers;
-      if (nesc > 0) {
-        expect(0, nesc);
-        for (int t = 0; t < ntraj; ++t) {
-          if (dead[t]) continue;
-          double er = 0;
-          for (int q = 0; q < nesc; ++q) {
-            const double c = p->jump_coeff[q];
-            const double *v = &outh[2 * (size_t(t) * nesc + q)];
-            rates[q] = std::abs(std::complex<double>(c * c * v[0], c * c * v[1]));
-            er += rates[q];
-          }
-          const double prob = std::exp(-er * p->dt);
-          if (draw(t) > prob) {
-            const double r = draw(t);
-            int idx = -1;
-            double cum = 0;
-            for (int q = 0; q < nesc && idx < 0; ++q) {
-              cum += rates[q];
-              if (r < cum / er) idx = q;
-            }
-            TNH_REQUIRE(idx >= 0, TNH_ERR_NUMERIC, "qjmc: no jump channel selected");
-            auto &x = E.tq[t];
-            E.push_move(x, 1, 0);
-            x.q.push_back({P_SITEOP, p->jump_site[idx], int(E.jump_off[idx]), 0});
-            E.push_move(x, N, 0);
-            E.push_move(x, 1, 1);
-            jumpers.push_back(t);
-            if (nj[t] < maxjumps) {
-              jumps[size_t(t) * maxjumps + nj[t]] = idx + 1;
-              jump_steps[size_t(t) * maxjumps + nj[t]] = step;
-            }
-            ++nj[t];
-          }
-        }
-      } else {
-        for (int t = 0; t < ntraj; ++t)
-          if (!dead[t]) (void)draw(t);  // the jump test draws even with no channels
-      }
-      if (!jumpers.empty()) {
-        E.run_queues();
-        normalize_all(jumpers);
-      }
-      if (step % p->savesteps == 0) measure();
-    }
-    check_flags(ctx, E, dead);
-    std::vector<int> bonds(size_t(ntraj) * (N + 1));
-    to_host(ctx, bonds.data(), E.b.bond, sizeof(int) * bonds.size());
-    for (int t = 0; t < ntraj; ++t) {
-      njumps[t] = dead[t] ? 0 : nj[t];
-      int mb = 0;
-      for (int i = 1; i < N; ++i) mb = std::max(mb, bonds[size_t(t) * (N + 1) + i]);
-      maxbond[t] = dead[t] ? -1 : mb;
     }
   });
-  // stream-ordered cache: later work on the stream reuses these only after this run's kernels
-  for (size_t i = 0; i < owned.size(); ++i) ctx->release(owned[i], owned_sz[i]);
-  if (eng.ops_dev) ctx->release(eng.ops_dev, eng.ops_cap);
-  return rc;
 }

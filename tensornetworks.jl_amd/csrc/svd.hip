@@ -819,66 +819,6 @@ void launch_tiny_eig(hipStream_t st, const double *G, int p, int64_t imG, double
 
 // ---- small Hermitian eigenproblems by reduction and divide and conquer (dc_eig.hpp)
 
-// Execution policy of dc_eig on one workgroup of NT threads. The reductions take one barrier
-// (the caller alternates two scratch buffers, so a fast thread's next write cannot overtake a
-// slow thread's read); secular-root groups are G consecutive lanes of one wave.
-template <int NT>
-struct DcDeviceEx {
-  template <class F> __device__ void par(int n, F &&f) const {
-#pragma unroll 4
-    for (int t = threadIdx.x; t < n; t += NT) f(t);
-  }
-  __device__ void sync() const { __syncthreads(); }
-  __device__ void mark(int) const {}
-  __device__ int nthreads() const { return NT; }
-  template <class F> __device__ double sum(int n, F &&f, double *red) const {
-    double s = 0;
-#pragma unroll 4
-    for (int t = threadIdx.x; t < n; t += NT) s += f(t);
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (NT == 64) return s;
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    double r = 0;
-#pragma unroll
-    for (int w = 0; w < NT / 64; ++w) r += red[w];
-    return r;
-  }
-  template <class F> __device__ double vmax(int n, F &&f, double *red) const {
-    double s = 0;
-    for (int t = threadIdx.x; t < n; t += NT) s = fmax(s, f(t));
-    for (int o = 32; o > 0; o >>= 1) s = fmax(s, __shfl_xor(s, o, 64));
-    if (NT == 64) return s;
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    double r = 0;
-#pragma unroll
-    for (int w = 0; w < NT / 64; ++w) r = fmax(r, red[w]);
-    return r;
-  }
-  // lanes per secular root: the largest power of two with p G <= NT, at most a wave
-  __device__ int gsize(int p) const {
-    int g = 1;
-    while (g < 64 && 2 * g * p <= NT) g *= 2;
-    return g;
-  }
-  __device__ double gsum(double x, int G) const {
-    for (int o = G >> 1; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    return x;
-  }
-  // four sums over the group, their shuffle chains interleaved
-  __device__ void gsum4(double &a, double &b, double &c, double &d, int G) const {
-    for (int o = G >> 1; o > 0; o >>= 1) {
-      const double ta = __shfl_xor(a, o, 64), tb = __shfl_xor(b, o, 64);
-      const double tc = __shfl_xor(c, o, 64), td = __shfl_xor(d, o, 64);
-      a += ta;
-      b += tb;
-      c += tc;
-      d += td;
-    }
-  }
-};
-
 // Reduction and divide and conquer of the p x p Hermitian G (planar; p <= kDcPMax): the
 // eigenvalues (ascending, out[0..p)), the success flag out[p], the eigenvectors of the
 // tridiagonal matrix in Q, the Householder vectors in G and their factors in taus[0..2p).
@@ -886,7 +826,7 @@ template <bool CPLX, int NT>
 __global__ __launch_bounds__(NT) void dc_eig_kernel(double *G, int p, int64_t imG, double *taus, double *Q,
                                                      double *Q2, double *U, double *out) {
   __shared__ dc::DcShared sh;
-  dc::dc_eig<CPLX>(DcDeviceEx<NT>{}, sh, G, CPLX ? G + imG : nullptr, p, taus, taus + p, Q, Q2, U, out);
+  dc::dc_eig<CPLX>(dc::DcDeviceEx<NT>{}, sh, G, CPLX ? G + imG : nullptr, p, taus, taus + p, Q, Q2, U, out);
 }
 
 // One 1024-thread workgroup for every size: measured, 64 threads for p <= 64 and 256 for

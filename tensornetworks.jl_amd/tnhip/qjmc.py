@@ -129,10 +129,11 @@ def qjmc_trajectories(st: Sitetypes, init_names, H: OpList, jumpops: OpList, tma
     each with its own stream. All trajectories start from productMPS(st, init_names);
     trajectory t consumes uniforms[t] in the reference's rand() order. Jump operators must
     be single-site (multi-site ones run through qjmc_simulation). engine: "batched" advances
-    all trajectories together (tnh_qjmc_run_batched: one launch per primitive for every
-    trajectory, bond dimension at most min(16, 32 / d)); "workers" runs them one by one on
-    `nworkers` streams (tnh_qjmc_run, no bond limit); "auto" runs the batched engine and
-    reruns on the workers only the trajectories whose bond dimension outgrew its slots
+    all trajectories together (tnh_qjmc_run_batched: one workgroup per trajectory in every
+    launch, narrow slots of bond dimension min(16, 32 / d) and, for the trajectories that
+    outgrow them, wide slots of up to 512 / d); "workers" runs them one by one on `nworkers`
+    streams (tnh_qjmc_run, no bond limit, d <= 8); "auto" runs the batched engine and reruns
+    on the workers only the trajectories whose bond dimension outgrew the wide slots too
     (each with its own uniforms; "engine" then reads "batched+workers"). Returns a dict with
     `jumps` and `jumptimes` (lists per trajectory, 1-based indices, as the reference stores
     them), `measurements` (ntraj x nsave x nobs, coefficient * <O>), `times`, `maxbonddim`."""

@@ -319,10 +319,10 @@ def test_qjmc_n40_workload_vs_oracle_trajectories(ctx):
 
 
 def test_qjmc_overflowing_rows_vs_oracle(ctx):
-    """The regime above the batched slots: dissipative TFIM at g = 1 (N = 10, gamma = 0.1,
-    cutoff 1e-12, 70 steps of dt = 0.05), where most trajectories reach bond dimension
-    24-32 and are rerun on the per-trajectory workers while the others complete in the
-    batched engine. Every row, batched or rerun, against the oracle MPS trajectory on its
+    """The regime above the narrow batched slots: dissipative TFIM at g = 1 (N = 10,
+    gamma = 0.1, cutoff 1e-12, 70 steps of dt = 0.05), where most trajectories reach bond
+    dimension 24-32 and are rerun on the wide slots (one 1024-thread workgroup per
+    trajectory, divide-and-conquer splits) while the others complete in the narrow ones. Every row, batched or rerun, against the oracle MPS trajectory on its
     uniforms: identical jump records, bond dimensions, <sz_i> to 1e-8."""
     from tnhip import spinhalf
     from tnhip.qjmc import qjmc_trajectories
@@ -333,7 +333,7 @@ def test_qjmc_overflowing_rows_vs_oracle(ctx):
     U = np.random.default_rng(31).random((8, 4 * steps + 8))
     out = qjmc_trajectories(spinhalf(), ["up"] * N, H, J, steps * dt, dt, U, observables=zs, cutoff=1e-12,
                             nworkers=4)
-    assert out["engine"] == "batched+workers"
+    assert out["engine"] == "batched"
     chis = out["maxbonddim"]
     assert min(chis) <= 16 < max(chis)
     up = np.array([1.0, 0.0])
