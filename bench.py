@@ -474,7 +474,7 @@ def qjmc_tfim_n40_critical(ntraj=256, steps=150, sample_traj=4, nworkers=4, chi_
     """Quantum-jump trajectories in the regime of uneven, growing bond dimensions: the
     critical dissipative TFIM, N = 40, g = 1, s-_i at rate 0.02, dt = 0.02, 150 second-order
     Trotter steps, cutoff 1e-10, chi_max 256, from all spins up, uniforms from seed 1. Bonds
-    pass 16 near t = 2 and reach 30-40 by t = 3 (fewer in trajectories that jump), so every
+    pass 16 near t = 2 and reach 31-32 by t = 3, so every
     trajectory outgrows the narrow batched slots and is rerun on the wide ones (one
     1024-thread workgroup per trajectory, divide-and-conquer splits); the time includes the
     narrow run. The first `sample_traj` trajectories also run on the per-trajectory workers
@@ -772,6 +772,8 @@ def main(argv=None):
     ap.add_argument("--tebd-only", action="store_true", help="run only the real-time TEBD leg")
     ap.add_argument("--qjmc-critical-only", action="store_true",
                     help="run only the critical-chain QJMC leg (bonds above 16, wide batched slots)")
+    ap.add_argument("--no-workers", action="store_true",
+                    help="with --qjmc-critical-only: skip the per-trajectory workers sample")
     ap.add_argument("--qjmc-only", action="store_true",
                     help="run only the QJMC leg on the batched engine and the host loop (one host thread)")
     ap.add_argument("--bond-only", action="store_true",
@@ -824,7 +826,8 @@ def main(argv=None):
         print(json.dumps({"tebd_tfim_N40": tebd_tfim_n40(cpu_steps=0 if a.no_cpu else 2)}))
         return 0
     if a.qjmc_critical_only:
-        print(json.dumps({"qjmc_tfim_N40_critical": qjmc_tfim_n40_critical()}))
+        print(json.dumps({"qjmc_tfim_N40_critical": qjmc_tfim_n40_critical(
+            sample_traj=0 if a.no_workers else 4, cpu_traj=0 if a.no_cpu else 1)}))
         return
     if a.qjmc_only:
         print(json.dumps({"qjmc_tfim_N40": qjmc_tfim_n40(sample_traj=0)}))

@@ -768,8 +768,9 @@ end
 
 Many trajectories of qjmc_simulation (src/algorithms/mps/qjmc.jl:28-167, classical branch):
 U[:, t] holds trajectory t's uniforms in the reference's rand() order. `batched` advances
-all trajectories together and reruns on the per-trajectory runner those whose bond
-outgrew the batched slots.
+all trajectories together (narrow slots, then wide slots of up to 512 / d for those that
+outgrow them) and reruns on the per-trajectory runner those whose bond outgrew the wide
+slots too.
 """
 function qjmc_trajectories!(prob::Ref{QjmcProblem}, U::Matrix{Float64}, meas::Array{Float64},
                             njumps::Vector{Cint}, jumps::Matrix{Cint}, jsteps::Matrix{Cint},
