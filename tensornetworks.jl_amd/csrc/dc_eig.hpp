@@ -81,6 +81,7 @@ struct DcShared {
 };
 
 TNH_HD inline bool finite(double v) { return fabs(v) <= 1.7976931348623157e308; }
+
 // Ordering key of the merge sorts: NaN after everything, so that with ties broken by position
 // every rank below is a permutation whatever the values (the gathers index with it).
 TNH_HD inline double order_key(double v) { return v == v ? v : INFINITY; }
@@ -381,25 +382,44 @@ TNH_HD void dc_eig(const Ex &ex, DcShared &sh, double *Gr, double *Gi, int p, do
         b = xr * si + xi * sr;
       }
     };
-    // y = B v: nc (<= 16) column chunks per row, partial sums in LDS; v formed on the fly
+    // y = B v: nc (<= 16) column chunks per row, partial sums in LDS; v formed on the fly.
+    // Each thread loads kMvU columns' entries (and x entries) before using any of them, so
+    // that many loads are in flight while A_k streams from L2.
     int nc = ex.nthreads() / n > 0 ? ex.nthreads() / n : 1;
     if (nc > 16) nc = 16;
+    constexpr int kMvU = 4;
     ex.par(n * nc, [&](int e) {
       const int i = e % n, cc = e / n;
       double sr2 = 0, si2 = 0;
+      for (int j0 = cc; j0 < n; j0 += kMvU * nc) {
+        double br[kMvU], bi[kMvU], xr[kMvU], xi[kMvU];
 #if defined(__HIP_DEVICE_COMPILE__)
-#pragma unroll 4
+#pragma unroll
 #endif
-      for (int j = cc; j < n; j += nc) {
-        double vrj, vij;
-        vget(j, vrj, vij);
-        const double br = Br[i + j * lda];
-        if (CPLX) {
-          const double bi = Bi[i + j * lda];
-          sr2 += br * vrj - bi * vij;
-          si2 += br * vij + bi * vrj;
-        } else {
-          sr2 += br * vrj;
+        for (int u = 0; u < kMvU; ++u) {
+          const int j = j0 + u * nc, jj = j < n ? j : j0;
+          br[u] = Br[i + jj * lda];
+          bi[u] = CPLX ? Bi[i + jj * lda] : 0.0;
+          xr[u] = col_r[jj];
+          xi[u] = CPLX ? col_i[jj] : 0.0;
+        }
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+        for (int u = 0; u < kMvU; ++u) {
+          const int j = j0 + u * nc;
+          if (j >= n) break;
+          double vrj = 1.0, vij = 0.0;  // v_0 = 1; v_j = x_j s
+          if (j != 0) {
+            vrj = xr[u] * sr - xi[u] * si;
+            vij = xr[u] * si + xi[u] * sr;
+          }
+          if (CPLX) {
+            sr2 += br[u] * vrj - bi[u] * vij;
+            si2 += br[u] * vij + bi[u] * vrj;
+          } else {
+            sr2 += br[u] * vrj;
+          }
         }
       }
       sh.u.h.part[e] = sr2;
@@ -425,28 +445,54 @@ TNH_HD void dc_eig(const Ex &ex, DcShared &sh, double *Gr, double *Gi, int p, do
     }, red());
     const double h = 0.5 * (tr * tr + ti * ti) * vy;
     // B -= w v^H + v w^H with w = tau y - h v; the updated column 0 of B is the next
-    // step's x, whose tail (rows 2..) is summed here
-    xn2 = ex.sum(n * n, [&](int e) {
-      const int i = e % n, j = e / n;
+    // step's x, whose tail (rows 2..) is summed here. Thread (i, c0) updates row i of the
+    // columns c0, c0 + nu, ...: no index division per entry, the row's w_i and v_i held in
+    // registers, and kUpU entries loaded before any of them is stored, so that many loads
+    // are in flight while A_k streams from L2.
+    int nu = ex.nthreads() / n > 0 ? ex.nthreads() / n : 1;
+    constexpr int kUpU = 1;
+    xn2 = ex.sum(n * nu, [&](int e) {
+      const int i = e % n, c0 = e / n;
       const double *vr = sh.u.h.vr, *vi = sh.u.h.vi, *yr = sh.u.h.yr, *yi = sh.u.h.yi;
-      const double wir = tr * yr[i] - ti * yi[i] - h * vr[i];
-      const double wjr = tr * yr[j] - ti * yi[j] - h * vr[j];
-      double nr, ni = 0.0;
-      if (CPLX) {
-        const double wii = tr * yi[i] + ti * yr[i] - h * vi[i];
-        const double wji = tr * yi[j] + ti * yr[j] - h * vi[j];
-        // w_i conj(v_j) + v_i conj(w_j)
-        const double re = wir * vr[j] + wii * vi[j] + vr[i] * wjr + vi[i] * wji;
-        const double im = wii * vr[j] - wir * vi[j] + vi[i] * wjr - vr[i] * wji;
-        nr = Br[i + j * lda] - re;
-        ni = Bi[i + j * lda] - im;
-        Br[i + j * lda] = nr;
-        Bi[i + j * lda] = ni;
-      } else {
-        nr = Br[i + j * lda] - (wir * vr[j] + vr[i] * wjr);
-        Br[i + j * lda] = nr;
+      const double vri = vr[i], vii = CPLX ? vi[i] : 0.0;
+      const double wir = tr * yr[i] - ti * yi[i] - h * vri;
+      const double wii = CPLX ? tr * yi[i] + ti * yr[i] - h * vii : 0.0;
+      double acc = 0.0;
+      for (int j0 = c0; j0 < n; j0 += kUpU * nu) {
+        double br[kUpU], bi[kUpU];
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+        for (int u = 0; u < kUpU; ++u) {
+          const int j = j0 + u * nu, jj = j < n ? j : j0;
+          br[u] = Br[i + jj * lda];
+          bi[u] = CPLX ? Bi[i + jj * lda] : 0.0;
+        }
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+        for (int u = 0; u < kUpU; ++u) {
+          const int j = j0 + u * nu;
+          if (j >= n) break;
+          const double wjr = tr * yr[j] - ti * yi[j] - h * vr[j];
+          double nr, ni = 0.0;
+          if (CPLX) {
+            const double wji = tr * yi[j] + ti * yr[j] - h * vi[j];
+            // w_i conj(v_j) + v_i conj(w_j)
+            const double re = wir * vr[j] + wii * vi[j] + vri * wjr + vii * wji;
+            const double im = wii * vr[j] - wir * vi[j] + vii * wjr - vri * wji;
+            nr = br[u] - re;
+            ni = bi[u] - im;
+            Br[i + j * lda] = nr;
+            Bi[i + j * lda] = ni;
+          } else {
+            nr = br[u] - (wir * vr[j] + vri * wjr);
+            Br[i + j * lda] = nr;
+          }
+          if (j == 0 && i >= 2) acc += nr * nr + ni * ni;
+        }
       }
-      return (j == 0 && i >= 2) ? nr * nr + ni * ni : 0.0;
+      return acc;
     }, red());
   };
   const int planes = CPLX ? 2 : 1;
