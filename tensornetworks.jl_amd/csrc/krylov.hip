@@ -4,10 +4,10 @@
 //
 // The Krylov basis lives in HBM in FP64 (planar complex when theta is complex); the
 // matvecs run in the requested precision. The projected matrix (at most 32 x 32) is
-// diagonalised on the host.
+// assembled and diagonalised on the device (ritz_kernel); the host reads back only the Ritz
+// value, the residual and the breakdown and finiteness flags of each cycle.
 #include <algorithm>
 #include <cmath>
-#include <complex>
 #include <vector>
 
 #include "tnh_internal.hpp"
@@ -21,11 +21,6 @@ namespace {
 
 constexpr int kMaxVec = 32;
 constexpr int kDotBlocks = 256;
-
-struct Coef {
-  double re[kMaxVec];
-  double im[kMaxVec];
-};
 
 __device__ inline double wsum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -171,11 +166,183 @@ __global__ __launch_bounds__(256) void normalize_copy_kernel(double *__restrict_
     dst[k] = src[k] * inv;
 }
 
-// out = scale * sum_i y_i v_i
-__global__ __launch_bounds__(256) void lincomb_kernel(double *__restrict__ out,
-                                                      const double *__restrict__ V,
-                                                      int64_t vstride, int nv, Coef y, int64_t n,
-                                                      bool cplx) {
+// out = sum_i y_i V[i] for i < m, m and the coefficients read from the state ritz_kernel
+// writes (y re at st[kSY..], im at st[kSY + kMaxVec..])
+__global__ __launch_bounds__(256) void lincomb_dev_kernel(double *__restrict__ out, const double *__restrict__ V,
+                                                          int64_t vstride, const double *__restrict__ st,
+                                                          int64_t n, bool cplx);
+
+// Per Krylov step j: the two Gram-Schmidt coefficient passes (2 x 2 kMaxVec values) and
+// ||w||^2 (2 values), kept on the device until the cycle ends.
+constexpr int kStep = 4 * kMaxVec + 2;
+// ritz_kernel's state and results: the Ritz value, |b|, b (re, im), m, the breakdown flag,
+// the finiteness flag (1 finite, 0 non-finite coefficient, -1 non-finite vector norm), then
+// the Ritz coefficients y (re at kSY, im at kSY + kMaxVec). Cycles after the first read the
+// previous Ritz value and b from it as their projected matrix's first column.
+enum { kSTheta = 0, kSAbsB = 1, kSBr = 2, kSBi = 3, kSM = 4, kSBrk = 5, kSFin = 6, kSY = 8 };
+constexpr int kStateLen = kSY + 2 * kMaxVec;
+constexpr int kEmb = 2 * kMaxVec;  // largest side of the symmetric problem
+
+// The projected matrix of one restart cycle and its lowest eigenpair, on one wave, with the
+// host's former bookkeeping: H (m x m, Hermitian) assembled from the coefficient passes of
+// steps start..kdim-1 (and, after a restart, the kept Ritz value and b in column 0); a step
+// whose new norm beta is at most 1e-14 times the largest |h| so far is a breakdown and ends
+// the cycle there (m = j + 1); non-finite coefficients stop it with a flag. The lowest
+// eigenpair comes from a cyclic Jacobi on H (real) or on its real symmetric embedding
+// [[Re, -Im], [Im, Re]] (complex: every eigenvalue twice, any vector of the lowest pair is
+// the Ritz vector up to a phase), run until the off-diagonal mass is below 1e-32 of the
+// total; the Ritz coefficients are normalised and b = sum_j h(m, j) y_j.
+__global__ __launch_bounds__(64) void ritz_kernel(const double *__restrict__ hdev, int start, int kdim, int cplx,
+                                                  double *__restrict__ st) {
+  constexpr int LH = kMaxVec + 1;  // leading dimension of H ((kdim + 1) x kdim)
+  __shared__ double Hr[LH * kMaxVec], Hi[LH * kMaxVec];
+  __shared__ double A[kEmb * kEmb], Z[kEmb * kEmb];
+  __shared__ int sm, sbrk, sfin;
+  const int t = threadIdx.x;
+  for (int e = t; e < LH * kMaxVec; e += 64) {
+    Hr[e] = 0.0;
+    Hi[e] = 0.0;
+  }
+  __syncthreads();
+  if (t == 0) {
+    if (start == 1) {
+      Hr[0] = st[kSTheta];
+      Hr[1] = st[kSBr];
+      Hi[1] = st[kSBi];
+    }
+    int m = kdim, brk = 0, fin = 1;
+    for (int j = start; j < kdim; ++j) {
+      const double *hj = hdev + int64_t(j) * kStep;
+      bool ok = isfinite(hj[4 * kMaxVec]);
+      for (int pass = 0; pass < 2; ++pass)
+        for (int i = 0; i < 2 * (j + 1); ++i) ok = ok && isfinite(hj[pass * 2 * kMaxVec + i]);
+      if (!ok) {
+        fin = 0;
+        break;
+      }
+      for (int pass = 0; pass < 2; ++pass)
+        for (int i = 0; i <= j; ++i) {
+          const double *cj = hj + pass * 2 * kMaxVec;
+          Hr[i + j * LH] += cj[2 * i];
+          if (cplx) Hi[i + j * LH] += cj[2 * i + 1];
+        }
+      const double beta = sqrt(fmax(0.0, hj[4 * kMaxVec]));
+      if (!isfinite(beta)) {
+        fin = -1;
+        break;
+      }
+      Hr[(j + 1) + j * LH] = beta;
+      double scale_h = 1.0;
+      for (int jj = 0; jj <= j; ++jj)
+        for (int ii = 0; ii <= j + 1; ++ii)
+          scale_h = fmax(scale_h, sqrt(Hr[ii + jj * LH] * Hr[ii + jj * LH] + Hi[ii + jj * LH] * Hi[ii + jj * LH]));
+      if (beta <= 1e-14 * scale_h) {
+        m = j + 1;
+        brk = 1;
+        break;
+      }
+    }
+    sm = m;
+    sbrk = brk;
+    sfin = fin;
+  }
+  __syncthreads();
+  const int m = sm;
+  if (sfin != 1) {
+    if (t == 0) {
+      st[kSFin] = sfin;
+      st[kSM] = m;
+    }
+    return;
+  }
+  // the symmetric problem: n = m (real) or 2 m (embedding), Hermitian part of H_m
+  const int n = cplx ? 2 * m : m;
+  for (int e = t; e < n * n; e += 64) {
+    const int i = e % n, j = e / n;
+    double v;
+    if (!cplx) {
+      v = 0.5 * (Hr[i + j * LH] + Hr[j + i * LH]);
+    } else {
+      const int bi = i / m, bj = j / m, ii = i % m, jj = j % m;
+      const double hre = 0.5 * (Hr[ii + jj * LH] + Hr[jj + ii * LH]);
+      const double him = 0.5 * (Hi[ii + jj * LH] - Hi[jj + ii * LH]);
+      v = bi == bj ? hre : (bi == 1 ? him : -him);  // [[Re, -Im], [Im, Re]]
+    }
+    A[i + j * n] = v;
+    Z[i + j * n] = i == j ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0, tot = 0;
+    for (int e = t; e < n * n; e += 64) {
+      const double a = A[e];
+      tot += a * a;
+      if (e % n != e / n) off += a * a;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      off += __shfl_xor(off, o, 64);
+      tot += __shfl_xor(tot, o, 64);
+    }
+    if (off <= 1e-32 * tot || off == 0.0) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = A[p + q * n];
+        if (fabs(apq) < 1e-300) continue;
+        const double tau = (A[q + q * n] - A[p + p * n]) / (2 * apq);
+        const double tt = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1 + tau * tau));
+        const double c = 1 / sqrt(1 + tt * tt), s = tt * c;
+        __syncthreads();
+        for (int k = t; k < n; k += 64) {  // A <- A J, Z <- Z J
+          const double akp = A[k + p * n], akq = A[k + q * n];
+          A[k + p * n] = c * akp - s * akq;
+          A[k + q * n] = s * akp + c * akq;
+          const double zkp = Z[k + p * n], zkq = Z[k + q * n];
+          Z[k + p * n] = c * zkp - s * zkq;
+          Z[k + q * n] = s * zkp + c * zkq;
+        }
+        __syncthreads();
+        for (int k = t; k < n; k += 64) {  // A <- J^T A
+          const double apk = A[p + k * n], aqk = A[q + k * n];
+          A[p + k * n] = c * apk - s * aqk;
+          A[q + k * n] = s * apk + c * aqk;
+        }
+        __syncthreads();
+      }
+  }
+  if (t == 0) {
+    int lo = 0;
+    for (int i = 1; i < n; ++i)
+      if (A[i + i * n] < A[lo + lo * n]) lo = i;
+    double nrm = 0;
+    for (int i = 0; i < m; ++i) {
+      const double yr = Z[i + lo * n], yi = cplx ? Z[(i + m) + lo * n] : 0.0;
+      nrm += yr * yr + yi * yi;
+    }
+    nrm = sqrt(nrm);
+    double br = 0, bi = 0;
+    for (int i = 0; i < m; ++i) {
+      const double yr = Z[i + lo * n] / nrm, yi = cplx ? Z[(i + m) + lo * n] / nrm : 0.0;
+      st[kSY + i] = yr;
+      st[kSY + kMaxVec + i] = yi;
+      const double hr = Hr[m + i * LH], hi = Hi[m + i * LH];  // h(m, i)
+      br += hr * yr - hi * yi;
+      bi += hr * yi + hi * yr;
+    }
+    st[kSTheta] = A[lo + lo * n];
+    st[kSAbsB] = sqrt(br * br + bi * bi);
+    st[kSBr] = br;
+    st[kSBi] = bi;
+    st[kSM] = m;
+    st[kSBrk] = sbrk;
+    st[kSFin] = 1;
+  }
+}
+
+__global__ __launch_bounds__(256) void lincomb_dev_kernel(double *__restrict__ out, const double *__restrict__ V,
+                                                          int64_t vstride, const double *__restrict__ st,
+                                                          int64_t n, bool cplx) {
+  const int nv = int(st[kSM]);
+  const double *yr = st + kSY, *yi = st + kSY + kMaxVec;
   for (int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; k < n;
        k += int64_t(gridDim.x) * blockDim.x) {
     double ar = 0, ai = 0;
@@ -183,10 +350,10 @@ __global__ __launch_bounds__(256) void lincomb_kernel(double *__restrict__ out,
       const double vr = V[i * vstride + k];
       if (cplx) {
         const double vi = V[i * vstride + n + k];
-        ar += y.re[i] * vr - y.im[i] * vi;
-        ai += y.re[i] * vi + y.im[i] * vr;
+        ar += yr[i] * vr - yi[i] * vi;
+        ai += yr[i] * vi + yi[i] * vr;
       } else {
-        ar += y.re[i] * vr;
+        ar += yr[i] * vr;
       }
     }
     out[k] = ar;
@@ -203,83 +370,6 @@ int grid1(tnh_ctx *ctx, int64_t n) {
 
 double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, int kdim,
                           int maxiter, double tol, int *nmv_out);
-
-// Symmetric eigenproblem of a small dense matrix (column-major n x n) by cyclic
-// Jacobi rotations; eigenvalues ascending in w, eigenvectors in the columns of Z.
-void jacobi_eig_sym_host(int n, std::vector<double> A, std::vector<double> &w,
-                         std::vector<double> &Z) {
-  Z.assign(size_t(n) * n, 0.0);
-  for (int i = 0; i < n; ++i) Z[i + size_t(i) * n] = 1.0;
-  auto a = [&](int i, int j) -> double & { return A[i + size_t(j) * n]; };
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0, tot = 0;
-    for (int j = 0; j < n; ++j)
-      for (int i = 0; i < n; ++i) {
-        tot += a(i, j) * a(i, j);
-        if (i != j) off += a(i, j) * a(i, j);
-      }
-    if (off <= 1e-32 * tot || off == 0.0) break;
-    for (int p = 0; p < n - 1; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        const double apq = a(p, q);
-        if (std::abs(apq) < 1e-300) continue;
-        const double tau = (a(q, q) - a(p, p)) / (2 * apq);
-        const double t = (tau >= 0 ? 1.0 : -1.0) / (std::abs(tau) + std::sqrt(1 + tau * tau));
-        const double c = 1 / std::sqrt(1 + t * t), s = t * c;
-        for (int k = 0; k < n; ++k) {  // A <- A J
-          const double akp = a(k, p), akq = a(k, q);
-          a(k, p) = c * akp - s * akq;
-          a(k, q) = s * akp + c * akq;
-        }
-        for (int k = 0; k < n; ++k) {  // A <- J^T A
-          const double apk = a(p, k), aqk = a(q, k);
-          a(p, k) = c * apk - s * aqk;
-          a(q, k) = s * apk + c * aqk;
-        }
-        for (int k = 0; k < n; ++k) {
-          const double zkp = Z[k + size_t(p) * n], zkq = Z[k + size_t(q) * n];
-          Z[k + size_t(p) * n] = c * zkp - s * zkq;
-          Z[k + size_t(q) * n] = s * zkp + c * zkq;
-        }
-      }
-  }
-  std::vector<int> idx(n);
-  for (int i = 0; i < n; ++i) idx[i] = i;
-  std::sort(idx.begin(), idx.end(), [&](int x, int y) { return a(x, x) < a(y, y); });
-  std::vector<double> Zs(size_t(n) * n);
-  w.resize(n);
-  for (int j = 0; j < n; ++j) {
-    w[j] = a(idx[j], idx[j]);
-    for (int k = 0; k < n; ++k) Zs[k + size_t(j) * n] = Z[k + size_t(idx[j]) * n];
-  }
-  Z.swap(Zs);
-}
-
-// Lowest eigenpair of a small Hermitian matrix H (m x m, column-major).
-static void herm_lowest(int m, const std::vector<std::complex<double>> &H, double *e0,
-                        std::vector<std::complex<double>> &y) {
-  // real symmetric embedding [[Re, -Im], [Im, Re]]: every eigenvalue appears twice
-  const int n = 2 * m;
-  std::vector<double> B(size_t(n) * n, 0.0), w, Z;
-  for (int j = 0; j < m; ++j)
-    for (int i = 0; i < m; ++i) {
-      const std::complex<double> h = 0.5 * (H[i + size_t(j) * m] + std::conj(H[j + size_t(i) * m]));
-      B[i + size_t(j) * n] = h.real();
-      B[(i + m) + size_t(j + m) * n] = h.real();
-      B[(i + m) + size_t(j) * n] = h.imag();
-      B[i + size_t(j + m) * n] = -h.imag();
-    }
-  jacobi_eig_sym_host(n, B, w, Z);
-  *e0 = w[0];
-  y.resize(m);
-  double nrm = 0;
-  for (int i = 0; i < m; ++i) {
-    y[i] = std::complex<double>(Z[i], Z[i + m]);
-    nrm += std::norm(y[i]);
-  }
-  nrm = std::sqrt(nrm);
-  for (auto &v : y) v /= nrm;
-}
 
 double eigsolve_ground(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *W1, const tnh_buf *W2,
                        const tnh_buf *R, tnh_buf *theta, int kdim, int maxiter, double tol,
@@ -306,10 +396,8 @@ double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, in
   double *Wv = V + vstride * (kdim + 1);
   double *part = static_cast<double *>(ctx->workspace(701, sizeof(double) * 2 * kDotBlocks * kMaxVec));
   double *dres = static_cast<double *>(ctx->workspace(702, sizeof(double) * 2 * kMaxVec));
-  // per Krylov step j: the two Gram-Schmidt coefficient passes (2 x 2 kMaxVec values) and
-  // ||w||^2 (2 values), kept on the device until the cycle ends
-  constexpr int kStep = 4 * kMaxVec + 2;
   double *hdev = static_cast<double *>(ctx->workspace(703, sizeof(double) * kStep * kMaxVec));
+  double *sdev = static_cast<double *>(ctx->workspace(704, sizeof(double) * kStateLen));
   double *hres = ctx->host_scalars;
   const int g = grid1(ctx, n);
   // out[2 i], out[2 i + 1] = <V[i], w> for the nv vectors V[0..nv) (stride vstride)
@@ -345,19 +433,13 @@ double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, in
     scale(ctx, V, TNH_F64, vstride, 1.0 / nr);
   }
 
-  using cd = std::complex<double>;
-  std::vector<cd> H(size_t(kdim + 1) * kdim, cd(0));  // (kdim+1) x kdim, column-major
-  auto h = [&](int i, int j) -> cd & { return H[i + size_t(j) * (kdim + 1)]; };
-  int start = 0, nmv = 0, m = kdim;
+  int start = 0, nmv = 0;
   double theta_e = 0;
-  std::vector<cd> y;
   for (int it = 0; it < maxiter; ++it) {
-    m = kdim;
-    bool breakdown = false;
     // The cycle runs on the device without host round trips: both Gram-Schmidt passes take
-    // their coefficients from device memory and V[j+1] = w / ||w|| is formed there. The host
-    // then replays the projected matrix and the breakdown test of each step in order, and
-    // steps after a breakdown (whose vectors are never used) are not counted.
+    // their coefficients from device memory and V[j+1] = w / ||w|| is formed there; then
+    // ritz_kernel replays the projected matrix and the breakdown test of each step in order
+    // and solves it. Steps after a breakdown (whose vectors are never used) are not counted.
     for (int j = start; j < kdim; ++j) {
       tnh_buf vj = view(V + j * vstride), wb = view(Wv);
       matvec(&vj, &wb);
@@ -374,68 +456,31 @@ double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, in
                          Wv, nj, vstride);
       TNH_HIP(hipGetLastError());
     }
-    std::vector<double> hh(size_t(kStep) * (kdim - start));
-    to_host(ctx, hh.data(), hdev + int64_t(start) * kStep, sizeof(double) * hh.size());
-    for (int j = start; j < kdim; ++j) {
-      const double *hj = hh.data() + size_t(j - start) * kStep;
-      // A non-finite coefficient (NaN or Inf in theta or the environments) is an error: it
-      // must not reach the host eigensolver, whose ordering of the eigenvalues needs a
-      // consistent comparison.
-      bool finite = std::isfinite(hj[4 * kMaxVec]);
-      for (int pass = 0; pass < 2; ++pass)
-        for (int i = 0; i < 2 * (j + 1); ++i) finite = finite && std::isfinite(hj[pass * 2 * kMaxVec + i]);
-      TNH_REQUIRE(finite, TNH_ERR_NUMERIC, "eigsolve: non-finite Krylov coefficient");
-      ++nmv;
-      for (int pass = 0; pass < 2; ++pass)
-        for (int i = 0; i <= j; ++i) {
-          const double *cj = hj + pass * 2 * kMaxVec;
-          h(i, j) += cd(cj[2 * i], cplx ? cj[2 * i + 1] : 0.0);
-        }
-      const double beta = std::sqrt(std::max(0.0, hj[4 * kMaxVec]));
-      TNH_REQUIRE(std::isfinite(beta), TNH_ERR_NUMERIC, "eigsolve: non-finite Krylov vector");
-      h(j + 1, j) = beta;
-      double scale_h = 1.0;
-      for (int jj = 0; jj <= j; ++jj)
-        for (int ii = 0; ii <= j + 1; ++ii) scale_h = std::max(scale_h, std::abs(h(ii, jj)));
-      if (beta <= 1e-14 * scale_h) {
-        m = j + 1;
-        breakdown = true;
-        break;
-      }
-    }
-    std::vector<cd> Hm(size_t(m) * m);
-    for (int j = 0; j < m; ++j)
-      for (int i = 0; i < m; ++i) Hm[i + size_t(j) * m] = h(i, j);
-    herm_lowest(m, Hm, &theta_e, y);
-    cd b(0);
-    for (int jj = 0; jj < m; ++jj) b += h(m, jj) * y[jj];
-    if (breakdown || std::abs(b) <= tol || it == maxiter - 1) break;
+    hipLaunchKernelGGL(ritz_kernel, dim3(1), dim3(64), 0, ctx->stream, hdev, start, kdim, cplx ? 1 : 0, sdev);
+    TNH_HIP(hipGetLastError());
+    double res[kSY];
+    to_host(ctx, res, sdev, sizeof(res));
+    // A non-finite coefficient (NaN or Inf in theta or the environments) is an error.
+    TNH_REQUIRE(res[kSFin] != 0.0, TNH_ERR_NUMERIC, "eigsolve: non-finite Krylov coefficient");
+    TNH_REQUIRE(res[kSFin] == 1.0, TNH_ERR_NUMERIC, "eigsolve: non-finite Krylov vector");
+    const int m = int(res[kSM]);
+    nmv += m - start;
+    theta_e = res[kSTheta];
+    TNH_REQUIRE(std::isfinite(theta_e) && std::isfinite(res[kSAbsB]), TNH_ERR_NUMERIC,
+                "eigsolve: non-finite Ritz value");
+    if (res[kSBrk] != 0.0 || res[kSAbsB] <= tol || it == maxiter - 1) break;
     // restart: V0 <- Ritz vector, V1 <- V[m]; projected matrix [[theta, conj(b)], [b, .]]
-    Coef yc{};
-    for (int i = 0; i < m; ++i) {
-      yc.re[i] = y[i].real();
-      yc.im[i] = y[i].imag();
-    }
-    hipLaunchKernelGGL(lincomb_kernel, dim3(g), dim3(256), 0, ctx->stream, Wv, V, vstride, m, yc, n,
-                       cplx);
+    hipLaunchKernelGGL(lincomb_dev_kernel, dim3(g), dim3(256), 0, ctx->stream, Wv, V, vstride, sdev, n, cplx);
     dots(Wv, 1, Wv, dres);
     TNH_HIP(hipGetLastError());
     copy_device(ctx, V + vstride, V + int64_t(m) * vstride, sizeof(double) * vstride);
     hipLaunchKernelGGL(normalize_copy_kernel, dim3(g), dim3(256), 0, ctx->stream, V, Wv, dres, vstride);
     TNH_HIP(hipGetLastError());
-    std::fill(H.begin(), H.end(), cd(0));
-    h(0, 0) = theta_e;
-    h(1, 0) = b;
     start = 1;
   }
   // Ritz vector into theta, normalised
-  Coef yc{};
-  for (int i = 0; i < m; ++i) {
-    yc.re[i] = y[i].real();
-    yc.im[i] = y[i].imag();
-  }
-  hipLaunchKernelGGL(lincomb_kernel, dim3(g), dim3(256), 0, ctx->stream,
-                     static_cast<double *>(theta->ptr), V, vstride, m, yc, n, cplx);
+  hipLaunchKernelGGL(lincomb_dev_kernel, dim3(g), dim3(256), 0, ctx->stream, static_cast<double *>(theta->ptr), V,
+                     vstride, sdev, n, cplx);
   TNH_HIP(hipGetLastError());
   const double rn = norm_of(static_cast<double *>(theta->ptr));
   TNH_REQUIRE(std::isfinite(rn) && rn > 0, TNH_ERR_NUMERIC, "eigsolve: zero Ritz vector");
