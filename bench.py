@@ -475,9 +475,9 @@ def qjmc_tfim_n40_critical(ntraj=256, steps=150, sample_traj=4, nworkers=4, chi_
     critical dissipative TFIM, N = 40, g = 1, s-_i at rate 0.02, dt = 0.02, 150 second-order
     Trotter steps, cutoff 1e-10, chi_max 256, from all spins up, uniforms from seed 1. Bonds
     pass 16 near t = 2 and reach 31-32 by t = 3, so every
-    trajectory outgrows the narrow batched slots and is rerun on the wide ones (one
-    1024-thread workgroup per trajectory, divide-and-conquer splits); the time includes the
-    narrow run. The first `sample_traj` trajectories also run on the per-trajectory workers
+    trajectory outgrows the narrow batched slots and continues on the wide ones from the
+    start of the step it outgrew them in (one 1024-thread workgroup per trajectory,
+    divide-and-conquer splits); the time includes the narrow part. The first `sample_traj` trajectories also run on the per-trajectory workers
     (`nworkers` streams, one trajectory per stream), and the first `cpu_traj` through the
     oracle's MPS trajectory; both must reproduce the batched rows (jump records, <sz_i> to
     1e-8). Reports trajectory steps per second for each."""

@@ -336,10 +336,10 @@ int tnh_qjmc_run(tnh_ctx *ctx, const tnh_qjmc_problem *problem, int ntraj, const
  * fixed-capacity slots, and every launch gives each trajectory one workgroup that applies its
  * list of primitives (centre moves, two-site gate splits, one-site operators,
  * normalisation). Narrow slots (bond dimension at most min(16, 32 / d)) keep each split's
- * Gram eigensolve in LDS; trajectories that outgrow them are rerun inside this call, from the
- * start on their own uniforms, on wide slots (bond dimension 64, then up to 512 / d, capped by
- * maxdim and the chain's largest Schmidt rank): a 1024-thread workgroup per trajectory with
- * the divide-and-conquer Gram eigensolve. Replaces the per-trajectory loop over
+ * Gram eigensolve in LDS; trajectories that outgrow them continue inside this call, each from
+ * the start of the step it outgrew them in (replayed on its own uniforms), on wide slots
+ * (bond dimension 64, then up to 512 / d, capped by maxdim and the chain's largest Schmidt
+ * rank): a 1024-thread workgroup per trajectory with the divide-and-conquer Gram eigensolve. Replaces the per-trajectory loop over
  * qjmc_simulation (qjmc.jl:28-167) for dissipative chains with uneven bond dimensions. A
  * trajectory whose split would keep more than the largest slot capacity (or whose
  * divide and conquer fails) stops there: its maxbond entry is set to -1 and its other

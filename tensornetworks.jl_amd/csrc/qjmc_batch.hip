@@ -5,9 +5,10 @@
 // Gram matrix of a split is at most 32 x 32 and is diagonalised in LDS by the cyclic Jacobi
 // of jacobi_lds.hpp. Wide slots (CH <= 512 / d) give each trajectory a 1024-thread
 // workgroup, work matrices in HBM and the divide-and-conquer eigensolver of dc_eig.hpp; a
-// call runs every trajectory on narrow slots and reruns the ones that outgrew them, from the
-// start and on their own uniforms, on wide slots of bond dimension 64 and then of the largest
-// the chain and the truncation allow. The host keeps each trajectory's orthogonality centre
+// call runs every trajectory on narrow slots and continues the ones that outgrew them, each
+// from the start of the step it outgrew them in (its state and bookkeeping are kept at every
+// step start; the step is replayed on its own uniforms), on wide slots of bond dimension 64
+// and then of the largest the chain and the truncation allow. The host keeps each trajectory's orthogonality centre
 // and turns the reference's step (applygates!, gatelist.jl:191-223; normalize!; the emission
 // rates of :170-220; the jump of :103-107) into a list of primitives per trajectory: centre
 // moves (gmps.jl:90-112), two-site gates (gatelist.jl:137-171) and one-site operators. A
@@ -21,6 +22,7 @@
 #include <cmath>
 #include <complex>
 #include <deque>
+#include <memory>
 #include <numeric>
 #include <vector>
 
@@ -671,6 +673,27 @@ __global__ void qb_init_kernel(Batch b, const double *__restrict__ init, int ntr
   }
   for (int e = threadIdx.x; e <= b.N; e += blockDim.x) b.bond[int64_t(tr) * (b.N + 1) + e] = 1;
   if (threadIdx.x == 0) b.flags[tr] = 0;
+}
+
+// Trajectory src[w]'s site tensors and bond dimensions, from slots of capacity scap (S, sbond)
+// into trajectory dst[w]'s slots of capacity dcap >= its tensors (D, dbond): the state every
+// live trajectory has at the start of a step (kept for a continuation), and the move of an
+// overflowed trajectory into larger slots. Only each site's Dl d Dr entries are copied.
+__global__ void qb_copy_state_kernel(const double *__restrict__ S, int64_t scap, const int *__restrict__ sbond,
+                                     const int *__restrict__ src, double *__restrict__ D, int64_t dcap,
+                                     int *__restrict__ dbond, const int *__restrict__ dst, int N, int d) {
+  const int ts = src[blockIdx.x], td = dst[blockIdx.x];
+  const int *bs = sbond + int64_t(ts) * (N + 1);
+  for (int site = 1; site <= N; ++site) {
+    const int ne = bs[site - 1] * d * bs[site];
+    const double *a = S + (int64_t(ts) * N + (site - 1)) * 2 * scap;
+    double *b = D + (int64_t(td) * N + (site - 1)) * 2 * dcap;
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+      b[e] = a[e];
+      b[dcap + e] = a[scap + e];
+    }
+  }
+  for (int e = threadIdx.x; e <= N; e += blockDim.x) dbond[int64_t(td) * (N + 1) + e] = bs[e];
 }
 
 // ---- wide slots: bond dimension up to kDcPMax / d, one 1024-thread workgroup per trajectory
@@ -1398,14 +1421,55 @@ struct Owned {
   }
 };
 
+// A trajectory's state at the start of a step: where a run on larger slots continues it.
+struct Resume {
+  int step0 = 1;       // the first step to run
+  int64_t pos = 0;     // uniforms consumed before it
+  int save = 0, nj = 0;  // saves written and jumps recorded before it
+  int center = 0;      // orthogonality centre
+};
+
+// Trajectories that outgrew their slots, each with its Resume point and its site tensors and
+// bonds at that point (packed, slot capacity cap), held in context memory until released.
+struct Pack {
+  tnh_ctx *ctx = nullptr;
+  std::vector<int> rows;
+  std::vector<Resume> res;
+  int64_t cap = 0;
+  double *S = nullptr;
+  int *bond = nullptr;
+  size_t scap = 0, bcap = 0;
+  Pack() = default;
+  Pack(const Pack &) = delete;
+  Pack &operator=(const Pack &) = delete;
+  ~Pack() {
+    if (S) ctx->release(S, scap);
+    if (bond) ctx->release(bond, bcap);
+  }
+};
+
+// A view of trajectories [i0, i1) of a Pack, as the input of a run.
+struct PackView {
+  const std::vector<int> *rows;
+  const std::vector<Resume> *res;
+  int i0, i1;
+  int64_t cap;
+  const double *S;
+  const int *bond;
+};
+
 // The step loop of qjmc_simulation over the trajectories `rows` (their indices into the
 // caller's uniform and output arrays) on slots of bond dimension CH: narrow slots (the whole
-// state of a split in LDS, CH <= 32 / d) or wide ones (work matrices in HBM, d CH <= 512). A
-// trajectory that outgrows its slots (or whose wide split needs the general solvers) stops
-// with maxbond[row] = -1; the others complete.
-void run_engine(tnh_ctx *ctx, const tnh_qjmc_problem *p, const std::vector<int> &rows, const double *uniforms,
+// state of a split in LDS, CH <= 32 / d) or wide ones (work matrices in HBM, d CH <= 512).
+// Fresh trajectories start from the product state at step 1; with `in`, each continues from
+// its Resume point with its packed state. At the start of every step the state of each live
+// trajectory is kept; a trajectory that outgrows its slots in a step (or whose wide split
+// needs the general solvers) stops, and goes into `out` with its state and bookkeeping from
+// the start of that step (maxbond[row] = -1 until a larger run completes it).
+void run_engine(tnh_ctx *ctx, const tnh_qjmc_problem *p, std::vector<int> rows, const double *uniforms,
                 int64_t nuniform, double *meas, int *njumps, int *jumps, int *jump_steps, int maxjumps, int *maxbond,
-                int CH, bool wide) {
+                int CH, bool wide, const PackView *in, Pack *out) {
+  if (in) rows.assign(in->rows->begin() + in->i0, in->rows->begin() + in->i1);
   const int N = p->N, d = p->d, dd = d * d, ntraj = int(rows.size());
   Owned own{ctx, {}};
   Engine E;
@@ -1471,6 +1535,24 @@ void run_engine(tnh_ctx *ctx, const tnh_qjmc_problem *p, const std::vector<int> 
   to_device(ctx, init_dev, p->init_state, sizeof(double) * 2 * N * d);
   hipLaunchKernelGGL(qb_init_kernel, dim3(ntraj), dim3(64), 0, ctx->stream, E.b, init_dev, ntraj);
   TNH_HIP(hipGetLastError());
+  // index lists of the state copies: [0, ntraj) for both sides, then a live list
+  std::vector<int> iota_h(ntraj);
+  std::iota(iota_h.begin(), iota_h.end(), 0);
+  int *idx_dev = static_cast<int *>(own.take(sizeof(int) * 2 * size_t(ntraj)));
+  int *live_dev = idx_dev + ntraj;
+  to_device(ctx, idx_dev, iota_h.data(), sizeof(int) * ntraj);
+  std::vector<Resume> rs(ntraj);  // each trajectory's latest step start
+  if (in) {
+    TNH_REQUIRE(in->cap <= E.b.cap, TNH_ERR_ARG, "qjmc batched: continuation into smaller slots");
+    for (int t = 0; t < ntraj; ++t) rs[t] = (*in->res)[in->i0 + t];
+    hipLaunchKernelGGL(qb_copy_state_kernel, dim3(ntraj), dim3(256), 0, ctx->stream,
+                       in->S + int64_t(in->i0) * N * 2 * in->cap, in->cap, in->bond + int64_t(in->i0) * (N + 1),
+                       idx_dev, E.b.S, E.b.cap, E.b.bond, idx_dev, N, d);
+    TNH_HIP(hipGetLastError());
+  }
+  // the state at the start of the current step, per trajectory
+  double *snap = static_cast<double *>(own.take(sizeof(double) * size_t(ntraj) * N * 2 * E.b.cap));
+  int *snap_bond = static_cast<int *>(own.take(sizeof(int) * size_t(ntraj) * (N + 1)));
   // expectation terms: escape operators, then observables
   const int nesc = p->njump, nobs = p->nobs;
   std::vector<int> tsite;
@@ -1520,21 +1602,38 @@ void run_engine(tnh_ctx *ctx, const tnh_qjmc_problem *p, const std::vector<int> 
   const int nsave = p->nsteps / p->savesteps + 1;
   std::vector<int64_t> pos(ntraj, 0);
   std::vector<int> save(ntraj, 0), nj(ntraj, 0);
+  int first = 1;
+  if (in) {
+    first = p->nsteps + 1;
+    for (int t = 0; t < ntraj; ++t) {
+      pos[t] = rs[t].pos;
+      save[t] = rs[t].save;
+      nj[t] = rs[t].nj;
+      E.tq[t].center = rs[t].center;
+      first = std::min(first, rs[t].step0);
+    }
+  }
   auto draw = [&](int t) {
     TNH_REQUIRE(pos[t] < nuniform, TNH_ERR_ARG, "qjmc: not enough uniforms for a trajectory");
     return uniforms[size_t(rows[t]) * nuniform + pos[t]++];
   };
-  auto measure = [&]() {
+  // the observables of the trajectories that have run `step` (step 0, fresh runs only: the
+  // initial state of every trajectory)
+  auto measure = [&](int step) {
+    auto started = [&](int t) { return step == 0 || rs[t].step0 <= step; };
     if (nobs > 0) {
       expect(nesc, nobs);
-      for (int t = 0; t < ntraj; ++t)
+      for (int t = 0; t < ntraj; ++t) {
+        if (!started(t)) continue;
         for (int o = 0; o < nobs; ++o) {
           double *m = meas + (size_t(rows[t]) * nsave + save[t]) * 2 * nobs;
           m[2 * o] = outh[2 * (size_t(t) * nobs + o)];
           m[2 * o + 1] = outh[2 * (size_t(t) * nobs + o) + 1];
         }
+      }
     }
-    for (int t = 0; t < ntraj; ++t) ++save[t];
+    for (int t = 0; t < ntraj; ++t)
+      if (started(t)) ++save[t];
   };
   auto normalize_all = [&](const std::vector<int> &which) {
     for (int t : which) {
@@ -1544,23 +1643,35 @@ void run_engine(tnh_ctx *ctx, const tnh_qjmc_problem *p, const std::vector<int> 
     }
     E.run_queues();
   };
-  measure();
+  if (!in) measure(0);
   std::vector<double> rates(std::max(1, nesc));
   std::vector<char> dead(ntraj, 0);  // trajectories that outgrew their slots
-  for (int step = 1; step <= p->nsteps; ++step) {
-    for (int t = 0; t < ntraj; ++t)
-      if (!dead[t]) E.push_applygates(E.tq[t]);
-    E.run_queues();
+  std::vector<int> live_up;          // the live list on the device
+  for (int step = first; step <= p->nsteps; ++step) {
     std::vector<int> live;
     for (int t = 0; t < ntraj; ++t)
-      if (!dead[t]) live.push_back(t);
+      if (!dead[t] && rs[t].step0 <= step) live.push_back(t);
+    // every live trajectory's state at the start of this step: a trajectory that outgrows
+    // its slots in the step continues from here on larger ones
+    if (!live.empty()) {
+      if (live != live_up) {
+        to_device(ctx, live_dev, live.data(), sizeof(int) * live.size());
+        live_up = live;
+      }
+      hipLaunchKernelGGL(qb_copy_state_kernel, dim3(unsigned(live.size())), dim3(256), 0, ctx->stream, E.b.S,
+                         E.b.cap, E.b.bond, live_dev, snap, E.b.cap, snap_bond, live_dev, N, d);
+      TNH_HIP(hipGetLastError());
+    }
+    for (int t : live) rs[t] = Resume{step, pos[t], save[t], nj[t], E.tq[t].center};
+    for (int t : live) E.push_applygates(E.tq[t]);
+    E.run_queues();
     for (int t : live) (void)draw(t);  // qjmc.jl:64: drawn and unused
     normalize_all(live);
     check_flags(ctx, E, dead);
     std::vector<int> jumpers;
     if (nesc > 0) {
       expect(0, nesc);
-      for (int t = 0; t < ntraj; ++t) {
+      for (int t : live) {
         if (dead[t]) continue;
         double er = 0;
         for (int q = 0; q < nesc; ++q) {
@@ -1594,23 +1705,42 @@ void run_engine(tnh_ctx *ctx, const tnh_qjmc_problem *p, const std::vector<int> 
         }
       }
     } else {
-      for (int t = 0; t < ntraj; ++t)
+      for (int t : live)
         if (!dead[t]) (void)draw(t);  // the jump test draws even with no channels
     }
     if (!jumpers.empty()) {
       E.run_queues();
       normalize_all(jumpers);
+      check_flags(ctx, E, dead);  // a split of the jump's sweeps may overflow: same step
     }
-    if (step % p->savesteps == 0) measure();
+    if (step % p->savesteps == 0) measure(step);
   }
   check_flags(ctx, E, dead);
   std::vector<int> bonds(size_t(ntraj) * (N + 1));
   to_host(ctx, bonds.data(), E.b.bond, sizeof(int) * bonds.size());
+  std::vector<int> gone;
   for (int t = 0; t < ntraj; ++t) {
     njumps[rows[t]] = dead[t] ? 0 : nj[t];
     int mb = 0;
     for (int i = 1; i < N; ++i) mb = std::max(mb, bonds[size_t(t) * (N + 1) + i]);
     maxbond[rows[t]] = dead[t] ? -1 : mb;
+    if (dead[t]) gone.push_back(t);
+  }
+  if (out && !gone.empty()) {
+    // the overflowed trajectories' states at the start of the step they outgrew the slots in
+    const int ng = int(gone.size());
+    out->ctx = ctx;
+    out->cap = E.b.cap;
+    out->S = static_cast<double *>(ctx->alloc(sizeof(double) * size_t(ng) * N * 2 * E.b.cap, &out->scap));
+    out->bond = static_cast<int *>(ctx->alloc(sizeof(int) * size_t(ng) * (N + 1), &out->bcap));
+    to_device(ctx, live_dev, gone.data(), sizeof(int) * ng);
+    hipLaunchKernelGGL(qb_copy_state_kernel, dim3(ng), dim3(256), 0, ctx->stream, snap, E.b.cap, snap_bond,
+                       live_dev, out->S, E.b.cap, out->bond, idx_dev, N, d);
+    TNH_HIP(hipGetLastError());
+    for (int t : gone) {
+      out->rows.push_back(rows[t]);
+      out->res.push_back(rs[t]);
+    }
   }
 }
 
@@ -1651,11 +1781,14 @@ extern "C" int tnh_qjmc_run_batched(tnh_ctx *ctx, const tnh_qjmc_problem *p, int
     std::iota(rows.begin(), rows.end(), 0);
     // narrow slots: Gram side d CH <= 32 in LDS; environments CH x CH <= 256
     const int chn = std::min(16, QM / d);
-    run_engine(ctx, p, rows, uniforms, nuniform, meas, njumps, jumps, jump_steps, maxjumps, maxbond, chn, false);
+    std::vector<std::unique_ptr<Pack>> packs;
+    packs.emplace_back(new Pack());
+    run_engine(ctx, p, rows, uniforms, nuniform, meas, njumps, jumps, jump_steps, maxjumps, maxbond, chn, false,
+               nullptr, packs.back().get());
     if (d < 2) return;  // every bond of a d = 1 chain is 1
-    // Wide slots for the trajectories that outgrew the narrow ones, rerun from the start on
-    // their own uniforms: bond dimension 64, then the largest the chain and the truncation
-    // allow (at most 512 / d, the divide and conquer's side).
+    // Wide slots for the trajectories that outgrew the narrow ones, each continued from the
+    // start of the step it outgrew them in: bond dimension 64, then the largest the chain
+    // and the truncation allow (at most 512 / d, the divide and conquer's side).
     int cmax = dc::kDcPMax / d;
     int64_t full = 1;  // d^floor(N / 2), the largest Schmidt rank of the chain
     for (int i = 0; i < N / 2 && full < cmax; ++i) full *= d;
@@ -1665,16 +1798,19 @@ extern "C" int tnh_qjmc_run_batched(tnh_ctx *ctx, const tnh_qjmc_problem *p, int
     for (const int ch : {std::min(cmax, 64), cmax}) {
       if (ch <= prev) continue;
       prev = ch;
-      std::vector<int> dead;
-      for (int r = 0; r < ntraj; ++r)
-        if (maxbond[r] == -1) dead.push_back(r);
-      if (dead.empty()) break;
-      const double per = 8.0 * (double(N) * 2 * ch * d * ch + double(wide_stride(d * ch)) + 2.0 * (N + 1) + 64);
-      const size_t group = size_t(std::max(1.0, std::floor(kWideBudgetBytes / per)));
-      for (size_t g0 = 0; g0 < dead.size(); g0 += group) {
-        const std::vector<int> sub(dead.begin() + g0, dead.begin() + std::min(dead.size(), g0 + group));
-        run_engine(ctx, p, sub, uniforms, nuniform, meas, njumps, jumps, jump_steps, maxjumps, maxbond, ch, true);
-      }
+      std::vector<std::unique_ptr<Pack>> next;
+      // slots and their step-start copies, work matrices, per trajectory
+      const double per = 8.0 * (2.0 * N * 2 * ch * d * ch + double(wide_stride(d * ch)) + 2.0 * (N + 1) + 64);
+      const int group = int(std::max(1.0, std::floor(kWideBudgetBytes / per)));
+      for (auto &pk : packs)
+        for (int g0 = 0; g0 < int(pk->rows.size()); g0 += group) {
+          const PackView v{&pk->rows, &pk->res, g0, std::min(int(pk->rows.size()), g0 + group), pk->cap, pk->S,
+                           pk->bond};
+          next.emplace_back(new Pack());
+          run_engine(ctx, p, {}, uniforms, nuniform, meas, njumps, jumps, jump_steps, maxjumps, maxbond, ch, true,
+                     &v, next.back().get());
+        }
+      packs.swap(next);
     }
   });
 }

@@ -321,8 +321,9 @@ def test_qjmc_n40_workload_vs_oracle_trajectories(ctx):
 def test_qjmc_overflowing_rows_vs_oracle(ctx):
     """The regime above the narrow batched slots: dissipative TFIM at g = 1 (N = 10,
     gamma = 0.1, cutoff 1e-12, 70 steps of dt = 0.05), where most trajectories reach bond
-    dimension 24-32 and are rerun on the wide slots (one 1024-thread workgroup per
-    trajectory, divide-and-conquer splits) while the others complete in the narrow ones. Every row, batched or rerun, against the oracle MPS trajectory on its
+    dimension 24-32 and continue on the wide slots (one 1024-thread workgroup per
+    trajectory, divide-and-conquer splits) while the others complete in the narrow ones.
+    Every row, narrow or wide, against the oracle MPS trajectory on its
     uniforms: identical jump records, bond dimensions, <sz_i> to 1e-8."""
     from tnhip import spinhalf
     from tnhip.qjmc import qjmc_trajectories

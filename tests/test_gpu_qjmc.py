@@ -115,7 +115,7 @@ def test_batched_matches_workers_truncated(ctx):
 
 def test_batched_wide_slots_without_truncation(ctx):
     """Without truncation the bond dimension of a 10-site chain reaches 32, above the narrow
-    batched slots (16 for spin-1/2): those trajectories are rerun inside the batched call on
+    batched slots (16 for spin-1/2): those trajectories continue inside the batched call on
     the wide slots (bond dimension 64, the divide-and-conquer split per trajectory), so the
     batched engine completes them itself and matches the per-trajectory workers. Untruncated
     splits resolve singular values near sqrt(eps) s_max only to that level (the Gram path's
@@ -139,8 +139,8 @@ def test_batched_wide_slots_without_truncation(ctx):
 def test_batched_partial_overflow(ctx):
     """Dissipative TFIM (g = 1, gamma = 0.1, cutoff 1e-12), 8 trajectories of 70 steps: most
     trajectories reach bond dimension 24-32, above the narrow slots (16 for spin-1/2), while
-    a few stay at 16 and complete there. The others are rerun on the wide slots in the same
-    call, and every trajectory matches the workers' own run: jump records exactly, bond
+    a few stay at 16 and complete there. The others continue on the wide slots in the same
+    call (from the start of the step they outgrew the narrow ones in), and every trajectory matches the workers' own run: jump records exactly, bond
     dimensions, measurements to 1e-8. At cutoff 1e-12 the kept singular values reach ~1e-6
     s_max, whose vectors the Gram path resolves only to ~eps s_max^2 / s_k^2; the engines
     form their Gram matrices in different orders, so after 70 steps their states differ at
@@ -167,8 +167,8 @@ def test_batched_wide_slots_second_step(ctx):
     """Weakly dissipative critical TFIM, N = 14 (g = 1, gamma = 0.02, cutoff 1e-10), 4
     trajectories of 84 steps of dt = 0.05: the central bond passes 16 near t = 2 and 64 near
     t = 3.7 (up to 90-128 without jumps), so trajectories go narrow slots -> wide slots of 64
-    -> wide slots of 128 (Gram sides up to 256), each rerun from the start on its own
-    uniforms. Against the per-trajectory workers: jump records, bond dimensions, measurements
+    -> wide slots of 128 (Gram sides up to 256), each continued from the start of the step
+    it outgrew the smaller slots in. Against the per-trajectory workers: jump records, bond dimensions, measurements
     to 1e-8 (the cross-engine bound of the truncated runs above)."""
     from tnhip import spinhalf
     from tnhip.qjmc import qjmc_trajectories
