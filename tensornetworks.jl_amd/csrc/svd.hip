@@ -1361,7 +1361,7 @@ static double *decompose(tnh_ctx *ctx, const double *Mp, int m, int n, bool cplx
       scale(ctx, G, TNH_F64, pp * planes, std::ldexp(1.0, -ex));
       dmax = std::ldexp(dmax, -ex);
     }
-    const double abstol = 4.0 * 2.220446049250313e-16 * std::sqrt(double(p)) * dmax;
+    const double abstol = std::max(4.0 * 2.220446049250313e-16 * std::sqrt(double(p)), ctx->gram_floor) * dmax;
     hipEvent_t pj = ctx->prof_begin();
     JacobiResult jr = jacobi(ctx, true, G, p, p, cplx, V, tol, abstol, 40);
     ctx->prof_end(use_warm ? "jacobi_warm" : "jacobi", pj, double(jr.sweeps), double(p));

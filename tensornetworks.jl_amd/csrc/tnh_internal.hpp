@@ -110,6 +110,10 @@ struct tnh_ctx {
     double flops, bytes;
   };
   bool profiling = false;
+  // Lowest stopping floor of the block-Jacobi Gram eigensolve, relative to max_i G_ii (0: the
+  // FP64 rounding floor alone). An FP32 DMRG pass raises it for its own splits: its theta
+  // carries far more error than the FP64 floor resolves.
+  double gram_floor = 0.0;
   std::map<std::string, std::vector<ProfRec>> prof_pending;
   struct ProfSum {
     int64_t launches = 0;

@@ -292,6 +292,36 @@ def test_dmrg_mixed_precision_refined(ctx, rng, precision):
     assert abs(e - tfim_critical_ground_energy(N)) / N <= 1e-10
 
 
+def test_dmrg_f32_pass_gram_floor_refined(ctx):
+    """An FP32 pass at bond dimension 300 (Gram sides 600, block Jacobi: the FP32 pass stops
+    its eigensolves at 1e-9 max G_ii instead of the FP64 rounding floor) followed by one FP64
+    pass lands on the energy of two FP64 passes from the same state (a copy of it) to
+    1e-10 per site: the critical TFIM at N = 20, bonds forced to 300 from a state converged
+    at chi 32."""
+    from tnhip import MPO, dmrg, randomMPS, spinhalf
+    from tnhip.models import GMPS
+    from tnhip.mps import DMRGState
+
+    N, chi = 20, 300
+    M = MPO(spinhalf(), tfim_H(N))
+    psi = randomMPS(2, N, 8, np.random.default_rng(3))
+    psi, _ = dmrg(psi, M, maxdim=32, cutoff=1e-12, maxsweeps=6, minsweeps=6, tol=0.0, verbose=False)
+    st = DMRGState(psi, [M])
+    for dr in (False, True) * 3:  # grow the bonds to chi (centre back at site 1)
+        st.sweep(dr, 3, 2, 1e-14, 0.0, chi, chi, "f64")
+    assert max(st.bonddims()) == chi
+    start = GMPS(1, 2, st.tensors(), st.center())
+    st.sweep(False, 3, 2, 1e-14, 0.0, chi, chi, "f32")
+    e_mixed, _, _ = st.sweep(True, 3, 2, 1e-14, 0.0, chi, chi, "f64")
+    st.close()
+    twin = DMRGState(start, [M])
+    twin.sweep(False, 3, 2, 1e-14, 0.0, chi, chi, "f64")
+    e_f64, _, _ = twin.sweep(True, 3, 2, 1e-14, 0.0, chi, chi, "f64")
+    twin.close()
+    print(f"refined {e_mixed:.13f} vs two FP64 passes {e_f64:.13f}", flush=True)
+    assert abs(e_mixed - e_f64) / N <= 1e-10
+
+
 def test_dmrg_two_terms_sum(ctx, rng):
     """ProjMPSSum: two MPO terms with coefficients (projmpssum.jl:63-73)."""
     from tnhip import MPO, dmrg, randomMPS, spinhalf
