@@ -372,7 +372,10 @@ int tnh_dmrg_destroy(tnh_dmrg *st);
 /* One directional pass (the reference's "sweep", dmrg.jl:35-63): for each bond,
  * move the environment centre, form theta, eigsolve, replacesites!. direction 0 =
  * left-to-right. Returns the last Ritz value, the max bond dim and the max
- * discarded weight of the pass. */
+ * discarded weight of the pass. In a TNH_PREC_F32 pass the block-Jacobi Gram
+ * eigensolves of the splits (Gram sides above 512) stop at 1e-9 max_i G_ii, the scale
+ * below which the pass's own FP32 rounding already blurs theta; FP64 passes use the FP64
+ * rounding floor, and a following FP64 pass restores full accuracy. */
 int tnh_dmrg_sweep(tnh_dmrg *st, int direction, const tnh_dmrg_params *p, double *energy,
                    int *maxbonddim, double *max_discarded);
 /* Current energy <psi|H|psi> at the centre: calculate(Hs), dmrg.jl:22. */
