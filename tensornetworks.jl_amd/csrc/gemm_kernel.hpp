@@ -76,7 +76,7 @@ template <>
 struct ModeTraits<MODE_F32> {
   using S = float;
   static constexpr int BK = 32;
-  static constexpr int KPAD = 4;
+  static constexpr int KPAD = 0;  // [row][k] tiles: unpadded rows, chunks permuted (kc_off)
 };
 template <>
 struct ModeTraits<MODE_BF16P> {
@@ -105,6 +105,21 @@ template <>
 struct Vec16<double> {
   using type = v2d;
 };
+
+// Offset of element (r, k) in a [row][k] LDS tile of row stride LDK. FP32 rows are 32 floats
+// (eight 16-byte k-chunks) with chunk c of row r at position c ^ ((r >> 1) & 7): a staging
+// store (8 lanes per row, two rows per 16 lanes) and a fragment read (one chunk of 16
+// consecutive rows) then both touch 64 distinct banks. The padded 36-float rows this
+// replaces kept the reads conflict-free but put the two rows of a store group on
+// overlapping banks (one SQ_LDS_BANK_CONFLICT cycle per LDS instruction in the stage-1 GEMM
+// of the matvec). FP64 rows keep their padding.
+template <int MODE, int LDK>
+__device__ __forceinline__ int kc_off(int r, int k) {
+  if constexpr (MODE == MODE_F32)
+    return r * LDK + ((((k >> 2) ^ ((r >> 1) & 7)) << 2) | (k & 3));
+  else
+    return r * LDK + k;
+}
 
 // One operand tile (ROWS x BK) of type T in global memory; KCG = k-contiguous globally.
 template <int MODE, typename T, bool KCG, int ROWS>
@@ -184,7 +199,7 @@ struct Tile {
       for (int e = 0; e < NV; ++e) {
         int r, k;
         pos(true, e * VW, r, k);
-        S *d = LDS_KC ? &lds[r * LDK + k] : &lds[k * LDR + r];
+        S *d = LDS_KC ? &lds[kc_off<MODE, LDK>(r, k)] : &lds[k * LDR + r];
         if constexpr (std::is_same<S, float>::value && VW == 4) {
           *reinterpret_cast<v4f *>(d) = v4f{v[e * VW], v[e * VW + 1], v[e * VW + 2], v[e * VW + 3]};
         } else if constexpr (std::is_same<S, float>::value) {  // two FP64 inputs
@@ -198,7 +213,7 @@ struct Tile {
       for (int e = 0; e < NE; ++e) {
         int r, k;
         pos(false, e, r, k);
-        (LDS_KC ? lds[r * LDK + k] : lds[k * LDR + r]) = S(v[e]);
+        (LDS_KC ? lds[kc_off<MODE, LDK>(r, k)] : lds[k * LDR + r]) = S(v[e]);
       }
     }
   }
@@ -263,7 +278,7 @@ __device__ __forceinline__ void frag4(const typename ModeTraits<MODE>::S *__rest
                                       int c, int g, typename ModeTraits<MODE>::S (&f)[4]) {
   using S = typename ModeTraits<MODE>::S;
   if constexpr (LDS_KC) {
-    const S *q = &lds[row * LDK + 16 * c + 4 * g];
+    const S *q = &lds[kc_off<MODE, LDK>(row, 16 * c + 4 * g)];
     if constexpr (std::is_same<S, float>::value) {
       const v4f x = *reinterpret_cast<const v4f *>(q);
       f[0] = x[0];
