@@ -365,9 +365,19 @@ typedef struct tnh_dmrg_params {
 /* Device-resident state of dmrg(psi, H::GMPS...), src/algorithms/mps/dmrg.jl:128-154:
  * copies the N site tensors of psi (any dtype; stored F64 or C128) and the nterms
  * MPOs (site tensors mpo[t*N + i]) with coefficients coeffs[t]; moves the centre of
- * psi to site 1 and builds all right blocks (projmps.jl:16-42). */
+ * psi to site 1 and builds all right blocks (projmps.jl:16-42). nterms may be 0 when
+ * rank-1 terms are added before the first sweep. */
 int tnh_dmrg_create(tnh_ctx *ctx, int N, const tnh_buf *const *psi, int nterms,
                     const tnh_buf *const *mpo, const double *coeffs, tnh_dmrg **out);
+/* Add a rank-1 Hamiltonian term coeff |phi><phi|: ProjMPS(phi, psi; rank=2, squared=true,
+ * coeff), which dmrg(psi, Hs...) builds for each rank-1 argument (dmrg.jl:144-145), with
+ * overlap blocks (projmps.jl:50-95), the product coeff * conj(P) * sum(P .* theta) of
+ * projmps.jl:135-144 (P = project(...), :152-179) and calculate = coeff <phi|psi>
+ * (:192-216). phi: N site tensors (chi_l, d, chi_r) of psi's physical dimension, copied in
+ * (a complex phi needs a state created from complex tensors). The overlap blocks are built
+ * at the state's current centre and kept in FP64 in every precision; the term joins the
+ * sum after the MPO terms. */
+int tnh_dmrg_add_projector(tnh_dmrg *st, const tnh_buf *const *phi, double coeff);
 int tnh_dmrg_destroy(tnh_dmrg *st);
 /* One directional pass (the reference's "sweep", dmrg.jl:35-63): for each bond,
  * move the environment centre, form theta, eigsolve, replacesites!. direction 0 =
@@ -378,7 +388,8 @@ int tnh_dmrg_destroy(tnh_dmrg *st);
  * rounding floor, and a following FP64 pass restores full accuracy. */
 int tnh_dmrg_sweep(tnh_dmrg *st, int direction, const tnh_dmrg_params *p, double *energy,
                    int *maxbonddim, double *max_discarded);
-/* Current energy <psi|H|psi> at the centre: calculate(Hs), dmrg.jl:22. */
+/* Current energy <psi|H|psi> at the centre: calculate(Hs), dmrg.jl:22 (a rank-1 term
+ * contributes coeff Re<phi|psi>, as its calculate does). */
 int tnh_dmrg_energy(tnh_dmrg *st, int precision, double *energy);
 /* Borrow site tensor `site` (1-based). Valid until the next sweep. */
 int tnh_dmrg_site(tnh_dmrg *st, int site, const tnh_buf **out);

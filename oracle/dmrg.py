@@ -5,22 +5,35 @@ import numpy as np
 
 from .krylov import eigsolve_ground
 from .mps import movecenter, replacesites2
-from .projmps import ProjMPS
+from .projmps import ProjMPS, ProjMPSSquared, ProjMPSSum
 
 
 def dmrg(psi, Ws, nsites=2, krylovdim=3, kryloviter=2, minsweeps=1, maxsweeps=1000,
          tol=1e-10, tolgrad=1e-5, numconverges=4, cutoff=1e-12, maxdim=1000, mindim=1,
-         verbose=False, coeff=1.0, history=None):
-    """dmrg(psi, H::GMPS; kwargs...), src/algorithms/mps/dmrg.jl:128-154 and :1-99.
+         verbose=False, coeff=1.0, history=None, terms=()):
+    """dmrg(psi, H::GMPS...; kwargs...), src/algorithms/mps/dmrg.jl:128-154 and :1-99.
 
-    psi is an oracle MPS updated in place; Ws the MPO site tensors. Returns
-    (psi, cost). `history` (a list) receives (sweep, cost, maxbonddim, max discarded).
-    A "sweep" is one directional pass, as in the reference (:66).
+    psi is an oracle MPS updated in place; Ws the MPO site tensors (with `coeff`).
+    `terms` holds further Hamiltonian terms as (site tensors, coeff): MPO tensors (4
+    indices) give ProjMPS(psi, H, psi; rank=2), MPS tensors (3 indices) the rank-1 term
+    ProjMPS(phi, psi; rank=2, squared=true) (:140-147); all are summed as ProjMPSSum.
+    Returns (psi, cost). `history` (a list) receives (sweep, cost, maxbonddim, max
+    discarded). A "sweep" is one directional pass, as in the reference (:66).
     """
     assert nsites == 2, "only the two-site update is on the hot path"
     N = len(psi)
     movecenter(psi, 1)                                       # :133
-    Hs = ProjMPS(psi.tensors, Ws, center=1, coeff=coeff)     # :143, projmps.jl:16-42
+    projs = [ProjMPS(psi.tensors, Ws, center=1, coeff=coeff)]  # :143, projmps.jl:16-42
+    for T, c in terms:                                       # :139-149
+        if len(T) != N:
+            raise ValueError("GMPS must share the same properties.")
+        if T[0].ndim == 4:
+            projs.append(ProjMPS(psi.tensors, T, center=1, coeff=c))
+        elif T[0].ndim == 3:
+            projs.append(ProjMPSSquared(T, psi.tensors, center=1, coeff=c))
+        else:
+            raise ValueError("Hamiltonian must be composed of MPOs (rank 2) or MPSs (rank 1).")
+    Hs = projs[0] if len(projs) == 1 else ProjMPSSum(projs)  # :151-152
     cost = Hs.calculate()                                    # :22
     lastcost = cost
     D = psi.maxbonddim()

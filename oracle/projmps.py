@@ -133,3 +133,120 @@ class ProjMPS:
         site = self.center
         return calculate(self.block(site - 1), self.psi[site - 1], self.Ws[site - 1],
                          self.block(site + 1), self.coeff)
+
+
+# ----------------------------------------------- rank-1 ("squared") terms and sums
+# dmrg(psi, Hs...) turns a rank-1 argument phi into ProjMPS(phi, psi; rank=2, squared=true,
+# coeff), src/algorithms/mps/dmrg.jl:144-145: a bra-ket pair with no MPO between, whose
+# blocks are 2-index overlaps L[a_phi, a_psi] and whose product is coeff |phi_p><phi_p| A.
+
+
+def edgeblock2():
+    """edgeblock(2), src/structures/mps/abstractprojmps.jl:33-36."""
+    return np.ones((1, 1), dtype=np.float64)
+
+
+def buildleft_overlap(L, phi, psi):
+    """buildleft!(projV, idx) for objects [phi, psi], src/structures/mps/projmps.jl:50-66
+    (the MPO loop of :58-61 is empty)."""
+    prod = contract(L, np.conj(phi), 1, 1)                 # :57
+    return contract(prod, psi, [1, prod.ndim - 1], [1, 2])  # :62
+
+
+def buildright_overlap(R, phi, psi):
+    """buildright!(projV, idx) for objects [phi, psi], src/structures/mps/projmps.jl:74-95."""
+    prod = contract(psi, R, 3, R.ndim)                     # :86
+    return contract(np.conj(phi), prod, [2, 3], [2, prod.ndim])  # :91
+
+
+def project(L, phi1, phi2, R):
+    """project(projV, A, direction, nsites=2) for objects [phi, psi],
+    src/structures/mps/projmps.jl:152-179: phi's two sites, conjugated, between the
+    overlap blocks; the result has theta's shape (chi_l, d, d, chi_r) of psi."""
+    nsites = 2
+    prod = moveidx(L, L.ndim, 1)                            # :160
+    for i, ph in ((1, phi1), (2, phi2)):                    # :163-173
+        prod = contract(prod, np.conj(ph), 1 + i, 1)        # :165
+        prod = moveidx(prod, prod.ndim - 1, 1 + i)          # :172
+    return contract(prod, R, list(range(2 + nsites, prod.ndim + 1)),
+                    list(range(1, R.ndim)))                 # :176-177
+
+
+def product_squared(L, phi1, phi2, R, A, coeff=1.0):
+    """product(projV, A, direction, nsites=2), squared branch, projmps.jl:135-144:
+    coeff * conj(P) * sum(P .* A) with P = project(...)."""
+    P = project(L, phi1, phi2, R)                           # :137
+    s = contract(P, A, list(range(1, P.ndim + 1)), list(range(1, A.ndim + 1)))[()]  # :140
+    return coeff * (s * np.conj(P))                         # :139, :141, :144
+
+
+def calculate_overlap(L, phi, psi, R, coeff=1.0):
+    """calculate(projV) for objects [phi, psi], projmps.jl:192-216: coeff <phi|psi>."""
+    prod = buildleft_overlap(L, phi, psi)
+    prod = contract(prod, R, list(range(1, prod.ndim + 1)), list(range(1, prod.ndim + 1)))
+    return coeff * prod.reshape(-1)[0]
+
+
+class ProjMPSSquared:
+    """ProjMPS(phi, psi; rank=2, squared=true, coeff), src/structures/mps/projmps.jl:16-42,
+    with overlap blocks per site and a movable centre (abstractprojmps.jl:33-81). `psi` is
+    shared by reference with the DMRG driver; `phi` is fixed."""
+
+    def __init__(self, phi, psi, center=1, coeff=1.0):
+        if len(phi) != len(psi):
+            raise ValueError("GMPS must share the same length.")
+        self.phi = phi
+        self.psi = psi
+        self.coeff = coeff
+        self.blocks = [edgeblock2() for _ in psi]
+        self.center = 0
+        self.movecenter(center)
+
+    def __len__(self):
+        return len(self.psi)
+
+    def block(self, idx):
+        if idx < 1 or idx > len(self):
+            return edgeblock2()
+        return self.blocks[idx - 1]
+
+    def buildleft(self, idx):
+        self.blocks[idx - 1] = buildleft_overlap(self.block(idx - 1), self.phi[idx - 1],
+                                                 self.psi[idx - 1])
+
+    def buildright(self, idx):
+        self.blocks[idx - 1] = buildright_overlap(self.block(idx + 1), self.phi[idx - 1],
+                                                  self.psi[idx - 1])
+
+    movecenter = ProjMPS.movecenter
+
+    def product(self, A, direction=False, nsites=2):
+        site = self.center - nsites + 1 if direction else self.center
+        return product_squared(self.block(site - 1), self.phi[site - 1], self.phi[site],
+                               self.block(site + nsites), A, self.coeff)
+
+    def calculate(self):
+        site = self.center
+        return calculate_overlap(self.block(site - 1), self.phi[site - 1], self.psi[site - 1],
+                                 self.block(site + 1), self.coeff)
+
+
+class ProjMPSSum:
+    """ProjMPSSum(projs), src/structures/mps/projmpssum.jl: movecenter!, product and
+    calculate summed over the terms in order (:63-73)."""
+
+    def __init__(self, projs):
+        self.projs = list(projs)
+
+    def movecenter(self, idx):
+        for p in self.projs:
+            p.movecenter(idx)
+
+    def product(self, A, direction=False, nsites=2):
+        out = self.projs[0].product(A, direction, nsites)
+        for p in self.projs[1:]:
+            out = out + p.product(A, direction, nsites)
+        return out
+
+    def calculate(self):
+        return sum(p.calculate() for p in self.projs)

@@ -28,6 +28,26 @@ struct tnh_dmrg {
   std::vector<tnh_buf *> warm[2];
   int env_center = 0;   // ProjMPS centre (0 = blocks not built)
   int psi_center = 0;
+  bool cplx = false;    // psi (and every term) held complex
+  // Rank-1 terms coeff |phi><phi|, ProjMPS(phi, psi; rank=2, squared=true, coeff)
+  // (dmrg.jl:144-145): phi's site tensors, the overlap blocks per site, and the projection of
+  // phi onto the bond being updated.
+  struct Proj {
+    std::vector<tnh_buf *> phi, blocks;
+    double coeff = 1.0;
+    tnh_buf *P = nullptr;
+  };
+  std::vector<Proj> proj;
+  tnh_buf *one = nullptr;  // the 1 x 1 overlap block beyond the chain ends (edgeblock(2))
+
+  void free_proj(Proj &q) {
+    for (auto &b : q.phi) buf_delete(ctx, b);
+    for (auto &b : q.blocks) buf_delete(ctx, b);
+    buf_delete(ctx, q.P);
+    q.phi.clear();
+    q.blocks.clear();
+    q.P = nullptr;
+  }
 
   ~tnh_dmrg() {
     auto del = [&](tnh_buf *&b) {
@@ -45,6 +65,8 @@ struct tnh_dmrg {
     del(tmp);
     for (auto &v : warm)
       for (auto &b : v) del(b);
+    for (auto &q : proj) free_proj(q);
+    del(one);
   }
 
   // block(projV, idx), abstractprojmps.jl:33-46 (nullptr = edge block)
@@ -53,13 +75,22 @@ struct tnh_dmrg {
     return blocks[t][idx - 1];
   }
 
+  const tnh_buf *pblock(const Proj &q, int idx) const {
+    if (idx < 1 || idx > N) return one;
+    return q.blocks[idx - 1];
+  }
+
   void buildleft(int idx, int prec) {
     for (int t = 0; t < nterms; ++t)
       env_left(ctx, block(t, idx - 1), psi[idx - 1], W[t][idx - 1], &blocks[t][idx - 1], prec);
+    for (auto &q : proj)  // overlap blocks stay FP64 in every pass
+      proj_env_left(ctx, pblock(q, idx - 1), q.phi[idx - 1], psi[idx - 1], &q.blocks[idx - 1]);
   }
   void buildright(int idx, int prec) {
     for (int t = 0; t < nterms; ++t)
       env_right(ctx, block(t, idx + 1), psi[idx - 1], W[t][idx - 1], &blocks[t][idx - 1], prec);
+    for (auto &q : proj)
+      proj_env_right(ctx, pblock(q, idx + 1), q.phi[idx - 1], psi[idx - 1], &q.blocks[idx - 1]);
   }
 
   // movecenter!(projV, idx), abstractprojmps.jl:60-81
@@ -90,9 +121,11 @@ void copy_in(tnh_ctx *ctx, const tnh_buf *src, bool cplx, tnh_buf **dst) {
   }
 }
 
-// y = sum_t coeff_t H_t x for the bond (site1, site1 + 1)
+// y = sum_t coeff_t H_t x + sum_q coeff_q |phi_q><phi_q| x for the bond (site1, site1 + 1),
+// product(::ProjMPSSum), projmpssum.jl:63-73 (the rank-1 terms after the MPO terms)
 void bond_matvec(tnh_dmrg *st, int site1, int prec, const tnh_buf *x, tnh_buf *y) {
   tnh_ctx *ctx = st->ctx;
+  if (st->nterms == 0) fill(ctx, y->ptr, y->dtype, y->numel, 0.0);
   for (int t = 0; t < st->nterms; ++t) {
     const tnh_buf *L = st->block(t, site1 - 1);
     const tnh_buf *R = st->block(t, site1 + 2);
@@ -110,6 +143,7 @@ void bond_matvec(tnh_dmrg *st, int site1, int prec, const tnh_buf *x, tnh_buf *y
             static_cast<const double *>(st->tmp->ptr), 1.0, static_cast<double *>(y->ptr));
     }
   }
+  for (const auto &q : st->proj) proj_apply(ctx, q.P, x, y, q.coeff);
 }
 
 void make_shadows(tnh_dmrg *st, int site1) {
@@ -143,8 +177,8 @@ int tnh_dmrg_create(tnh_ctx *ctx, int N, const tnh_buf *const *psi, int nterms,
   st->ctx = ctx;
   int rc = tnh_guard(ctx, [&] {
     TNH_REQUIRE(N >= 2, TNH_ERR_ARG, "two-site DMRG needs N >= 2");
-    TNH_REQUIRE(nterms >= 1, TNH_ERR_ARG, "You must provide atleast one MPS/MPO for the Hamiltonian.");
-    TNH_REQUIRE(psi && mpo, TNH_ERR_ARG, "NULL psi or mpo array");
+    TNH_REQUIRE(nterms >= 0, TNH_ERR_ARG, "negative number of MPO terms");
+    TNH_REQUIRE(psi && (mpo || nterms == 0), TNH_ERR_ARG, "NULL psi or mpo array");
     st->N = N;
     st->nterms = nterms;
     bool cplx = false;
@@ -174,6 +208,7 @@ int tnh_dmrg_create(tnh_ctx *ctx, int N, const tnh_buf *const *psi, int nterms,
       TNH_REQUIRE(mpo[size_t(t) * N]->dims[0] == 1 && mpo[size_t(t) * N + N - 1]->dims[3] == 1,
                   TNH_ERR_SHAPE, "MPO edge bonds must be 1");
     }
+    st->cplx = cplx;
     st->psi.assign(N, nullptr);
     for (int i = 0; i < N; ++i) copy_in(ctx, psi[i], cplx, &st->psi[i]);
     st->W.assign(nterms, std::vector<tnh_buf *>(N, nullptr));
@@ -202,6 +237,52 @@ int tnh_dmrg_create(tnh_ctx *ctx, int N, const tnh_buf *const *psi, int nterms,
   return TNH_OK;
 }
 
+int tnh_dmrg_add_projector(tnh_dmrg *st, const tnh_buf *const *phi, double coeff) {
+  if (!st || !phi) return TNH_ERR_ARG;
+  tnh_ctx *ctx = st->ctx;
+  return tnh_guard(ctx, [&] {
+    const int N = st->N;
+    const int64_t d = st->psi[0]->dims[1];
+    bool pc = false;
+    for (int i = 0; i < N; ++i) {
+      TNH_REQUIRE(phi[i] && phi[i]->ndim == 3, TNH_ERR_SHAPE,
+                  "Hamiltonian must be composed of MPOs (rank 2) or MPSs (rank 1).");
+      TNH_REQUIRE(phi[i]->dims[1] == d, TNH_ERR_SHAPE, "GMPS must share the same properties.");
+      pc |= is_complex(phi[i]->dtype);
+    }
+    for (int i = 0; i + 1 < N; ++i)
+      TNH_REQUIRE(phi[i]->dims[2] == phi[i + 1]->dims[0], TNH_ERR_SHAPE, "phi bond dims do not match");
+    TNH_REQUIRE(phi[0]->dims[0] == 1 && phi[N - 1]->dims[2] == 1, TNH_ERR_SHAPE,
+                "phi edge bonds must be 1");
+    TNH_REQUIRE(!pc || st->cplx, TNH_ERR_ARG,
+                "a complex rank-1 term needs a complex psi (create the state from complex tensors)");
+    if (!st->one) {
+      const int64_t od[2] = {1, 1};
+      st->one = buf_new(ctx, TNH_F64, 2, od);
+      fill(ctx, st->one->ptr, TNH_F64, 1, 1.0);
+    }
+    st->proj.emplace_back();
+    tnh_dmrg::Proj &q = st->proj.back();
+    q.coeff = coeff;
+    q.phi.assign(N, nullptr);
+    q.blocks.assign(N, nullptr);
+    try {
+      for (int i = 0; i < N; ++i) copy_in(ctx, phi[i], st->cplx, &q.phi[i]);
+      // ProjMPS(...; center) builds every block on its side of the centre, projmps.jl:38-40
+      const int c = std::max(st->env_center, 1);
+      for (int i = 1; i < c; ++i)
+        proj_env_left(ctx, st->pblock(q, i - 1), q.phi[i - 1], st->psi[i - 1], &q.blocks[i - 1]);
+      for (int i = N; i > c; --i)
+        proj_env_right(ctx, st->pblock(q, i + 1), q.phi[i - 1], st->psi[i - 1], &q.blocks[i - 1]);
+      TNH_HIP(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+      st->free_proj(q);
+      st->proj.pop_back();
+      throw;
+    }
+  });
+}
+
 int tnh_dmrg_destroy(tnh_dmrg *st) {
   delete st;
   return TNH_OK;
@@ -226,6 +307,8 @@ int tnh_dmrg_sweep(tnh_dmrg *st, int direction, const tnh_dmrg_params *p, double
     const int prec = p->precision;
     TNH_REQUIRE(prec >= TNH_PREC_F64 && prec <= TNH_PREC_BF16X3, TNH_ERR_ARG, "bad precision");
     TNH_REQUIRE(p->maxdim >= 0 && p->mindim >= 0, TNH_ERR_ARG, "maxdim, mindim must be >= 0");
+    TNH_REQUIRE(st->nterms + st->proj.size() >= 1, TNH_ERR_ARG,
+                "You must provide atleast one MPS/MPO for the Hamiltonian.");
     double cost = 0, maxdisc = 0;
     for (int j = 1; j <= N - 1; ++j) {  // dmrg.jl:35
       const int site = direction ? N + 1 - j : j;
@@ -239,6 +322,9 @@ int tnh_dmrg_sweep(tnh_dmrg *st, int direction, const tnh_dmrg_params *p, double
       // the psi centre must sit on the bond: it does after the previous split
       two_site_theta(ctx, st->psi[site1 - 1], st->psi[site1], &st->theta);  // :44-47
       if (prec != TNH_PREC_F64) make_shadows(st, site1);
+      for (auto &q : st->proj)  // project(projV, A, direction, 2), projmps.jl:152-179
+        proj_project(ctx, st->pblock(q, site1 - 1), q.phi[site1 - 1], q.phi[site1],
+                     st->pblock(q, site1 + 2), &q.P);
       int nmv = 0;
       cost = eigsolve_ground_op(
           ctx, [&](const tnh_buf *x, tnh_buf *y) { bond_matvec(st, site1, prec, x, y); },
@@ -287,6 +373,13 @@ int tnh_dmrg_energy(tnh_dmrg *st, int precision, double *energy) {
       double im = 0;
       e += st->coeffs[t] * env_calculate(st->ctx, st->block(t, c - 1), st->psi[c - 1],
                                          st->W[t][c - 1], st->block(t, c + 1), &im);
+    }
+    // a rank-1 term's calculate is coeff <phi|psi> (projmps.jl:192-216 with objects [phi, psi])
+    for (const auto &q : st->proj) {
+      double re = 0, im = 0;
+      proj_calculate(st->ctx, st->pblock(q, c - 1), q.phi[c - 1], st->psi[c - 1],
+                     st->pblock(q, c + 1), &re, &im);
+      e += q.coeff * re;
     }
     if (energy) *energy = e;
   });

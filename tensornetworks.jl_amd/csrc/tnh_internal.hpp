@@ -245,6 +245,16 @@ void env_right(tnh_ctx *ctx, const tnh_buf *Rp, const tnh_buf *A, const tnh_buf 
 double env_calculate(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *A, const tnh_buf *W,
                      const tnh_buf *R, double *im_out);
 void two_site_theta(tnh_ctx *ctx, const tnh_buf *A1, const tnh_buf *A2, tnh_buf **theta);
+// rank-1 Hamiltonian terms (projector.hip): 2-index overlap blocks B[a_phi, a_psi] (FP64)
+void proj_env_left(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *phi, const tnh_buf *psi,
+                   tnh_buf **Lout);
+void proj_env_right(tnh_ctx *ctx, const tnh_buf *R, const tnh_buf *phi, const tnh_buf *psi,
+                    tnh_buf **Rout);
+void proj_project(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *phi1, const tnh_buf *phi2,
+                  const tnh_buf *R, tnh_buf **P);
+void proj_apply(tnh_ctx *ctx, const tnh_buf *P, const tnh_buf *x, tnh_buf *y, double coeff);
+void proj_calculate(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *phi, const tnh_buf *psi,
+                    const tnh_buf *R, double *re, double *im);
 void cscale(tnh_ctx *ctx, void *x, int t, int64_t n, double cr, double ci);
 void cdot_host(tnh_ctx *ctx, const tnh_buf *x, const tnh_buf *y, double *re, double *im);
 

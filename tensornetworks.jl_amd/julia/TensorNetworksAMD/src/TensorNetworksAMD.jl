@@ -551,7 +551,8 @@ end
          kwargs...) -> (psi, cost)
 
 dmrg(psi, H...) with the whole bond step on the GPU (tnh_dmrg_sweep): keyword arguments
-and convergence rules of src/algorithms/mps/dmrg.jl:1-99. `precision` (:f64, :f32,
+and convergence rules of src/algorithms/mps/dmrg.jl:1-99. `Hs` mixes MPOs and rank-1 GMPS
+(each a term coeff |phi><phi|, as in dmrg.jl:139-149), weighted by `coeffs`. `precision` (:f64, :f32,
 :bf16x3) sets the arithmetic of the matvecs and environment updates; with a lower
 precision the last `refine_sweeps` passes run in FP64. A non-empty `checkpoint` writes psi
 after every pass in the reference's HDF5 layout (group "psi", gmps.jl:359-370), from which
@@ -564,18 +565,34 @@ function dmrg(g::GPU, psi::GMPS, Hs::GMPS...; nsites::Int = 2, krylovdim::Int = 
               mindim::Int = 1, coeffs = ones(length(Hs)), precision::Symbol = :f64,
               refine_sweeps::Int = 2, checkpoint::AbstractString = "")
     nsites == 2 || error("the GPU path implements the two-site update")
+    length(Hs) == 0 && error("You must provide atleast one MPS/MPO for the Hamiltonian.")
+    length(coeffs) == length(Hs) || error("one coefficient per Hamiltonian term")
     ctx = g.ctx
     N = length(psi)
-    dpsi = [upload(ctx, psi[i]) for i = 1:N]
-    dW = [upload(ctx, H[i]) for H in Hs for i = 1:N]
+    # MPOs give ProjMPS(psi, H, psi), MPSs the rank-1 term ProjMPS(phi, psi; squared=true)
+    # (dmrg.jl:139-149); the rank-1 terms join the sum after the MPO terms
+    mpos = [j for j = 1:length(Hs) if TensorNetworks.rank(Hs[j]) == 2]
+    phis = [j for j = 1:length(Hs) if TensorNetworks.rank(Hs[j]) == 1]
+    length(mpos) + length(phis) == length(Hs) ||
+        error("Hamiltonian must be composed of MPOs (rank 2) or MPSs (rank 1).")
+    cplx = any(!all(iszero ∘ imag, Hs[j][i]) for j in phis for i = 1:N)  # a complex phi needs a complex psi
+    dpsi = [upload(ctx, psi[i]; real_if_possible = !cplx) for i = 1:N]
+    dW = [upload(ctx, Hs[j][i]) for j in mpos for i = 1:N]
     st = Ref{Ptr{Cvoid}}(C_NULL)
-    c = Cdouble[coeffs...]
+    c = Cdouble[coeffs[j] for j in mpos]
     hp = [t.h for t in dpsi]; hw = [t.h for t in dW]
     GC.@preserve hp hw c check(ccall((:tnh_dmrg_create, libtnhip), Cint,
             (Ptr{Cvoid}, Cint, Ptr{Ptr{Cvoid}}, Cint, Ptr{Ptr{Cvoid}}, Ptr{Cdouble}, Ptr{Ptr{Cvoid}}),
-            ctx.h, N, hp, length(Hs), hw, c, st), ctx.h, "tnh_dmrg_create")
+            ctx.h, N, hp, length(mpos), hw, c, st), ctx.h, "tnh_dmrg_create")
     cost = 0.0
     try
+        for j in phis
+            dphi = [upload(ctx, Hs[j][i]; real_if_possible = !cplx) for i = 1:N]
+            hf = [t.h for t in dphi]
+            GC.@preserve dphi hf check(ccall((:tnh_dmrg_add_projector, libtnhip), Cint,
+                    (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Cdouble), st[], hf, Cdouble(coeffs[j])),
+                    ctx.h, "tnh_dmrg_add_projector")
+        end
         e = Ref{Cdouble}(0); D = Ref{Cint}(0); disc = Ref{Cdouble}(0)
         check(ccall((:tnh_dmrg_energy, libtnhip), Cint, (Ptr{Cvoid}, Cint, Ptr{Cdouble}),
                     st[], 0, e), ctx.h, "tnh_dmrg_energy")

@@ -122,6 +122,7 @@ SIGNATURES = {
     "tnh_qjmc_run_batched": (C.c_int, [ctx_p, C.POINTER(QjmcProblem), C.c_int, C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
     "tnh_mpo_expect_grad":(C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.POINTER(buf_p), C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(buf_p)]),
     "tnh_dmrg_create":(C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.c_int, C.POINTER(buf_p), C.POINTER(C.c_double), C.POINTER(C.c_void_p)]),
+    "tnh_dmrg_add_projector":(C.c_int, [C.c_void_p, C.POINTER(buf_p), C.c_double]),
     "tnh_dmrg_destroy": (C.c_int, [C.c_void_p]),
     "tnh_dmrg_sweep": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(DmrgParams), C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "tnh_dmrg_energy": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),

@@ -10,6 +10,7 @@ import time
 
 import numpy as np
 
+from . import _lib
 from ._lib import DmrgParams
 from .core import Context, Tensor, precision_code
 from .models import GMPS
@@ -81,22 +82,42 @@ def replacesites(theta, direction: bool, normalize_: bool = True, cutoff=0.0, ma
 
 
 class DMRGState:
-    """Device-resident dmrg state (tnh_dmrg): psi, MPO terms and environment blocks."""
+    """Device-resident dmrg state (tnh_dmrg): psi, the Hamiltonian terms and their blocks.
+
+    Hs mixes MPOs (rank 2: ProjMPS(psi, H, psi)) and MPSs (rank 1: the term
+    coeff |phi><phi|, ProjMPS(phi, psi; squared=true)), as dmrg(psi, Hs...) does
+    (dmrg.jl:139-149); the rank-1 terms are summed after the MPO terms."""
 
     def __init__(self, psi: GMPS, Hs, coeffs=None, ctx: Context | None = None):
         self.ctx = ctx or Context.default()
         N = len(psi)
         Hs = list(Hs)
         self.N = N
-        dpsi = _upload_all(psi.tensors, self.ctx)
-        dW = [Tensor.from_numpy(w, ctx=self.ctx) for H in Hs for w in H.tensors]
         coeffs = list(coeffs) if coeffs is not None else [1.0] * len(Hs)
-        carr = (C.c_double * len(Hs))(*coeffs)
+        if len(coeffs) != len(Hs):
+            raise ValueError("coeffs must have one entry per Hamiltonian term")
+        mpos = [(H, c) for H, c in zip(Hs, coeffs) if H.rank == 2]
+        phis = [(H, c) for H, c in zip(Hs, coeffs) if H.rank == 1]
+        if len(mpos) + len(phis) != len(Hs):
+            raise ValueError("Hamiltonian must be composed of MPOs (rank 2) or MPSs (rank 1).")
+        # a complex rank-1 term makes the state complex (the library keeps one dtype)
+        cplx = any(np.iscomplexobj(t) for H, _ in phis for t in H.tensors)
+        pts = [np.asarray(t, dtype=np.complex128) for t in psi.tensors] if cplx else psi.tensors
+        dpsi = _upload_all(pts, self.ctx)
+        dW = [Tensor.from_numpy(w, ctx=self.ctx) for H, _ in mpos for w in H.tensors]
+        carr = (C.c_double * max(1, len(mpos)))(*[c for _, c in mpos])
         h = C.c_void_p()
-        self.ctx.call("tnh_dmrg_create", N, _handles(dpsi), len(Hs), _handles(dW), carr, C.byref(h))
+        self.ctx.call("tnh_dmrg_create", N, _handles(dpsi), len(mpos),
+                      _handles(dW) if dW else None, carr, C.byref(h))
         self.h = h
-        self._keep = (dpsi, dW)  # inputs are copied; drop them now
-        self._keep = None
+        try:
+            for H, c in phis:
+                dphi = _upload_all(H.tensors, self.ctx)
+                self.ctx.check(_lib.lib.tnh_dmrg_add_projector(h, _handles(dphi), float(c)),
+                               "tnh_dmrg_add_projector")
+        except Exception:
+            self.close()
+            raise
 
     def sweep(self, direction: bool, krylovdim=3, maxiter=2, lanczos_tol=1e-14, cutoff=1e-12,
               maxdim=1000, mindim=1, precision="f64"):
@@ -192,8 +213,8 @@ def dmrg(psi: GMPS, *Hs: GMPS, nsites=2, krylovdim=3, kryloviter=2, ishermitian=
     for H in Hs:
         if len(H) != len(psi) or H.dim != psi.dim:
             raise ValueError("GMPS must share the same properties.")
-        if H.rank != 2:
-            raise NotImplementedError("rank-1 (projector) Hamiltonian terms are not on the GPU path")
+        if H.rank not in (1, 2):
+            raise ValueError("Hamiltonian must be composed of MPOs (rank 2) or MPSs (rank 1).")
     st = DMRGState(psi, Hs, coeffs)
     if state_out is not None:
         state_out.append(st)
