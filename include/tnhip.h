@@ -244,6 +244,14 @@ int tnh_applygate2(tnh_ctx *ctx, tnh_buf **A1, tnh_buf **A2, const tnh_buf *G, i
  * (re, im). */
 int tnh_inner_oplist(tnh_ctx *ctx, int N, const tnh_buf *const *psi, int nterms,
                      const int *first_site, const int *range, const double *ops, double *out);
+/* The same between two states: <psi| O_t |phi> (unnormalised, coefficients applied by the
+ * caller) for the N FP64/C128 site tensors of psi and of phi (same N and physical dim;
+ * bonds may differ). Replaces inner(st, psi, oplist, phi), mps.jl:87-134, with the
+ * overlap blocks of ProjMPS(psi, phi; rank=1) (:90) from both ends and each term's operators
+ * applied to phi's sites; terms and `ops` as in tnh_inner_oplist. */
+int tnh_inner_oplist_pair(tnh_ctx *ctx, int N, const tnh_buf *const *psi,
+                          const tnh_buf *const *phi, int nterms, const int *first_site,
+                          const int *range, const double *ops, double *out);
 
 /* <psi|H|psi> for an MPO H = W[0..N-1] (inner(psi, H, psi), src/structures/mps/mpo.jl:181-217,
  * no normalisation) into (energy_re, energy_im), and, when dW is not NULL, its gradient

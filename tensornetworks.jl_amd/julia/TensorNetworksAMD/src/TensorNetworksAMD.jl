@@ -450,30 +450,50 @@ function applygate!(g::GPU, psi::GMPS, site::Int, gate, direction::Bool = false;
     return store!(psi, d)
 end
 
-"""
-    inner_oplist(d::DeviceMPS, st::Sitetypes, ops::OpList) -> Vector{ComplexF64}
-
-inner(st, psi, oplist, psi), src/structures/mps/mps.jl:87-134: coeff_t <psi|O_t|psi> for
-every term, from environments on the device (tnh_inner_oplist).
-"""
-function inner_oplist(d::DeviceMPS, st::Sitetypes, ol::OpList)
-    nt = length(ol.sites)
+# first sites, ranges and the operator matrices of every term (identity in a term's gaps)
+function _oplist_payload(st::Sitetypes, ol::OpList)
     first = Cint[s[1] for s in ol.sites]
     rng = Cint[s[end] - s[1] + 1 for s in ol.sites]
     mats = ComplexF64[]
-    dd = st.dim
-    for t = 1:nt
+    for t = 1:length(ol.sites)
         for s = ol.sites[t][1]:ol.sites[t][end]
             k = findfirst(==(s), ol.sites[t])
             m = k === nothing ? op(st, "id") : op(st, ol.ops[t][k])
             append!(mats, vec(Matrix{ComplexF64}(m)))
         end
     end
+    return first, rng, mats
+end
+
+"""
+    inner_oplist(d::DeviceMPS, st::Sitetypes, ops::OpList) -> Vector{ComplexF64}
+    inner_oplist(d::DeviceMPS, st::Sitetypes, ops::OpList, phi::DeviceMPS)
+
+inner(st, psi, oplist, phi), src/structures/mps/mps.jl:87-134: coeff_t <psi|O_t|phi> for
+every term (phi = psi in the first form), from environments on the device
+(tnh_inner_oplist, tnh_inner_oplist_pair).
+"""
+function inner_oplist(d::DeviceMPS, st::Sitetypes, ol::OpList)
+    nt = length(ol.sites)
+    first, rng, mats = _oplist_payload(st, ol)
     out = zeros(Float64, 2 * nt)
     GC.@preserve d first rng mats out check(ccall((:tnh_inner_oplist, libtnhip), Cint,
             (Ptr{Cvoid}, Cint, Ptr{Ptr{Cvoid}}, Cint, Ptr{Cint}, Ptr{Cint}, Ptr{Float64}, Ptr{Float64}),
             d.ctx.h, length(d), d.sites, nt, first, rng, Ptr{Float64}(pointer(mats)), out),
           d.ctx.h, "tnh_inner_oplist")
+    return [ol.coeffs[t] * complex(out[2t-1], out[2t]) for t = 1:nt]
+end
+
+function inner_oplist(d::DeviceMPS, st::Sitetypes, ol::OpList, phi::DeviceMPS)
+    length(phi) == length(d) || error("GMPS must share the same length.")
+    nt = length(ol.sites)
+    first, rng, mats = _oplist_payload(st, ol)
+    out = zeros(Float64, 2 * nt)
+    GC.@preserve d phi first rng mats out check(ccall((:tnh_inner_oplist_pair, libtnhip), Cint,
+            (Ptr{Cvoid}, Cint, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Cint, Ptr{Cint}, Ptr{Cint},
+             Ptr{Float64}, Ptr{Float64}),
+            d.ctx.h, length(d), d.sites, phi.sites, nt, first, rng, Ptr{Float64}(pointer(mats)), out),
+          d.ctx.h, "tnh_inner_oplist_pair")
     return [ol.coeffs[t] * complex(out[2t-1], out[2t]) for t = 1:nt]
 end
 

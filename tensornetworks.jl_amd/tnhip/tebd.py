@@ -241,17 +241,20 @@ def _oplist_payload(st: Sitetypes, oplist: OpList):
 
 
 def inner(st: Sitetypes, psi: DeviceMPS, oplist: OpList, phi: DeviceMPS | None = None) -> np.ndarray:
-    """inner(st, psi, oplist, phi), src/structures/mps/mps.jl:87-134 (phi = psi):
-    coeff * <psi|O|psi> for every term, computed on the GPU."""
-    if phi is not None and phi is not psi:
-        raise NotImplementedError("inner with phi != psi is not on the GPU path")
+    """inner(st, psi, oplist, phi), src/structures/mps/mps.jl:87-134: coeff * <psi|O|phi>
+    for every term (phi defaults to psi), computed on the GPU."""
     n = len(oplist.ops)
     firsts, ranges, flat = _oplist_payload(st, oplist)
     out = np.zeros(2 * max(n, 1))
     hs = psi._handles()
-    psi.ctx.call("tnh_inner_oplist", len(psi), hs, n, (C.c_int * max(n, 1))(*firsts),
-                 (C.c_int * max(n, 1))(*ranges), flat.ctypes.data_as(C.POINTER(C.c_double)),
-                 out.ctypes.data_as(C.POINTER(C.c_double)))
+    args = (n, (C.c_int * max(n, 1))(*firsts), (C.c_int * max(n, 1))(*ranges),
+            flat.ctypes.data_as(C.POINTER(C.c_double)), out.ctypes.data_as(C.POINTER(C.c_double)))
+    if phi is None or phi is psi:
+        psi.ctx.call("tnh_inner_oplist", len(psi), hs, *args)
+    else:
+        if len(phi) != len(psi):
+            raise ValueError("GMPS must share the same length.")
+        psi.ctx.call("tnh_inner_oplist_pair", len(psi), hs, phi._handles(), *args)
     vals = out[0::2][:n] + 1j * out[1::2][:n]
     return np.array([c * v for c, v in zip(oplist.coeffs, vals)])
 

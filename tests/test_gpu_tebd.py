@@ -50,6 +50,40 @@ def test_inner_matches_dense(ctx, rng):
     assert np.allclose(got, ref, rtol=1e-12, atol=1e-12)
 
 
+@pytest.mark.parametrize("cplx_phi", [False, True])
+def test_inner_two_states_matches_dense(ctx, rng, cplx_phi):
+    """inner(st, psi, oplist, phi) with phi != psi (mps.jl:87-134): <psi|O_t|phi> per term,
+    psi complex and phi real or complex with other bond dimensions, terms at both chain
+    ends and one with an identity gap."""
+    from tnhip import OpList, spinhalf
+    from tnhip.tebd import DeviceMPS, inner
+    from oracle.mps import MPS, to_dense
+    from oracle import exact
+
+    N = 7
+
+    def rand(chi, cplx):
+        out = []
+        for i in range(N):
+            shape = (1 if i == 0 else chi, 2, 1 if i == N - 1 else chi)
+            t = rng.standard_normal(shape)
+            out.append(t + 1j * rng.standard_normal(shape) if cplx else t)
+        return out
+
+    ps, ph = rand(3, True), rand(5, cplx_phi)
+    vpsi, vphi = to_dense(MPS(ps)), to_dense(MPS(ph))
+    ol = OpList(N)
+    ol.add("z", 1, 1.0)
+    ol.add(["x", "y"], [3, 5], 0.5)  # range-3 term with an identity gap
+    ol.add(["s+", "s-"], [6, 7], 2.0)
+    ol.add("x", 4, -1.5)
+    got = inner(spinhalf(), DeviceMPS(ps), ol, DeviceMPS(ph))
+    ref = [np.vdot(vpsi, exact.dense_hamiltonian(N, [(o, s, c)]) @ vphi)
+           for o, s, c in terms_of(ol)]
+    scale = np.linalg.norm(vpsi) * np.linalg.norm(vphi)
+    assert np.allclose(got, ref, rtol=0, atol=1e-12 * scale)
+
+
 def test_trotterize_layers():
     from tnhip import spinhalf
     from tnhip.tebd import trotterize

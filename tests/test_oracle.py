@@ -240,6 +240,24 @@ def test_inner_oplist_matches_dense(rng):
     assert np.allclose(got, [np.vdot(v, Z2 @ v), np.vdot(v, XX @ v)])
 
 
+def test_inner_oplist_two_states_matches_dense(rng):
+    """inner(st, psi, oplist, phi) with phi != psi (complex, other bond dims): both oracle
+    forms against <psi|O|phi> of the dense vectors."""
+    from oracle.mps import inner_oplist_env
+
+    psi = random_mps(6, 2, 3, rng)
+    phi = random_mps(6, 2, 4, rng)
+    for i in range(1, 7):
+        phi[i] = phi[i] + 1j * rng.standard_normal(phi[i].shape)
+    vpsi, vphi = to_dense(psi), to_dense(phi)
+    terms = [([exact.OPS["z"]], [1], 1.0), ([exact.OPS["x"], exact.OPS["y"]], [3, 5], 0.5),
+             ([exact.OPS["x"]], [6], -2.0)]
+    want = [c * np.vdot(vpsi, exact.dense_hamiltonian(6, [(ops, sites, 1.0)]) @ vphi)
+            for ops, sites, c in terms]
+    assert np.allclose(inner_oplist(psi, terms, phi), want, atol=1e-12)
+    assert np.allclose(inner_oplist_env(psi, terms, phi), want, atol=1e-12)
+
+
 # --------------------------------------------------------------- closed forms
 
 @pytest.mark.parametrize("N", [3, 8, 10])
