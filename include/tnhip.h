@@ -234,6 +234,16 @@ int tnh_two_site_theta(tnh_ctx *ctx, const tnh_buf *A1, const tnh_buf *A2, tnh_b
  * A1 and A2 are replaced in place by the split factors. */
 int tnh_applygate2(tnh_ctx *ctx, tnh_buf **A1, tnh_buf **A2, const tnh_buf *G, int direction,
                    double cutoff, int maxdim, int mindim, int *k, double *discarded);
+/* applygate for a gate on nsites >= 2 sites (gatelist.jl:137-171 with rng = nsites; the
+ * gates trotterize makes for Hamiltonians whose terms span more than two sites): A[0..nsites-1]
+ * are the window's FP64/C128 site tensors, replaced in place; G has 2 nsites indices
+ * G[o1, i1, ..., on, in] (nsites <= 4: tensors have at most 8 indices; d^nsites <= 64). theta = A[0] ... A[n-1], theta' = G theta, then the
+ * r-site branch of replacesites! without normalisation (gmps.jl:215-257): nsites - 1 truncated
+ * splits, left to right for direction 0 (the centre ends on the last site), right to left
+ * for direction 1 (the centre ends on the first). k (may be NULL) receives the nsites - 1
+ * kept bonds, *discarded the largest discarded weight. */
+int tnh_applygate_sites(tnh_ctx *ctx, int nsites, tnh_buf **A, const tnh_buf *G, int direction,
+                        double cutoff, int maxdim, int mindim, int *k, double *discarded);
 
 /* Local expectation values <psi| O_t |psi> (unnormalised) of nterms operator-list terms
  * over the N device site tensors of psi. Replaces inner(st, psi, oplist, psi),

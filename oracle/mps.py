@@ -87,6 +87,40 @@ def movecenter(psi, idx, **kw):
     psi.center = idx
 
 
+def replacesites_n(A, direction, cutoff=0.0, maxdim=0, mindim=1):
+    """replacesites!(psi, A, site, direction, false; kw) for nsites >= 2, rank 1,
+    src/structures/mps/gmps.jl:215-261: A[cl, d, ..., d, cr] is split one site at a time by
+    the truncated SVD. Sweeping right (direction false) every site but the last gets V of
+    svd(U, -1) over the combined (left bond, d) index and the rest carries U S; sweeping left
+    the mirror image. Returns (site tensors, largest discarded weight)."""
+    from .mpo import combineidxs
+
+    n = A.ndim - 2
+    d = A.shape[1]
+    out = [None] * n
+    U = A
+    dmax = 0.0
+    for i in range(1, n):
+        if direction:  # :220-238
+            U, _ = combineidxs(U, [U.ndim - 1, U.ndim])
+            site1 = n - i + 1
+            U, S, V, disc = svd(U, -1, cutoff=cutoff, maxdim=maxdim, mindim=mindim)
+            U = U * S                                             # :229
+            D = A.shape[-1] if site1 == n else out[site1].shape[0]  # :230
+            out[site1 - 1] = V.reshape((S.shape[0], d, D), order="F")  # :231-232
+        else:  # :239-256
+            U, _ = combineidxs(U, [1, 2])
+            site1 = i
+            U, S, V, disc = svd(U, -1, cutoff=cutoff, maxdim=maxdim, mindim=mindim)
+            U = moveidx(U * S, U.ndim, 1)                         # :247-248
+            D = A.shape[0] if site1 == 1 else out[site1 - 2].shape[2]  # :249
+            V = V.reshape((S.shape[0], D, d), order="F")          # :250-251
+            out[site1 - 1] = moveidx(V, 1, -1)                    # :252
+        dmax = max(dmax, disc)
+    out[0 if direction else n - 1] = U                            # :259-261
+    return out, dmax
+
+
 def replacesites2(theta, direction, normalize_=False, cutoff=0.0, maxdim=0, mindim=1):
     """replacesites!(psi, A, site, direction, normalize; kw) for nsites=2, rank 1,
     src/structures/mps/gmps.jl:199-267. Returns (A1, A2, S, discarded)."""

@@ -11,9 +11,9 @@ from .exact import OPS
 
 
 def local_tensor(terms, st_ops, site, N, d=2):
-    """sitetensor(oplist, st, site), src/structures/mps/oplist.jl:164-183, for terms of range
-    <= 2: G[o1,i1,o2,i2] (or G[o,i] for a one-site block at the last site). An operator is
-    a name in st_ops or a d x d matrix."""
+    """sitetensor(oplist, st, site), src/structures/mps/oplist.jl:164-183: the terms starting
+    at `site`, padded with identities to the list's site range r (fewer sites at the chain's
+    end), as G[o1, i1, ..., or, ir]. An operator is a name in st_ops or a d x d matrix."""
     rng = max([1] + [s[-1] - s[0] + 1 for _, s, _ in terms])
     r = min(rng, N - site + 1)
     G = None
@@ -36,17 +36,16 @@ def local_tensor(terms, st_ops, site, N, d=2):
 
 
 def gate_from_tensor(G, tau):
-    """exp(tau * G, [2, 4]) as a dense operator on (site, site+1), kron order (site slowest)."""
-    if G.ndim == 2:
-        return sla.expm(tau * G)
+    """exp(tau * G, [2, 4, ..]) as a dense operator on the r sites from `site`, kron order
+    (site slowest): G[o1, i1, ..., or, ir] -> matrix [(o1..or), (i1..ir)]."""
+    r = G.ndim // 2
     d = G.shape[0]
-    # G[o1,i1,o2,i2] -> matrix [(o1,o2), (i1,i2)] with o1 slowest (kron order)
-    M = np.transpose(G, (0, 2, 1, 3)).reshape(d * d, d * d)
+    M = np.transpose(G, list(range(0, 2 * r, 2)) + list(range(1, 2 * r, 2))).reshape(d ** r, d ** r)
     return sla.expm(tau * M)
 
 
 def trotter_layers(terms, N, dt, order=2, evol="imag", st_ops=OPS):
-    """Layers of (site, dense gate) following trotterize, gatelist.jl:75-121 (range <= 2)."""
+    """Layers of (site, dense gate) following trotterize, gatelist.jl:75-121."""
     rng = max([1] + [s[-1] - s[0] + 1 for _, s, _ in terms])
     order = 1 if rng == 1 else order
     dt = -1j * dt if evol == "real" else dt
@@ -93,16 +92,15 @@ def expect_dense(v, N, op, site):
 
 def rows_from_layers(layers, d=2):
     """Trotter layers of dense gates (kron order, site slowest) as the rows of a GateList:
-    (sites, tensors) with two-site gates G[o1, i1, o2, i2] and one-site gates O[o, i]."""
+    (sites, tensors) with r-site gates G[o1, i1, ..., or, ir] (O[o, i] for one site)."""
     rows = []
     for layer in layers:
         sites, tensors = [], []
         for site, U in layer:
             sites.append(site)
-            if U.shape[0] == d * d:
-                tensors.append(np.transpose(U.reshape(d, d, d, d), (0, 2, 1, 3)))
-            else:
-                tensors.append(np.asarray(U))
+            r = int(round(np.log(U.shape[0]) / np.log(d)))
+            perm = [p for k in range(r) for p in (k, r + k)]
+            tensors.append(np.transpose(np.asarray(U).reshape((d,) * (2 * r)), perm))
         rows.append((sites, tensors))
     return rows
 
@@ -111,12 +109,25 @@ def applygate_mps(psi, site, gate, direction, cutoff=0.0, maxdim=0, mindim=1):
     """applygate(psi, site, gate, direction; cutoff, maxdim, mindim, error=false),
     src/structures/mps/gatelist.jl:137-171, on an oracle MPS: theta = A1 A2 (:142-145),
     theta'[a,o1,o2,b] = sum G[o1,i1,o2,i2] theta[a,i1,i2,b] (:149-154), split without
-    normalisation (:157). A one-site gate O[o, i] acts on the site, which becomes the centre,
-    and the centre then moves one site on (replacesites!'s one-site branch, gmps.jl:204-213).
-    Returns the discarded weight."""
-    from .mps import movecenter, replacesites2
+    normalisation (:157). A gate on r > 2 sites contracts r sites and splits them with
+    replacesites!'s r-site branch (gmps.jl:215-261). A one-site gate O[o, i] acts on the
+    site, which becomes the centre, and the centre then moves one site on (replacesites!'s
+    one-site branch, gmps.jl:204-213). Returns the (largest) discarded weight."""
+    from .mps import movecenter, replacesites2, replacesites_n
 
     g = np.asarray(gate)
+    if g.ndim >= 6:
+        r = g.ndim // 2
+        theta = psi[site]
+        for j in range(1, r):                                  # :142-145
+            theta = np.tensordot(theta, psi[site + j], axes=([theta.ndim - 1], [0]))
+        theta = np.tensordot(theta, g, axes=(list(range(1, r + 1)), list(range(1, 2 * r, 2))))
+        theta = np.moveaxis(theta, 1, -1)                      # (a, o1..or, b), :149-154
+        sites, disc = replacesites_n(theta, direction, cutoff, maxdim, mindim)  # :157
+        for j in range(r):
+            psi[site + j] = sites[j]
+        psi.center = site if direction else site + r - 1
+        return disc
     if g.ndim == 4:
         theta = np.tensordot(psi[site], psi[site + 1], axes=([2], [0]))
         theta = np.einsum("xiyj,aijb->axyb", g, theta)

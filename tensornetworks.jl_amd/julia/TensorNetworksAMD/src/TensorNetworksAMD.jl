@@ -380,8 +380,9 @@ DeviceGates(ctx::Context, gl::GateList) =
     applygate!(d::DeviceMPS, site, gate::DeviceTensor, rng, direction; cutoff, maxdim, mindim)
 
 applygate!(psi, site, gate, direction; kwargs...), gatelist.jl:137-171 with error=false:
-a two-site gate and the truncated split without normalisation (tnh_applygate2), or a
-one-site gate followed by replacesites!'s untruncated centre move.
+a two-site gate and the truncated split without normalisation (tnh_applygate2), a wider
+gate and its rng - 1 truncated splits (tnh_applygate_sites), or a one-site gate followed by
+replacesites!'s untruncated centre move.
 """
 function applygate!(d::DeviceMPS, site::Int, gate::DeviceTensor, rng::Int, direction::Bool;
                     cutoff::Real = 0.0, maxdim::Int = 0, mindim::Int = 1)
@@ -404,7 +405,21 @@ function applygate!(d::DeviceMPS, site::Int, gate::DeviceTensor, rng::Int, direc
         nxt = site + 1 - 2 * direction
         0 < nxt <= length(d) && movecenter!(d, nxt)
     else
-        error("the GPU path applies one- and two-site gates")
+        # wider gates: the r-site branch of replacesites! (gmps.jl:215-257) on the device
+        hs = Ptr{Cvoid}[d.sites[site+j] for j = 0:rng-1]
+        ks = zeros(Cint, rng - 1); disc = Ref{Cdouble}(0)
+        try
+            GC.@preserve hs ks check(ccall((:tnh_applygate_sites, libtnhip), Cint,
+                        (Ptr{Cvoid}, Cint, Ptr{Ptr{Cvoid}}, Ptr{Cvoid}, Cint, Cdouble, Cint, Cint,
+                         Ptr{Cint}, Ptr{Cdouble}),
+                        d.ctx.h, rng, hs, gate.h, direction, cutoff, maxdim, mindim, ks, disc),
+                  d.ctx.h, "tnh_applygate_sites")
+        finally
+            for j = 0:rng-1  # the library replaces the handles in place, also before a failure
+                d.sites[site+j] = hs[j+1]
+            end
+        end
+        d.center = direction ? site : site + rng - 1
     end
     return d
 end

@@ -175,8 +175,8 @@ def trotterize(st: Sitetypes, ops: OpList, dt, order=2, evol="imag") -> GateList
 
 def applygate(psi: DeviceMPS, site, gate_dev: Tensor, gate_np, direction, cutoff=0.0, maxdim=0, mindim=1):
     """applygate(psi, site, gate, direction; cutoff, maxdim, mindim), gatelist.jl:137-171
-    (error=false). Two-site gates run tnh_applygate2; one-site gates tnh_apply_site_op
-    followed by replacesites!'s one-site branch (gmps.jl:204-213)."""
+    (error=false). Two-site gates run tnh_applygate2, wider ones tnh_applygate_sites; one-site
+    gates tnh_apply_site_op followed by replacesites!'s one-site branch (gmps.jl:204-213)."""
     rng = gatesize(gate_np)
     if rng == 2:
         h1 = C.c_void_p(psi[site].h.value)
@@ -203,7 +203,20 @@ def applygate(psi: DeviceMPS, site, gate_dev: Tensor, gate_np, direction, cutoff
         if 0 < nxt <= len(psi):
             psi.movecenter(nxt)
         return 0.0
-    raise NotImplementedError("gates on more than two sites are not on the GPU path")
+    # rng >= 3: the r-site branch of replacesites! (gmps.jl:215-257) on the device
+    arr = (C.c_void_p * rng)(*[psi[site + j].h.value for j in range(rng)])
+    ks = (C.c_int * (rng - 1))()
+    disc = C.c_double()
+    try:
+        psi.ctx.call("tnh_applygate_sites", rng, arr, gate_dev.h, int(bool(direction)), float(cutoff),
+                     int(maxdim), int(mindim), ks, C.byref(disc))
+    finally:
+        for j in range(rng):  # replaced buffers, also before a failure
+            if psi.sites[site + j - 1].h.value != arr[j]:
+                psi.sites[site + j - 1].h = None
+                psi.sites[site + j - 1] = Tensor(psi.ctx, C.c_void_p(arr[j]))
+    psi.center = site if direction else site + rng - 1
+    return disc.value
 
 
 def applygates(psi: DeviceMPS, gates: GateList, cutoff=0.0, maxdim=0, mindim=1):

@@ -84,6 +84,142 @@ def test_inner_two_states_matches_dense(ctx, rng, cplx_phi):
     assert np.allclose(got, ref, rtol=0, atol=1e-12 * scale)
 
 
+def _window_mps(rng, N, chi, site):
+    from oracle.mps import movecenter, random_mps
+
+    psi = random_mps(N, 2, chi, rng)
+    for i in range(1, N + 1):
+        psi[i] = psi[i] + 1j * rng.standard_normal(psi[i].shape)
+    psi.center = 0
+    movecenter(psi, site)
+    return psi
+
+
+@pytest.mark.parametrize("direction", [False, True])
+@pytest.mark.parametrize("r", [2, 3, 4])
+def test_applygate_sites_matches_oracle(ctx, rng, r, direction):
+    """tnh_applygate_sites (gatelist.jl:137-171 for an r-site gate, then replacesites!'s
+    r-site branch, gmps.jl:215-257) against the oracle's applygate_mps with truncation at
+    every split (maxdim 3): same bonds, same state (the kept subspaces do not depend on the
+    SVD gauge), and the canonical form the sweep direction leaves."""
+    import ctypes as C
+    from tnhip import Tensor
+    from tnhip.tebd import DeviceMPS
+    from oracle.mps import MPS, to_dense
+    from oracle.tebd import applygate_mps
+
+    N, site = 7, 2
+    ref = _window_mps(rng, N, 4, site if not direction else site + r - 1)
+    shape = (2,) * (2 * r)
+    G = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    dev = DeviceMPS([t.copy() for t in ref.tensors], ref.center)
+    arr = (C.c_void_p * r)(*[dev[site + j].h.value for j in range(r)])
+    ks = (C.c_int * (r - 1))()
+    disc = C.c_double()
+    Gd = Tensor.from_numpy(np.asfortranarray(G), ctx=dev.ctx)
+    try:
+        ctx.call("tnh_applygate_sites", r, arr, Gd.h, int(direction), 1e-14, 3, 1, ks, C.byref(disc))
+    finally:
+        for j in range(r):  # the window's handles are replaced in place
+            if dev.sites[site - 1 + j].h.value != arr[j]:
+                dev.sites[site - 1 + j].h = None
+                dev.sites[site - 1 + j] = Tensor(dev.ctx, C.c_void_p(arr[j]))
+    d_ref = applygate_mps(ref, site, G, direction, cutoff=1e-14, maxdim=3, mindim=1)
+    got = [t.numpy() for t in dev.sites]
+    assert [t.shape for t in got] == [t.shape for t in ref.tensors]
+    assert list(ks) == [ref.tensors[site - 1 + j].shape[2] for j in range(r - 1)]
+    v, w = to_dense(MPS(got)), to_dense(ref)
+    assert np.allclose(v, w, atol=1e-10 * np.linalg.norm(w))
+    assert abs(disc.value - d_ref) <= 1e-10 * max(1.0, d_ref)
+    for j in range(r):
+        t = got[site - 1 + j]
+        cl, d, cr = t.shape
+        if not direction and j < r - 1:
+            M = t.reshape(cl * d, cr, order="F")
+            assert np.allclose(M.conj().T @ M, np.eye(cr), atol=1e-12)
+        if direction and j > 0:
+            M = t.reshape(cl, d * cr, order="F")
+            assert np.allclose(M @ M.conj().T, np.eye(cl), atol=1e-12)
+
+
+def range3_oplist(N):
+    from tnhip import OpList
+
+    H = OpList(N)
+    for i in range(1, N):
+        H.add(["z", "z"], [i, i + 1], -1.0)
+    for i in range(1, N - 1):
+        H.add(["z", "z"], [i, i + 2], -0.45)
+        H.add(["z", "x", "z"], [i, i + 1, i + 2], 0.3)
+    for i in range(1, N + 1):
+        H.add("x", i, -0.8)
+    return H
+
+
+@pytest.mark.parametrize("evol", ["imag", "real"])
+def test_tebd_range3_exact_without_truncation(ctx, evol):
+    """TEBD with three-site gates (next-nearest zz and a z x z term: trotterize's range is
+    3) reproduces the dense Trotter evolution when nothing is truncated."""
+    from tnhip import productMPS, spinhalf, OpList
+    from tnhip.tebd import tebd, TEBDOperators
+    from oracle.tebd import trotter_layers, tebd_dense, expect_dense
+    from oracle.exact import OPS, dense_hamiltonian
+
+    N, dt, steps = 8, 0.05, 6
+    st = spinhalf()
+    H = range3_oplist(N)
+    names = ["up", "dn", "dn", "up", "dn", "up", "up", "dn"]
+    zs = OpList(N)
+    for i in range(1, N + 1):
+        zs.add("z", i)
+    obs = TEBDOperators(st, zs)
+    psi, energy = tebd(st, productMPS(st, names), H, dt, dt * steps, dt * steps, [obs], cutoff=0.0,
+                       maxdim=0, evol=evol, verbose=False)
+    layers = trotter_layers(terms_of(H), N, dt, order=2, evol=evol)
+    assert len(layers) == 5
+    v0 = np.zeros(2 ** N, complex)
+    v0[int("".join("0" if n == "up" else "1" for n in names), 2)] = 1.0
+    v = tebd_dense(v0, layers, N, steps)
+    ref = np.array([expect_dense(v, N, OPS["z"], i) for i in range(1, N + 1)])
+    assert np.allclose(obs.measurements[-1], ref, atol=1e-9)
+    e_ref = np.vdot(v, dense_hamiltonian(N, terms_of(H)) @ v).real
+    assert abs(energy - e_ref) <= 1e-8 * max(1, abs(e_ref))
+
+
+def test_tebd_range3_truncated_matches_oracle(ctx):
+    """Real-time TEBD with three-site gates and the bond dimension capped at 4: device
+    against the oracle's applygates_mps on the same rows, step by step."""
+    from tnhip import productMPS, spinhalf, OpList
+    from tnhip.tebd import TEBDOperators, tebd
+    from oracle.exact import OPS
+    from oracle.mps import normalize, product_mps
+    from oracle.tebd import applygates_mps, rows_from_layers, trotter_layers
+
+    N, dt, steps = 10, 0.05, 8
+    trunc = dict(cutoff=1e-10, maxdim=4, mindim=1)
+    st = spinhalf()
+    H = range3_oplist(N)
+    names = ["up" if i % 2 else "dn" for i in range(1, N + 1)]
+    zs = OpList(N)
+    for i in range(1, N + 1):
+        zs.add("z", i)
+    obs = TEBDOperators(st, zs)
+    psi, _ = tebd(st, productMPS(st, names), H, dt, dt * steps, dt, [obs], evol="real",
+                  verbose=False, **trunc)
+    rows = rows_from_layers(trotter_layers(terms_of(H), N, dt, order=2, evol="real"))
+    up, dn = np.array([1.0, 0.0]), np.array([0.0, 1.0])
+    ref = product_mps([up if n == "up" else dn for n in names])
+    zref = []
+    for _ in range(steps):
+        applygates_mps(ref, rows, **trunc)
+        normalize(ref)
+        zref.append([_expect_z(ref, i, OPS["z"]) for i in range(1, N + 1)])
+    assert [np.asarray(t).shape[2] for t in psi.tensors[:-1]] == ref.bonddims()
+    assert max(ref.bonddims()) == 4
+    got = np.real(np.array(obs.measurements[1:]))
+    assert np.allclose(got, np.array(zref), atol=1e-9)
+
+
 def test_trotterize_layers():
     from tnhip import spinhalf
     from tnhip.tebd import trotterize

@@ -143,6 +143,96 @@ def test_trotter_layer_structure():
     assert len(full) == 2
 
 
+def _range3_terms(N):
+    """Nearest and next-nearest zz, a three-site z x z term and a field: trotterize's site
+    range is 3, so its gates act on three sites (fewer at the chain's end)."""
+    return ([(["z", "z"], [i, i + 1], -1.0) for i in range(1, N)]
+            + [(["z", "z"], [i, i + 2], -0.45) for i in range(1, N - 1)]
+            + [(["z", "x", "z"], [i, i + 1, i + 2], 0.3) for i in range(1, N - 1)]
+            + [(["x"], [i], -0.8) for i in range(1, N + 1)])
+
+
+@pytest.mark.parametrize("evol", ["imag", "real"])
+def test_second_order_trotter_range3_error_scales_as_dt_cubed(evol):
+    """The same order check for a range-3 list: layers (1, 2 at dt/2; 3 at dt; 2, 1 at
+    dt/2) form a symmetric split, local error O(dt^3)."""
+    N = 6
+    terms = _range3_terms(N)
+    H = exact.dense_hamiltonian(N, terms)
+
+    def err(dt, order):
+        layers = trotter_layers(terms, N, dt, order=order, evol=evol)
+        tau = dt if evol == "imag" else -1j * dt
+        return np.linalg.norm(_layers_operator(layers, N) - sla.expm(tau * H), 2)
+
+    r2 = err(0.02, 2) / err(0.01, 2)
+    r1 = err(0.02, 1) / err(0.01, 1)
+    assert 7.6 < r2 < 8.4, r2
+    assert 3.6 < r1 < 4.4, r1
+
+
+def test_trotter_layer_structure_range3():
+    N = 7
+    layers = trotter_layers(_range3_terms(N), N, 0.1, order=2, evol="imag")
+    assert len(layers) == 5
+    assert [[s for s, _ in L] for L in layers[:3]] == [[1, 4, 7], [2, 5], [3, 6]]
+    assert [U.shape[0] for _, U in layers[0]] == [8, 8, 2]  # site 7: a one-site block
+    assert [U.shape[0] for _, U in layers[2]] == [8, 4]     # site 6: cut to two sites
+    assert layers[3] is layers[1] and layers[4] is layers[0]
+
+
+@pytest.mark.parametrize("direction", [False, True])
+def test_replacesites_n_untruncated(rng, direction):
+    """The r-site branch of replacesites! (gmps.jl:215-261): the sites reproduce theta,
+    sweeping right every site but the last is left-orthonormal, sweeping left every site but
+    the first right-orthonormal; bonds are the Schmidt ranks of the cuts."""
+    from oracle.mps import replacesites_n
+
+    shape = (3, 2, 2, 2, 5)
+    theta = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    sites, disc = replacesites_n(theta, direction)
+    assert disc < 1e-28
+    out = sites[0]
+    for t in sites[1:]:
+        out = np.tensordot(out, t, axes=([out.ndim - 1], [0]))
+    assert np.allclose(out, theta, atol=1e-12)
+    for i, t in enumerate(sites):
+        cl, d, cr = t.shape
+        if not direction and i < len(sites) - 1:
+            M = t.reshape(cl * d, cr, order="F")
+            assert np.allclose(M.conj().T @ M, np.eye(cr), atol=1e-12)
+        if direction and i > 0:
+            M = t.reshape(cl, d * cr, order="F")
+            assert np.allclose(M @ M.conj().T, np.eye(cl), atol=1e-12)
+    for j in range(1, 4):  # Schmidt rank of each cut
+        M = theta.reshape(int(np.prod(shape[: j + 1])), -1, order="F")
+        assert sites[j - 1].shape[2] == np.linalg.matrix_rank(M)
+
+
+def test_mps_range3_gates_match_dense():
+    """applygates! of range-3 Trotter rows on an MPS (gatelist.jl:137-223 with r-site gates)
+    reproduces the dense Trotter evolution when nothing is truncated."""
+    from oracle.mps import normalize, product_mps, to_dense
+    from oracle.tebd import applygates_mps, rows_from_layers, tebd_dense
+
+    N, dt, steps = 7, 0.05, 5
+    terms = _range3_terms(N)
+    layers = trotter_layers(terms, N, dt, order=2, evol="real")
+    up, dn = np.array([1.0, 0.0]), np.array([0.0, 1.0])
+    states = [up, dn, dn, up, dn, up, up]
+    psi = product_mps(states)
+    rows = rows_from_layers(layers)
+    assert any(np.ndim(g) == 6 for _, gs in rows for g in gs)
+    for _ in range(steps):
+        applygates_mps(psi, rows)
+        normalize(psi)
+    v0 = states[0]
+    for s in states[1:]:
+        v0 = np.kron(v0, s)
+    v = tebd_dense(v0.astype(complex), layers, N, steps)
+    assert np.allclose(to_dense(psi), v, atol=1e-11)
+
+
 # ------------------------------------------------------------------- Krylov
 
 
