@@ -378,6 +378,7 @@ typedef struct tnh_dmrg_params {
   int maxdim;     /* dmrg.jl:18 default 1000 */
   int mindim;     /* dmrg.jl:19 default 1 */
   int precision;  /* TNH_PREC_* for the matvecs and environment updates */
+  int nsites;     /* dmrg.jl:3: 2 (or 0) for the two-site update, 1 for the one-site update */
 } tnh_dmrg_params;
 
 /* Device-resident state of dmrg(psi, H::GMPS...), src/algorithms/mps/dmrg.jl:128-154:
@@ -399,7 +400,9 @@ int tnh_dmrg_add_projector(tnh_dmrg *st, const tnh_buf *const *phi, double coeff
 int tnh_dmrg_destroy(tnh_dmrg *st);
 /* One directional pass (the reference's "sweep", dmrg.jl:35-63): for each bond,
  * move the environment centre, form theta, eigsolve, replacesites!. direction 0 =
- * left-to-right. Returns the last Ritz value, the max bond dim and the max
+ * left-to-right. With p->nsites = 1 each site in turn is updated alone and the centre
+ * then moves one site on by an untruncated split, normalised (replacesites!'s one-site
+ * branch, gmps.jl:204-213): bond dimensions do not change. Returns the last Ritz value, the max bond dim and the max
  * discarded weight of the pass. In a TNH_PREC_F32 pass the block-Jacobi Gram
  * eigensolves of the splits (Gram sides above 512) stop at 1e-9 max_i G_ii, the scale
  * below which the pass's own FP32 rounding already blurs theta; FP64 passes use the FP64

@@ -4,7 +4,7 @@ from __future__ import annotations
 import numpy as np
 
 from .krylov import eigsolve_ground
-from .mps import movecenter, replacesites2
+from .mps import movecenter, normalize, replacesites2
 from .projmps import ProjMPS, ProjMPSSquared, ProjMPSSum
 
 
@@ -20,7 +20,7 @@ def dmrg(psi, Ws, nsites=2, krylovdim=3, kryloviter=2, minsweeps=1, maxsweeps=10
     Returns (psi, cost). `history` (a list) receives (sweep, cost, maxbonddim, max
     discarded). A "sweep" is one directional pass, as in the reference (:66).
     """
-    assert nsites == 2, "only the two-site update is on the hot path"
+    assert nsites in (1, 2), "the one- and two-site updates"
     N = len(psi)
     movecenter(psi, 1)                                       # :133
     projs = [ProjMPS(psi.tensors, Ws, center=1, coeff=coeff)]  # :143, projmps.jl:16-42
@@ -49,10 +49,19 @@ def dmrg(psi, Ws, nsites=2, krylovdim=3, kryloviter=2, minsweeps=1, maxsweeps=10
             site = N + 1 - j if direction else j
             site1 = site + 1 - nsites if direction else site
             Hs.movecenter(site)                              # :41
-            A0 = np.tensordot(psi[site1], psi[site1 + 1], axes=([2], [0]))  # :44-47
+            A0 = psi[site1]                                  # :44-47
+            if nsites == 2:
+                A0 = np.tensordot(A0, psi[site1 + 1], axes=([2], [0]))
             e, vec, _ = eigsolve_ground(lambda x: Hs.product(x, direction, nsites), A0,
                                         krylovdim, kryloviter, 1e-14)       # :50-53
             cost = e                                         # :55
+            if nsites == 1:  # replacesites!'s one-site branch, gmps.jl:204-213
+                psi[site1] = vec
+                nxt = site1 + 1 - 2 * int(direction)
+                if 0 < nxt <= N:
+                    movecenter(psi, nxt)
+                normalize(psi)
+                continue
             A1, A2, _, disc = replacesites2(vec, direction, True, cutoff, maxdim, mindim)  # :57
             psi[site1] = A1
             psi[site1 + 1] = A2

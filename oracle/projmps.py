@@ -35,8 +35,12 @@ def buildright(R, A, W):
 def product_reference(L, W1, W2, R, A, coeff=1.0):
     """product(projV, A, direction, nsites=2), rank-2 branch, in the reference's own
     contraction order, src/structures/mps/projmps.jl:107-134 and :144."""
-    Ws = [W1, W2]
-    nsites = 2
+    return product_reference_n(L, [W1, W2], R, A, coeff)
+
+
+def product_reference_n(L, Ws, R, A, coeff=1.0):
+    """The same for nsites = len(Ws) sites (the one-site update: one MPO tensor)."""
+    nsites = len(Ws)
     prod = moveidx(L, L.ndim, 2)                       # :116
     for i in range(1, nsites + 1):                     # :119-129
         prod = contract(prod, Ws[i - 1], 1 + 2 * i, 1)  # :122 (one MPO object)
@@ -123,10 +127,13 @@ class ProjMPS:
         self.center = idx
 
     def product(self, A, direction=False, nsites=2):
-        """product(projV, A, direction, nsites), projmps.jl:103-144 (optimal order)."""
+        """product(projV, A, direction, nsites), projmps.jl:103-144 (optimal order for two
+        sites, the reference's order for one)."""
         site = self.center - nsites + 1 if direction else self.center
         L = self.block(site - 1)
         R = self.block(site + nsites)
+        if nsites == 1:
+            return product_reference_n(L, [self.Ws[site - 1]], R, A, self.coeff)
         return product_optimal(L, self.Ws[site - 1], self.Ws[site], R, A, self.coeff)
 
     def calculate(self):
@@ -163,9 +170,14 @@ def project(L, phi1, phi2, R):
     """project(projV, A, direction, nsites=2) for objects [phi, psi],
     src/structures/mps/projmps.jl:152-179: phi's two sites, conjugated, between the
     overlap blocks; the result has theta's shape (chi_l, d, d, chi_r) of psi."""
-    nsites = 2
+    return project_n(L, [phi1, phi2], R)
+
+
+def project_n(L, phis, R):
+    """The same for nsites = len(phis) sites."""
+    nsites = len(phis)
     prod = moveidx(L, L.ndim, 1)                            # :160
-    for i, ph in ((1, phi1), (2, phi2)):                    # :163-173
+    for i, ph in enumerate(phis, start=1):                  # :163-173
         prod = contract(prod, np.conj(ph), 1 + i, 1)        # :165
         prod = moveidx(prod, prod.ndim - 1, 1 + i)          # :172
     return contract(prod, R, list(range(2 + nsites, prod.ndim + 1)),
@@ -175,7 +187,12 @@ def project(L, phi1, phi2, R):
 def product_squared(L, phi1, phi2, R, A, coeff=1.0):
     """product(projV, A, direction, nsites=2), squared branch, projmps.jl:135-144:
     coeff * conj(P) * sum(P .* A) with P = project(...)."""
-    P = project(L, phi1, phi2, R)                           # :137
+    return product_squared_n(L, [phi1, phi2], R, A, coeff)
+
+
+def product_squared_n(L, phis, R, A, coeff=1.0):
+    """The same for nsites = len(phis) sites."""
+    P = project_n(L, phis, R)                               # :137
     s = contract(P, A, list(range(1, P.ndim + 1)), list(range(1, A.ndim + 1)))[()]  # :140
     return coeff * (s * np.conj(P))                         # :139, :141, :144
 
@@ -222,8 +239,8 @@ class ProjMPSSquared:
 
     def product(self, A, direction=False, nsites=2):
         site = self.center - nsites + 1 if direction else self.center
-        return product_squared(self.block(site - 1), self.phi[site - 1], self.phi[site],
-                               self.block(site + nsites), A, self.coeff)
+        return product_squared_n(self.block(site - 1), self.phi[site - 1:site - 1 + nsites],
+                                 self.block(site + nsites), A, self.coeff)
 
     def calculate(self):
         site = self.center

@@ -813,6 +813,89 @@ void two_site_theta(tnh_ctx *ctx, const tnh_buf *A1, const tnh_buf *A2, tnh_buf 
   cgemm(ctx, g, cop(A1), cop(A2), (*theta)->re(), c ? (*theta)->im() : nullptr, real_of(dt), prec);
 }
 
+// One-site H_eff matvec, product(projV, A, direction, nsites=1), projmps.jl:103-144:
+// out[a', s, b'] = coeff sum L[a', w, a] W[w, s, t, w'] x[a, t, b] R[b', w', b], as the first
+// two stages of the environment update (X = L x, Y = X Wl) followed by out = Y R^T.
+// x and out hold chi_l d chi_r values (any shape: the DMRG passes a (chi_l, d, 1, chi_r) view).
+void heff1_apply(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *W, const tnh_buf *R,
+                 const tnh_buf *x, tnh_buf *out, double coeff, int prec) {
+  if (!L) L = edge_block(ctx);
+  if (!R) R = edge_block(ctx);
+  require_dims(L, 3, "L");
+  require_dims(W, 4, "W");
+  require_dims(R, 3, "R");
+  TNH_REQUIRE(x && out, TNH_ERR_ARG, "heff1: NULL vector");
+  const int64_t ca = L->dims[0], cb = R->dims[0], d = W->dims[1];
+  const int64_t Dl = W->dims[0], Dr = W->dims[3];
+  TNH_REQUIRE(W->dims[2] == d && L->dims[2] == ca && L->dims[1] == Dl && R->dims[2] == cb &&
+                  R->dims[1] == Dr,
+              TNH_ERR_SHAPE, "heff1: L " + shape_str(L) + ", W " + shape_str(W) + ", R " +
+                                 shape_str(R) + " do not chain");
+  TNH_REQUIRE(x->numel == ca * d * cb && out->numel == x->numel, TNH_ERR_SHAPE,
+              "heff1: site tensor " + shape_str(x) + " does not match the blocks");
+  TNH_REQUIRE(is_double(out->dtype), TNH_ERR_ARG, "heff1: FP64 output");
+  check_prec_operand(prec, L, "L");
+  check_prec_operand(prec, R, "R");
+  TNH_REQUIRE(Dl * d <= kMaxIJ && d * Dr <= kMaxIJ, TNH_ERR_UNSUPPORTED, "D*d > 64");
+  bool wcc = false;
+  double *Wc = make_wc_env(ctx, W, true, 853, &wcc);
+  const bool xc = is_complex(L->dtype) || is_complex(x->dtype);
+  const bool yc = xc || wcc;
+  TNH_REQUIRE(is_complex(out->dtype) || !(yc || is_complex(R->dtype)), TNH_ERR_ARG,
+              "heff1: complex operands need a complex output");
+  const int tX = inter_dtype(prec, xc), tY = inter_dtype(prec, yc);
+  const int64_t nx = ca * Dl * d * cb, ny = ca * d * Dr * cb;
+  const size_t rs = real_size(tX);
+  char *X = static_cast<char *>(ctx->workspace(854, nx * rs * (xc ? 2 : 1)));
+  char *Y = static_cast<char *>(ctx->workspace(855, ny * rs * (yc ? 2 : 1)));
+  // (1) X[(a',w),(t,b)] = L[(a',w),a] x[a,(t,b)]
+  Gemm g1;
+  g1.M = int(ca * Dl);
+  g1.N = int(d * cb);
+  g1.K = int(ca);
+  g1.lda = ca * Dl;
+  g1.ldb = ca;
+  g1.ldc = ca * Dl;
+  cgemm(ctx, g1, cop(L), cop(x), X, xc ? X + nx * rs : nullptr, real_of(tX), prec);
+  // (2) Y[a',(s,w'),b] = X[a',(w,t),b] Wl[(w,t),(s,w')]
+  Small sm;
+  sm.na = int(ca);
+  sm.nb = int(cb);
+  sm.I = int(Dl * d);
+  sm.J = int(d * Dr);
+  sm.in = X;
+  sm.in_im = xc ? nx : 0;
+  sm.tin = real_of(tX);
+  sm.out = Y;
+  sm.out_im = yc ? ny : 0;
+  sm.tout = real_of(tY);
+  sm.Wc = Wc;
+  sm.Wc_cplx = wcc;
+  sm.sbI = ca * Dl * d;
+  sm.sbJ = ca * d * Dr;
+  sm.offI.resize(sm.I);
+  sm.offJ.resize(sm.J);
+  for (int i = 0; i < sm.I; ++i) sm.offI[i] = ca * i;
+  for (int j = 0; j < sm.J; ++j) sm.offJ[j] = ca * j;
+  small_contract(ctx, sm);
+  // (3) out[(a',s),b'] = coeff Y[(a',s),(w',b)] R[b',(w',b)]^T
+  Gemm g3;
+  g3.M = int(ca * d);
+  g3.N = int(cb);
+  g3.K = int(Dr * cb);
+  g3.transB = true;
+  g3.lda = ca * d;
+  g3.ldb = cb;
+  g3.ldc = ca * d;
+  g3.alpha = coeff;
+  COperand y;
+  y.re = Y;
+  y.im = yc ? Y + ny * rs : nullptr;
+  y.dtype = real_of(tY);
+  cgemm(ctx, g3, y, cop(R), out->re(), is_complex(out->dtype) ? out->im() : nullptr,
+        real_of(out->dtype), prec);
+}
+
 double env_calculate(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *A, const tnh_buf *W,
                      const tnh_buf *R, double *im_out) {
   tnh_buf *Lp = nullptr;

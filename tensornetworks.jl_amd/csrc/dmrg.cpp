@@ -146,11 +146,35 @@ void bond_matvec(tnh_dmrg *st, int site1, int prec, const tnh_buf *x, tnh_buf *y
   for (const auto &q : st->proj) proj_apply(ctx, q.P, x, y, q.coeff);
 }
 
-void make_shadows(tnh_dmrg *st, int site1) {
+// y = sum_t coeff_t H_t x + the rank-1 terms for the single site `site` (nsites = 1)
+void site_matvec(tnh_dmrg *st, int site, int prec, const tnh_buf *x, tnh_buf *y) {
+  tnh_ctx *ctx = st->ctx;
+  if (st->nterms == 0) fill(ctx, y->ptr, y->dtype, y->numel, 0.0);
+  for (int t = 0; t < st->nterms; ++t) {
+    const tnh_buf *L = st->block(t, site - 1);
+    const tnh_buf *R = st->block(t, site + 1);
+    if (prec != TNH_PREC_F64) {
+      if (L) L = st->shadowL[t];
+      if (R) R = st->shadowR[t];
+    }
+    if (t == 0) {
+      heff1_apply(ctx, L, st->W[t][site - 1], R, x, y, st->coeffs[t], prec);
+    } else {
+      buf_ensure(ctx, &st->tmp, y->dtype, y->ndim, y->dims);
+      heff1_apply(ctx, L, st->W[t][site - 1], R, x, st->tmp, st->coeffs[t], prec);
+      axpby(ctx, y->numel * (is_complex(y->dtype) ? 2 : 1), 1.0,
+            static_cast<const double *>(st->tmp->ptr), 1.0, static_cast<double *>(y->ptr));
+    }
+  }
+  for (const auto &q : st->proj) proj_apply(ctx, q.P, x, y, q.coeff);
+}
+
+// FP32 copies of the blocks left of site `lo` and right of site `hi` (the sites updated)
+void make_shadows(tnh_dmrg *st, int lo, int hi) {
   tnh_ctx *ctx = st->ctx;
   for (int t = 0; t < st->nterms; ++t) {
-    const tnh_buf *L = st->block(t, site1 - 1);
-    const tnh_buf *R = st->block(t, site1 + 2);
+    const tnh_buf *L = st->block(t, lo - 1);
+    const tnh_buf *R = st->block(t, hi + 1);
     if (L) {
       const int dt = is_complex(L->dtype) ? TNH_C64 : TNH_F32;
       buf_ensure(ctx, &st->shadowL[t], dt, L->ndim, L->dims);
@@ -162,6 +186,55 @@ void make_shadows(tnh_dmrg *st, int site1) {
       convert(ctx, R->ptr, R->dtype, st->shadowR[t]->ptr, dt, R->numel);
     }
   }
+}
+
+// One pass of the one-site update (dmrg.jl:35-63 with nsites = 1): at each site the
+// environments move there, the site tensor is the Krylov start, the Ritz vector replaces
+// it, and replacesites!'s one-site branch (gmps.jl:204-213) moves the centre one site on
+// with an untruncated split and normalises the new centre (at the last site of the pass
+// the centre stays and that tensor is normalised). Returns the last Ritz value.
+double one_site_pass(tnh_dmrg *st, int direction, const tnh_dmrg_params *p) {
+  tnh_ctx *ctx = st->ctx;
+  const int N = st->N, prec = p->precision;
+  double cost = 0;
+  for (int j = 1; j <= N; ++j) {
+    const int site = direction ? N + 1 - j : j;
+    st->env_movecenter(site, prec);  // :41
+    tnh_buf *A = st->psi[site - 1];
+    const int64_t td[4] = {A->dims[0], A->dims[1], 1, A->dims[2]};  // the Krylov solver's view
+    buf_ensure(ctx, &st->theta, A->dtype, 4, td);
+    copy_device(ctx, st->theta->ptr, A->ptr, A->bytes);
+    if (prec != TNH_PREC_F64) make_shadows(st, site, site);
+    for (auto &q : st->proj)
+      proj_project1(ctx, st->pblock(q, site - 1), q.phi[site - 1], st->pblock(q, site + 1), &q.P);
+    int nmv = 0;
+    cost = eigsolve_ground_op(
+        ctx, [&](const tnh_buf *x, tnh_buf *y) { site_matvec(st, site, prec, x, y); }, st->theta,
+        p->krylovdim, p->maxiter, p->lanczos_tol, &nmv);
+    copy_device(ctx, A->ptr, st->theta->ptr, A->bytes);  // psi[site] = A, gmps.jl:205
+    const int nxt = site + 1 - 2 * direction;
+    int c = site;
+    if (nxt >= 1 && nxt <= N) {  // movecenter!(psi, site + 1 - 2 direction), :206-208
+      if (direction)
+        mps_moveleft(ctx, &st->psi[nxt - 1], &st->psi[site - 1], 0.0, 0, 1);
+      else
+        mps_moveright(ctx, &st->psi[site - 1], &st->psi[nxt - 1], 0.0, 0, 1);
+      c = nxt;
+    }
+    st->psi_center = c;
+    // normalize!(psi), gmps.jl:46-51: the centre tensor divided by its norm
+    tnh_buf *C = st->psi[c - 1];
+    tnh_buf v = *C;
+    v.dtype = TNH_F64;
+    v.numel = C->numel * (is_complex(C->dtype) ? 2 : 1);
+    double nr = 0, ni = 0;
+    cdot_host(ctx, &v, &v, &nr, &ni);
+    nr = std::sqrt(std::max(nr, 0.0));
+    TNH_REQUIRE(std::isfinite(nr) && nr > 0, TNH_ERR_NUMERIC, "dmrg: zero centre tensor");
+    scale(ctx, C->ptr, C->dtype, C->numel, 1.0 / nr);
+  }
+  st->env_movecenter(direction ? 1 : N, prec);  // :62
+  return cost;
 }
 
 }  // namespace
@@ -309,7 +382,20 @@ int tnh_dmrg_sweep(tnh_dmrg *st, int direction, const tnh_dmrg_params *p, double
     TNH_REQUIRE(p->maxdim >= 0 && p->mindim >= 0, TNH_ERR_ARG, "maxdim, mindim must be >= 0");
     TNH_REQUIRE(st->nterms + st->proj.size() >= 1, TNH_ERR_ARG,
                 "You must provide atleast one MPS/MPO for the Hamiltonian.");
+    TNH_REQUIRE(p->nsites >= 0 && p->nsites <= 2, TNH_ERR_ARG, "nsites must be 1 or 2");
     double cost = 0, maxdisc = 0;
+    if (p->nsites == 1) {
+      cost = one_site_pass(st, direction, p);
+      TNH_HIP(hipStreamSynchronize(ctx->stream));
+      if (energy) *energy = cost;
+      if (maxbonddim) {
+        int D = 0;
+        for (int i = 0; i + 1 < N; ++i) D = std::max<int>(D, int(st->psi[i]->dims[2]));
+        *maxbonddim = D;
+      }
+      if (max_discarded) *max_discarded = 0.0;
+      return;
+    }
     for (int j = 1; j <= N - 1; ++j) {  // dmrg.jl:35
       const int site = direction ? N + 1 - j : j;
       const int site1 = direction ? site - 1 : site;
@@ -321,7 +407,7 @@ int tnh_dmrg_sweep(tnh_dmrg *st, int direction, const tnh_dmrg_params *p, double
       ph = ctx->prof_begin();
       // the psi centre must sit on the bond: it does after the previous split
       two_site_theta(ctx, st->psi[site1 - 1], st->psi[site1], &st->theta);  // :44-47
-      if (prec != TNH_PREC_F64) make_shadows(st, site1);
+      if (prec != TNH_PREC_F64) make_shadows(st, site1, site1 + 1);
       for (auto &q : st->proj)  // project(projV, A, direction, 2), projmps.jl:152-179
         proj_project(ctx, st->pblock(q, site1 - 1), q.phi[site1 - 1], q.phi[site1],
                      st->pblock(q, site1 + 2), &q.P);

@@ -240,6 +240,47 @@ void proj_project(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *phi1, const tnh
         TNH_PREC_F64);
 }
 
+// project(projV, A, direction, 1), projmps.jl:152-179 with nsites = 1:
+// P[a, s, b] = sum L[f, a] conj(phi[f, s, h]) R[h, b]
+void proj_project1(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *phi, const tnh_buf *R,
+                   tnh_buf **P) {
+  require_f64(L, 2, "left overlap block");
+  require_f64(R, 2, "right overlap block");
+  require_f64(phi, 3, "phi");
+  const int64_t cf = phi->dims[0], d = phi->dims[1], ch = phi->dims[2];
+  const int64_t ca = L->dims[1], cb = R->dims[1];
+  TNH_REQUIRE(L->dims[0] == cf && R->dims[0] == ch, TNH_ERR_SHAPE,
+              "overlap blocks " + shape_str(L) + " / " + shape_str(R) + " do not match phi");
+  const bool c1 = any_complex({L, phi}), c2 = c1 || is_complex(R->dtype);
+  const int64_t n1 = ca * d * ch;
+  void *Q1 = ctx->workspace(960, n1 * sizeof(double) * (c1 ? 2 : 1));
+  // (1) Q1[a, (s, h)] = L[f, a]^T conj(phi)[f, (s, h)]
+  Gemm g1;
+  g1.M = int(ca);
+  g1.N = int(d * ch);
+  g1.K = int(cf);
+  g1.transA = true;
+  g1.lda = cf;
+  g1.ldb = cf;
+  g1.ldc = ca;
+  cgemm(ctx, g1, cop(L, false), cop(phi, true), Q1, plane_im(Q1, c1, n1), TNH_F64, TNH_PREC_F64);
+  // (2) P[(a, s), b] = Q1[(a, s), h] R[h, b]
+  int64_t pd[3] = {ca, d, cb};
+  buf_ensure(ctx, P, c2 ? TNH_C128 : TNH_F64, 3, pd);
+  Gemm g2;
+  g2.M = int(ca * d);
+  g2.N = int(cb);
+  g2.K = int(ch);
+  g2.lda = ca * d;
+  g2.ldb = ch;
+  g2.ldc = ca * d;
+  COperand q;
+  q.re = Q1;
+  q.im = plane_im(Q1, c1, n1);
+  q.dtype = TNH_F64;
+  cgemm(ctx, g2, q, cop(R, false), (*P)->re(), c2 ? (*P)->im() : nullptr, TNH_F64, TNH_PREC_F64);
+}
+
 // y += coeff * conj(P) * sum(P .* x): the squared branch of product, projmps.jl:135-144
 void proj_apply(tnh_ctx *ctx, const tnh_buf *P, const tnh_buf *x, tnh_buf *y, double coeff) {
   TNH_REQUIRE(P->numel == x->numel && y->numel == x->numel, TNH_ERR_SHAPE,

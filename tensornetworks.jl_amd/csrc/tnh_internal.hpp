@@ -242,6 +242,9 @@ void env_left(tnh_ctx *ctx, const tnh_buf *Lp, const tnh_buf *A, const tnh_buf *
               tnh_buf **Lout, int prec);
 void env_right(tnh_ctx *ctx, const tnh_buf *Rp, const tnh_buf *A, const tnh_buf *W,
                tnh_buf **Rout, int prec);
+// one-site H_eff matvec (nsites = 1 DMRG): out = coeff H_eff x
+void heff1_apply(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *W, const tnh_buf *R,
+                 const tnh_buf *x, tnh_buf *out, double coeff, int prec);
 double env_calculate(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *A, const tnh_buf *W,
                      const tnh_buf *R, double *im_out);
 void two_site_theta(tnh_ctx *ctx, const tnh_buf *A1, const tnh_buf *A2, tnh_buf **theta);
@@ -252,6 +255,8 @@ void proj_env_right(tnh_ctx *ctx, const tnh_buf *R, const tnh_buf *phi, const tn
                     tnh_buf **Rout);
 void proj_project(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *phi1, const tnh_buf *phi2,
                   const tnh_buf *R, tnh_buf **P);
+void proj_project1(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *phi, const tnh_buf *R,
+                   tnh_buf **P);
 void proj_apply(tnh_ctx *ctx, const tnh_buf *P, const tnh_buf *x, tnh_buf *y, double coeff);
 void proj_calculate(tnh_ctx *ctx, const tnh_buf *L, const tnh_buf *phi, const tnh_buf *psi,
                     const tnh_buf *R, double *re, double *im);

@@ -579,6 +579,7 @@ struct DmrgParams
     maxdim::Cint
     mindim::Cint
     precision::Cint
+    nsites::Cint
 end
 
 """
@@ -599,7 +600,7 @@ function dmrg(g::GPU, psi::GMPS, Hs::GMPS...; nsites::Int = 2, krylovdim::Int = 
               verbose::Bool = true, cutoff::Float64 = 1e-12, maxdim::Int = 1000,
               mindim::Int = 1, coeffs = ones(length(Hs)), precision::Symbol = :f64,
               refine_sweeps::Int = 2, checkpoint::AbstractString = "")
-    nsites == 2 || error("the GPU path implements the two-site update")
+    (nsites == 1 || nsites == 2) || error("the GPU path implements the one- and two-site updates")
     length(Hs) == 0 && error("You must provide atleast one MPS/MPO for the Hamiltonian.")
     length(coeffs) == length(Hs) || error("one coefficient per Hamiltonian term")
     ctx = g.ctx
@@ -640,7 +641,8 @@ function dmrg(g::GPU, psi::GMPS, Hs::GMPS...; nsites::Int = 2, krylovdim::Int = 
             if prec != :f64 && maxsweeps != 0 && sweeps >= maxsweeps - refine_sweeps
                 prec = :f64
             end
-            p = Ref(DmrgParams(krylovdim, kryloviter, 1e-14, cutoff, maxdim, mindim, PRECISION[prec]))
+            p = Ref(DmrgParams(krylovdim, kryloviter, 1e-14, cutoff, maxdim, mindim, PRECISION[prec],
+                               nsites))
             check(ccall((:tnh_dmrg_sweep, libtnhip), Cint,
                         (Ptr{Cvoid}, Cint, Ptr{DmrgParams}, Ptr{Cdouble}, Ptr{Cint}, Ptr{Cdouble}),
                         st[], direction, p, e, D, disc), ctx.h, "tnh_dmrg_sweep")

@@ -48,6 +48,7 @@ class DmrgParams(C.Structure):
         ("maxdim", C.c_int),
         ("mindim", C.c_int),
         ("precision", C.c_int),
+        ("nsites", C.c_int),
     ]
 
 

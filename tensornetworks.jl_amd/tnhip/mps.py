@@ -120,9 +120,9 @@ class DMRGState:
             raise
 
     def sweep(self, direction: bool, krylovdim=3, maxiter=2, lanczos_tol=1e-14, cutoff=1e-12,
-              maxdim=1000, mindim=1, precision="f64"):
+              maxdim=1000, mindim=1, precision="f64", nsites=2):
         p = DmrgParams(int(krylovdim), int(maxiter), float(lanczos_tol), float(cutoff), int(maxdim),
-                       int(mindim), precision_code(precision))
+                       int(mindim), precision_code(precision), int(nsites))
         e = C.c_double()
         D = C.c_int()
         disc = C.c_double()
@@ -202,8 +202,8 @@ def dmrg(psi: GMPS, *Hs: GMPS, nsites=2, krylovdim=3, kryloviter=2, ishermitian=
                      HDF5 layout (HDF5.write of GMPS, gmps.jl:359-370, group "psi"); a run
                      resumes by reading it back (tnhip.checkpoint.read_mps) and calling dmrg.
     """
-    if nsites != 2:
-        raise NotImplementedError("only the two-site update (nsites=2) is on the GPU path")
+    if nsites not in (1, 2):
+        raise ValueError("nsites must be 1 or 2")
     if not ishermitian:
         raise NotImplementedError("the GPU eigensolver is the Hermitian Lanczos")
     if psi.rank != 1:
@@ -239,7 +239,8 @@ def dmrg(psi: GMPS, *Hs: GMPS, nsites=2, krylovdim=3, kryloviter=2, ishermitian=
             prec = "f64"  # the last refine_sweeps passes before maxsweeps (all, if fewer)
         t_sweep = time.perf_counter()
         pass_prec = prec
-        cost, D, disc = st.sweep(direction, krylovdim, kryloviter, 1e-14, cutoff, maxdim, mindim, prec)
+        cost, D, disc = st.sweep(direction, krylovdim, kryloviter, 1e-14, cutoff, maxdim, mindim, prec,
+                                 nsites)
         t_sweep = time.perf_counter() - t_sweep  # the sweep returns host scalars: synchronised
         if prec == "f64":
             fp64_passes += 1

@@ -7,7 +7,9 @@ Checks: the state's calculate() against the dense <psi|H|psi> + w <phi|psi> (1e-
 penalty run H + w |phi0><phi0| against the oracle's DMRG on the same start (1e-10 per site)
 and against exact diagonalisation of that operator, which with phi0 the ground state is the
 first excited state of H (1e-10 per site); a phase-rotated complex phi0; and a Hamiltonian
-made of a rank-1 term alone, whose ground state is phi itself with energy -|phi|^2."""
+made of a rank-1 term alone, whose ground state is phi itself with energy -|phi|^2.
+Also the one-site update, dmrg(...; nsites=1): every pass against the oracle's one-site
+DMRG (1e-10 per site), and ED with bonds that hold the exact state."""
 import numpy as np
 import pytest
 
@@ -141,3 +143,48 @@ def test_dmrg_rank1_term_alone(ctx):
     assert abs(e + nrm2) <= 1e-12 * nrm2
     v = to_dense(MPS(psi.tensors))
     assert abs(abs(np.vdot(phid, v)) / np.sqrt(nrm2 * np.vdot(v, v).real) - 1) <= 1e-10
+
+
+def test_dmrg_one_site_matches_oracle_per_pass(ctx):
+    """dmrg(...; nsites=1) (dmrg.jl:35-63, replacesites!'s one-site branch gmps.jl:204-213)
+    from a start whose bonds (6) cannot hold the exact state: every pass's energy against
+    the oracle's one-site DMRG on the same start, 1e-10 per site; the bonds never change."""
+    from tnhip import GMPS, dmrg
+    from oracle.dmrg import dmrg as odmrg
+    from oracle.mps import MPS, random_mps
+
+    N = 10
+    M, _ = _tfim(N, 0.9)
+    start = random_mps(N, 2, 6, np.random.default_rng(5))
+    kw = dict(nsites=1, maxsweeps=6, krylovdim=4)
+    hist, ohist = [], []
+    psi = GMPS(1, 2, [t.copy() for t in start.tensors], 1)
+    psi, e = dmrg(psi, M, verbose=False, history=hist, **kw)
+    _, e_ref = odmrg(MPS([t.copy() for t in start.tensors], 1), M.tensors, history=ohist, **kw)
+    assert len(hist) == len(ohist) == 6
+    for h, o in zip(hist, ohist):
+        assert abs(h["energy"] - o[1]) / N <= 1e-10
+    assert [t.shape for t in psi.tensors] == [t.shape for t in start.tensors]
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_dmrg_one_site_full_bonds_reach_ed(ctx, precision):
+    """With bonds that hold the exact state (N = 8, bond dimension 16) the one-site update
+    reaches the ED ground energy, in FP64 and with FP32 passes refined in FP64; with the
+    ground state as a penalty term it reaches the first excited state."""
+    from tnhip import GMPS, dmrg
+    from oracle.mps import random_mps
+
+    N, w = 8, 10.0
+    M, Hd = _tfim(N, 0.9)
+    ev = np.linalg.eigvalsh(Hd)
+    rng = np.random.default_rng(6)
+    psi = GMPS(1, 2, random_mps(N, 2, 16, rng).tensors, 1)
+    psi, e0 = dmrg(psi, M, nsites=1, maxsweeps=10, krylovdim=4, precision=precision,
+                   verbose=False)
+    assert abs(e0 - ev[0]) / N <= 1e-10
+    phi0 = GMPS(1, 2, [t.copy() for t in psi.tensors], psi.center)
+    psi1 = GMPS(1, 2, random_mps(N, 2, 16, rng).tensors, 1)
+    psi1, e1 = dmrg(psi1, M, phi0, coeffs=[1.0, w], nsites=1, maxsweeps=14, krylovdim=4,
+                    precision=precision, verbose=False)
+    assert abs(e1 - ev[1]) / N <= 1e-10

@@ -133,3 +133,64 @@ def test_projmps_sum_adds_terms():
     psid, phid = to_dense(psi), to_dense(phi)
     want = 0.5 * np.vdot(psid, Hd @ psid) - 2.0 * np.vdot(phid, psid)
     assert abs(s.calculate() - want) < 1e-10
+
+
+# ------------------------------------------------------------ one-site update (nsites = 1)
+
+
+def _site_window_dense(psi, site, A):
+    """Dense state of psi with site `site` replaced by the one-site tensor A."""
+    N = len(psi)
+    left = np.ones((1, 1))
+    for i in range(1, site):
+        t = psi[i]
+        left = np.einsum("Ia,asb->Isb", left, t).reshape(-1, t.shape[2])
+    right = np.ones((1, 1))
+    for i in range(N, site, -1):
+        t = psi[i]
+        right = np.einsum("asb,bJ->asJ", t, right).reshape(t.shape[0], -1)
+    return np.einsum("Ia,asb,bJ->IsJ", left, A, right).reshape(-1)
+
+
+@pytest.mark.parametrize("site", [1, 3, 6])
+def test_one_site_products_are_projected_operators(site):
+    """product(projV, A, direction, nsites=1) of an MPO term and of a rank-1 term
+    (projmps.jl:103-179) against B^H O B for the one-site window's dense isometry B."""
+    rng = np.random.default_rng(30 + site)
+    N, w = 6, 0.9
+    Ws, Hd = _tfim(N, 0.7)
+    psi = _random_mps(N, 2, 3, rng, True)
+    phi = _random_mps(N, 2, 2, rng, True)
+    movecenter(psi, site)
+    shape = psi[site].shape
+    B = np.zeros((2 ** N, int(np.prod(shape))), dtype=complex)
+    for k, idx in enumerate(np.ndindex(shape)):
+        E = np.zeros(shape)
+        E[idx] = 1.0
+        B[:, k] = _site_window_dense(psi, site, E)
+    assert np.allclose(B.conj().T @ B, np.eye(B.shape[1]), atol=1e-12)
+    A = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    H = ProjMPS(psi.tensors, Ws, center=site, coeff=0.5)
+    want = 0.5 * (B.conj().T @ Hd @ B) @ A.reshape(-1)
+    assert np.allclose(H.product(A, False, 1), want.reshape(shape), atol=1e-12)
+    assert np.allclose(H.product(A, True, 1), want.reshape(shape), atol=1e-12)
+    P = ProjMPSSquared(phi.tensors, psi.tensors, center=site, coeff=w)
+    phid = to_dense(phi)
+    want = w * (B.conj().T @ phid) * (phid.conj() @ B @ A.reshape(-1))
+    assert np.allclose(P.product(A, False, 1), want.reshape(shape), atol=1e-12)
+
+
+def test_one_site_dmrg_reaches_ed_and_excited_state():
+    """dmrg(...; nsites=1) (dmrg.jl:35-63 with replacesites!'s one-site branch) from a
+    start whose bonds already hold the exact state (N = 8, bond dimension 16) reaches the
+    ED ground energy, and with the penalty term the first excited state."""
+    N, h, w = 8, 0.9, 10.0
+    Ws, Hd = _tfim(N, h)
+    ev = np.linalg.eigvalsh(Hd)
+    rng = np.random.default_rng(17)
+    phi0, e0 = dmrg(random_mps(N, 2, 16, rng), Ws, nsites=1, maxsweeps=10, krylovdim=4)
+    assert abs(e0 - ev[0]) < 1e-10
+    assert [t.shape[2] for t in phi0.tensors[:-1]] == [2, 4, 8, 16, 8, 4, 2]
+    psi, e1 = dmrg(random_mps(N, 2, 16, rng), Ws, nsites=1, maxsweeps=14, krylovdim=4,
+                   terms=[(phi0.tensors, w)])
+    assert abs(e1 - ev[1]) < 1e-9
