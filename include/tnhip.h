@@ -244,6 +244,15 @@ int tnh_applygate2(tnh_ctx *ctx, tnh_buf **A1, tnh_buf **A2, const tnh_buf *G, i
  * kept bonds, *discarded the largest discarded weight. */
 int tnh_applygate_sites(tnh_ctx *ctx, int nsites, tnh_buf **A, const tnh_buf *G, int direction,
                         double cutoff, int maxdim, int mindim, int *k, double *discarded);
+/* replacesites!(psi, theta, site, direction, normalize; cutoff, maxdim, mindim) for
+ * nsites in [2, 6] (gmps.jl:199-267, the r-site branch :215-257): theta is the FP64/C128
+ * (chi_l, d, ..., d, chi_r) tensor of nsites sites; A[0..nsites-1] receive the new site
+ * tensors (an entry may be NULL or a buffer to reuse; the results are library-owned). The
+ * splits and the outputs k, discarded are those of tnh_applygate_sites; normalize divides the
+ * new centre (the last site for direction 0, the first for direction 1) by its norm. */
+int tnh_replacesites_sites(tnh_ctx *ctx, int nsites, const tnh_buf *theta, int direction,
+                           int normalize, double cutoff, int maxdim, int mindim, tnh_buf **A,
+                           int *k, double *discarded);
 
 /* Local expectation values <psi| O_t |psi> (unnormalised) of nterms operator-list terms
  * over the N device site tensors of psi. Replaces inner(st, psi, oplist, psi),

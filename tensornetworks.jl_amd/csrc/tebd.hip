@@ -4,6 +4,8 @@
 //   normalisation (:157).
 // The gate stage is an HBM-bound mixing pass over theta (the same kernel as the MPO
 // mixing stage of the matvec, with the d^2 x d^2 gate as its matrix).
+#include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "tnh_internal.hpp"
@@ -110,6 +112,63 @@ void applygate2(tnh_ctx *ctx, tnh_buf **A1, tnh_buf **A2, const tnh_buf *G, int 
   buf_delete(ctx, theta2);
 }
 
+// The r - 1 truncated splits of replacesites!'s r-site branch (gmps.jl:215-257) of *cur, a
+// (chi_l, d^r, chi_r) tensor that is consumed (on success *cur ends as a site of A and is set
+// to NULL; on failure the caller still owns it). Sweeping right (direction 0) site i becomes
+// U_k of [(left bond, d), rest] and the rest S V^H; sweeping left site j becomes V_k^H of
+// [rest, (d, right bond)] and the rest U S. k[i] (may be NULL) receives the bond between
+// sites i and i + 1, *disc the largest discarded weight.
+void split_sites(tnh_ctx *ctx, int r, int64_t d, tnh_buf **cur, int direction, double cutoff,
+                 int maxdim, int mindim, tnh_buf **A, int *k, double *disc) {
+  const int64_t cl = (*cur)->dims[0], cr = (*cur)->dims[2];
+  int64_t D = 1;
+  for (int i = 0; i < r; ++i) D *= d;
+  TNH_REQUIRE((*cur)->ndim == 3 && (*cur)->dims[1] == D, TNH_ERR_SHAPE, "split: theta shape");
+  double dmax = 0;
+  SvdOut o;
+  tnh_buf *nxt = nullptr;
+  try {
+    if (!direction) {
+      int64_t c = cl, rem = D;
+      for (int i = 0; i + 1 < r; ++i) {
+        rem /= d;
+        const int64_t ld[3] = {c, d, 0}, rd[3] = {0, rem, cr};
+        const int kk = split_matrix(ctx, *cur, int(c * d), int(rem * cr), 0, cutoff, maxdim,
+                                    mindim, false, 3, ld, 3, rd, &A[i], &nxt, &o, nullptr);
+        if (k) k[i] = kk;
+        dmax = std::max(dmax, o.discarded);
+        std::swap(*cur, nxt);
+        buf_delete(ctx, nxt);
+        nxt = nullptr;
+        c = kk;
+      }
+      buf_delete(ctx, A[r - 1]);
+      A[r - 1] = *cur;
+    } else {
+      int64_t c = cr, rem = D;
+      for (int j = r - 1; j >= 1; --j) {
+        rem /= d;
+        const int64_t ld[3] = {cl, rem, 0}, rd[3] = {0, d, c};
+        const int kk = split_matrix(ctx, *cur, int(cl * rem), int(d * c), 1, cutoff, maxdim,
+                                    mindim, false, 3, ld, 3, rd, &nxt, &A[j], &o, nullptr);
+        if (k) k[j - 1] = kk;
+        dmax = std::max(dmax, o.discarded);
+        std::swap(*cur, nxt);
+        buf_delete(ctx, nxt);
+        nxt = nullptr;
+        c = kk;
+      }
+      buf_delete(ctx, A[0]);
+      A[0] = *cur;
+    }
+  } catch (...) {
+    buf_delete(ctx, nxt);
+    throw;
+  }
+  *cur = nullptr;
+  if (disc) *disc = dmax;
+}
+
 // applygate(psi, site, gate, direction; ...) for a gate on r >= 2 sites, gatelist.jl:137-171:
 // theta = A[0] ... A[r-1] (:142-145), theta'[a, O, b] = sum_I G[O, I] theta[a, I, b]
 // (:149-154), then replacesites! without normalisation (:157), whose r-site branch
@@ -198,43 +257,7 @@ void applygate_sites(tnh_ctx *ctx, int r, tnh_buf **A, const tnh_buf *G, int dir
     buf_delete(ctx, t0);
     buf_delete(ctx, t1);
     t0 = t1 = nullptr;
-    // the r - 1 splits of replacesites! (gmps.jl:215-257)
-    double dmax = 0;
-    SvdOut o;
-    if (!direction) {
-      int64_t c = cl, rem = D;
-      for (int i = 0; i + 1 < r; ++i) {  // sweeping right: site i = U_k, the rest S V^H
-        rem /= d;
-        const int64_t ld[3] = {c, d, 0}, rd[3] = {0, rem, cr};
-        const int kk = split_matrix(ctx, cur, int(c * d), int(rem * cr), 0, cutoff, maxdim, mindim,
-                                    false, 3, ld, 3, rd, &A[i], &nxt, &o, nullptr);
-        if (k) k[i] = kk;
-        dmax = std::max(dmax, o.discarded);
-        std::swap(cur, nxt);
-        c = kk;
-      }
-      buf_delete(ctx, A[r - 1]);
-      A[r - 1] = cur;
-      cur = nullptr;
-    } else {
-      int64_t c = cr, rem = D;
-      for (int j = r - 1; j >= 1; --j) {  // sweeping left: site j = V_k^H, the rest U S
-        rem /= d;
-        const int64_t ld[3] = {cl, rem, 0}, rd[3] = {0, d, c};
-        const int kk = split_matrix(ctx, cur, int(cl * rem), int(d * c), 1, cutoff, maxdim, mindim,
-                                    false, 3, ld, 3, rd, &nxt, &A[j], &o, nullptr);
-        if (k) k[j - 1] = kk;
-        dmax = std::max(dmax, o.discarded);
-        std::swap(cur, nxt);
-        c = kk;
-      }
-      buf_delete(ctx, A[0]);
-      A[0] = cur;
-      cur = nullptr;
-    }
-    buf_delete(ctx, nxt);
-    nxt = nullptr;
-    if (disc) *disc = dmax;
+    split_sites(ctx, r, d, &cur, direction, cutoff, maxdim, mindim, A, k, disc);  // :157
   } catch (...) {
     for (auto *b : {t0, t1, cur, nxt}) buf_delete(ctx, b);
     throw;
@@ -262,5 +285,43 @@ extern "C" int tnh_applygate_sites(tnh_ctx *ctx, int nsites, tnh_buf **A, const 
   return tnh_guard(ctx, [&] {
     TNH_REQUIRE(maxdim >= 0 && mindim >= 0, TNH_ERR_ARG, "maxdim and mindim must be >= 0");
     applygate_sites(ctx, nsites, A, G, direction, cutoff, maxdim, mindim, k, discarded);
+  });
+}
+
+extern "C" int tnh_replacesites_sites(tnh_ctx *ctx, int nsites, const tnh_buf *theta,
+                                      int direction, int normalize, double cutoff, int maxdim,
+                                      int mindim, tnh_buf **A, int *k, double *discarded) {
+  if (!ctx) return TNH_ERR_ARG;
+  return tnh_guard(ctx, [&] {
+    TNH_REQUIRE(maxdim >= 0 && mindim >= 0, TNH_ERR_ARG, "maxdim and mindim must be >= 0");
+    TNH_REQUIRE(nsites >= 2 && nsites <= 6 && A && theta, TNH_ERR_ARG,
+                "replacesites: nsites must be in [2, 6]");
+    TNH_REQUIRE(theta->ndim == nsites + 2 && is_double(theta->dtype), TNH_ERR_SHAPE,
+                "replacesites: theta must be FP64 (chi_l, d, ..., d, chi_r) with nsites d's");
+    const int64_t d = theta->dims[1];
+    for (int i = 1; i <= nsites; ++i)
+      TNH_REQUIRE(theta->dims[i] == d, TNH_ERR_SHAPE, "replacesites: physical dimensions differ");
+    int64_t D = 1;
+    for (int i = 0; i < nsites; ++i) D *= d;
+    const int64_t cd[3] = {theta->dims[0], D, theta->dims[nsites + 1]};
+    tnh_buf *cur = buf_new(ctx, theta->dtype, 3, cd);
+    try {
+      copy_device(ctx, cur->ptr, theta->ptr, theta->bytes);
+      split_sites(ctx, nsites, d, &cur, direction, cutoff, maxdim, mindim, A, k, discarded);
+    } catch (...) {
+      buf_delete(ctx, cur);
+      throw;
+    }
+    if (normalize) {  // normalize!(psi) on the new centre, gmps.jl:262-264
+      tnh_buf *c = direction ? A[0] : A[nsites - 1];
+      tnh_buf v = *c;
+      v.dtype = TNH_F64;
+      v.numel = c->numel * (is_complex(c->dtype) ? 2 : 1);
+      double nr = 0, ni = 0;
+      cdot_host(ctx, &v, &v, &nr, &ni);
+      nr = std::sqrt(std::max(nr, 0.0));
+      TNH_REQUIRE(std::isfinite(nr) && nr > 0, TNH_ERR_NUMERIC, "replacesites: zero centre tensor");
+      scale(ctx, c->ptr, c->dtype, c->numel, 1.0 / nr);
+    }
   });
 }

@@ -326,8 +326,8 @@ end
 """
     replacesites!(GPU(), psi, A, site, direction=false, normalize=false; cutoff, maxdim, mindim)
 
-replacesites!, src/structures/mps/gmps.jl:199-267, for one- and two-site tensors of a
-rank-1 GMPS: the two-site split runs on the GPU (tnh_replacesites2).
+replacesites!, src/structures/mps/gmps.jl:199-267, for a rank-1 GMPS: the splits of two
+sites (tnh_replacesites2) and of three to six sites (tnh_replacesites_sites) run on the GPU.
 """
 function replacesites!(g::GPU, psi::GMPS, A, site::Int, direction::Bool = false,
                        normalize::Bool = false; cutoff::Real = 0.0, maxdim::Int = 0, mindim::Int = 1)
@@ -339,7 +339,24 @@ function replacesites!(g::GPU, psi::GMPS, A, site::Int, direction::Bool = false,
         normalize && TensorNetworks.normalize!(psi)
         return nothing
     end
-    nsites == 2 || error("the GPU split supports one- and two-site tensors")
+    if nsites > 2                                                   # gmps.jl:215-257
+        th = upload(g.ctx, A)
+        hs = fill(C_NULL, nsites); ks = zeros(Cint, nsites - 1); disc = Ref{Cdouble}(0)
+        try
+            GC.@preserve hs ks check(ccall((:tnh_replacesites_sites, libtnhip), Cint,
+                        (Ptr{Cvoid}, Cint, Ptr{Cvoid}, Cint, Cint, Cdouble, Cint, Cint,
+                         Ptr{Ptr{Cvoid}}, Ptr{Cint}, Ptr{Cdouble}),
+                        g.ctx.h, nsites, th.h, direction, normalize, cutoff, maxdim, mindim, hs,
+                        ks, disc), g.ctx.h, "tnh_replacesites_sites")
+        finally
+            outs = [owned(g.ctx, h) for h in hs if h != C_NULL]  # library-owned results
+        end
+        for j = 1:nsites
+            psi[site+j-1] = to_host(outs[j])
+        end
+        psi.center = direction ? site : site + nsites - 1
+        return true
+    end
     th = upload(g.ctx, A)
     a1 = Ref{Ptr{Cvoid}}(C_NULL); a2 = Ref{Ptr{Cvoid}}(C_NULL)
     k = Ref{Cint}(0); disc = Ref{Cdouble}(0)

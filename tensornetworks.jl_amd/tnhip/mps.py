@@ -72,13 +72,29 @@ def randomMPS(dim: int, length: int, bonddim: int, rng=None) -> GMPS:
 
 
 def replacesites(theta, direction: bool, normalize_: bool = True, cutoff=0.0, maxdim=0, mindim=1):
-    """replacesites!(psi, theta, site, direction, normalize; ...), gmps.jl:199-267 for two
-    sites on a host theta; returns (A1, A2, schmidt values, discarded weight)."""
+    """replacesites!(psi, theta, site, direction, normalize; ...), gmps.jl:199-267 on a host
+    theta of two sites, returning (A1, A2, schmidt values, discarded weight); for three to six
+    sites (the r-site branch, :215-257) (list of site tensors, None, largest discarded
+    weight)."""
     from . import ops
 
     t = Tensor.from_numpy(theta)
-    A1, A2, s, disc = ops.replacesites2(t, direction, normalize_, cutoff, maxdim, mindim)
-    return A1.numpy(), A2.numpy(), s, disc
+    n = np.ndim(theta) - 2
+    if n == 2:
+        A1, A2, s, disc = ops.replacesites2(t, direction, normalize_, cutoff, maxdim, mindim)
+        return A1.numpy(), A2.numpy(), s, disc
+    if n < 2:
+        raise ValueError("replacesites: theta must hold at least two sites")
+    ctx = t.ctx
+    arr = (C.c_void_p * n)()
+    ks = (C.c_int * (n - 1))()
+    disc = C.c_double()
+    try:
+        ctx.call("tnh_replacesites_sites", n, t.h, int(bool(direction)), int(bool(normalize_)),
+                 float(cutoff), int(maxdim), int(mindim), arr, ks, C.byref(disc))
+    finally:
+        out = [Tensor(ctx, C.c_void_p(arr[i])) for i in range(n) if arr[i]]
+    return [o.numpy() for o in out], None, disc.value
 
 
 class DMRGState:

@@ -117,6 +117,45 @@ def test_replacesites_vs_oracle(ctx, rng, direction, cplx):
         assert abs(np.linalg.norm(a2) - 1) <= 1e-13
 
 
+@pytest.mark.parametrize("direction", [False, True])
+@pytest.mark.parametrize("n", [3, 4])
+def test_replacesites_multi_site_vs_oracle(ctx, rng, n, direction):
+    """replacesites! of a theta of n sites (gmps.jl:215-257, tnh_replacesites_sites) with
+    truncation (maxdim 5) and normalize = true, against the oracle's restatement: same bonds
+    and discarded weight, the same truncated state, the canonical form of the sweep and a
+    unit-norm centre."""
+    from tnhip.mps import replacesites
+    from oracle.mps import replacesites_n
+
+    shape = (3,) + (2,) * n + (4,)
+    th = randn(rng, shape, True)
+    th /= np.linalg.norm(th)
+    sites, _, disc = replacesites(th, direction, True, 1e-12, 5, 1)
+    ref, disc_ref = replacesites_n(th, direction, 1e-12, 5, 1)
+    assert [t.shape for t in sites] == [t.shape for t in ref]
+    assert max(t.shape[2] for t in sites[:-1]) == 5
+    assert abs(disc - disc_ref) <= 1e-12
+
+    def chain(ts):
+        out = ts[0]
+        for t in ts[1:]:
+            out = np.tensordot(out, t, axes=([out.ndim - 1], [0]))
+        return out
+
+    c = ref[0 if direction else -1]
+    ref[0 if direction else -1] = c / np.linalg.norm(c)  # normalize!(psi) on the new centre
+    assert np.linalg.norm(chain(sites) - chain(ref)) <= 1e-11
+    assert abs(np.linalg.norm(sites[0 if direction else -1]) - 1) <= 1e-13
+    for i, t in enumerate(sites):
+        cl, d, cr = t.shape
+        if not direction and i < n - 1:
+            B = t.reshape(cl * d, cr, order="F")
+            assert np.allclose(B.conj().T @ B, np.eye(cr), atol=1e-12)
+        if direction and i > 0:
+            B = t.reshape(cl, d * cr, order="F")
+            assert np.allclose(B @ B.conj().T, np.eye(cl), atol=1e-12)
+
+
 def test_split_rank_deficient_keeps_orthonormal_factor(ctx, rng):
     """theta of rank 3, cutoff 1e-10 (k_cut = 3) and mindim 6 (tensors.jl:215: k = 6):
     the kept orthonormal factor stays orthonormal."""
