@@ -388,6 +388,8 @@ typedef struct tnh_dmrg_params {
   int mindim;     /* dmrg.jl:19 default 1 */
   int precision;  /* TNH_PREC_* for the matvecs and environment updates */
   int nsites;     /* dmrg.jl:3: 2 (or 0) for the two-site update, 1 for the one-site update */
+  int nonhermitian; /* dmrg.jl:6 ishermitian = false: Arnoldi for the Ritz value of smallest
+                       real part instead of the Lanczos (needs a complex state) */
 } tnh_dmrg_params;
 
 /* Device-resident state of dmrg(psi, H::GMPS...), src/algorithms/mps/dmrg.jl:128-154:
@@ -418,6 +420,9 @@ int tnh_dmrg_destroy(tnh_dmrg *st);
  * rounding floor, and a following FP64 pass restores full accuracy. */
 int tnh_dmrg_sweep(tnh_dmrg *st, int direction, const tnh_dmrg_params *p, double *energy,
                    int *maxbonddim, double *max_discarded);
+/* The last Ritz value of the last pass (dmrg.jl:55 `cost = eig[1]`), complex: with
+ * nonhermitian set the eigenvalue of smallest real part may have an imaginary part. */
+int tnh_dmrg_last_ritz(tnh_dmrg *st, double *re, double *im);
 /* Current energy <psi|H|psi> at the centre: calculate(Hs), dmrg.jl:22 (a rank-1 term
  * contributes coeff Re<phi|psi>, as its calculate does). */
 int tnh_dmrg_energy(tnh_dmrg *st, int precision, double *energy);

@@ -3,14 +3,14 @@ from __future__ import annotations
 
 import numpy as np
 
-from .krylov import eigsolve_ground
+from .krylov import eigsolve_arnoldi, eigsolve_ground
 from .mps import movecenter, normalize, replacesites2
 from .projmps import ProjMPS, ProjMPSSquared, ProjMPSSum
 
 
 def dmrg(psi, Ws, nsites=2, krylovdim=3, kryloviter=2, minsweeps=1, maxsweeps=1000,
          tol=1e-10, tolgrad=1e-5, numconverges=4, cutoff=1e-12, maxdim=1000, mindim=1,
-         verbose=False, coeff=1.0, history=None, terms=()):
+         verbose=False, coeff=1.0, history=None, terms=(), ishermitian=True):
     """dmrg(psi, H::GMPS...; kwargs...), src/algorithms/mps/dmrg.jl:128-154 and :1-99.
 
     psi is an oracle MPS updated in place; Ws the MPO site tensors (with `coeff`).
@@ -19,9 +19,15 @@ def dmrg(psi, Ws, nsites=2, krylovdim=3, kryloviter=2, minsweeps=1, maxsweeps=10
     ProjMPS(phi, psi; rank=2, squared=true) (:140-147); all are summed as ProjMPSSum.
     Returns (psi, cost). `history` (a list) receives (sweep, cost, maxbonddim, max
     discarded). A "sweep" is one directional pass, as in the reference (:66).
+    ishermitian=False solves each local problem with eigsolve_arnoldi (the eigenvalue of
+    smallest real part; psi is made complex) and returns a complex cost.
     """
     assert nsites in (1, 2), "the one- and two-site updates"
     N = len(psi)
+    if not ishermitian:
+        for i in range(1, N + 1):
+            psi[i] = psi[i].astype(np.complex128)
+    solve = eigsolve_ground if ishermitian else eigsolve_arnoldi
     movecenter(psi, 1)                                       # :133
     projs = [ProjMPS(psi.tensors, Ws, center=1, coeff=coeff)]  # :143, projmps.jl:16-42
     for T, c in terms:                                       # :139-149
@@ -52,8 +58,8 @@ def dmrg(psi, Ws, nsites=2, krylovdim=3, kryloviter=2, minsweeps=1, maxsweeps=10
             A0 = psi[site1]                                  # :44-47
             if nsites == 2:
                 A0 = np.tensordot(A0, psi[site1 + 1], axes=([2], [0]))
-            e, vec, _ = eigsolve_ground(lambda x: Hs.product(x, direction, nsites), A0,
-                                        krylovdim, kryloviter, 1e-14)       # :50-53
+            e, vec, _ = solve(lambda x: Hs.product(x, direction, nsites), A0,
+                              krylovdim, kryloviter, 1e-14)                 # :50-53
             cost = e                                         # :55
             if nsites == 1:  # replacesites!'s one-site branch, gmps.jl:204-213
                 psi[site1] = vec

@@ -79,3 +79,66 @@ def eigsolve_ground(matvec, x0, krylovdim=3, maxiter=2, tol=1e-14):
     ritz = y @ V[:m]
     ritz = ritz / np.linalg.norm(ritz)
     return theta, ritz.reshape(shape, order="F"), nmv
+
+
+def eigsolve_arnoldi(matvec, x0, krylovdim=3, maxiter=2, tol=1e-14):
+    """The non-Hermitian counterpart, eigsolve(Heff, A0, 1, :SR; ishermitian=false, ...) of
+    dmrg.jl:51-53: Arnoldi with the same two-pass Gram-Schmidt and breakdown test, the Ritz
+    value of smallest real part of the Hessenberg projection (scipy.linalg.eig) and the same
+    one-vector restart (A u = lambda u + b v_m, so the next projection starts with
+    [lambda; b] in column 0). KrylovKit's non-Hermitian eigsolve restarts from a Schur form;
+    with one wanted value and one kept vector that is the same subspace. Returns (complex Ritz
+    value, normalised Ritz vector, number of matvecs). Pinned like the Lanczos: cycle c is
+    Rayleigh-Ritz on K_k(A, y_{c-1}), and the converged pair against dense eigenvalues."""
+    import scipy.linalg as sla
+
+    shape = x0.shape
+    x = x0.reshape(-1, order="F").astype(np.complex128, copy=True)
+    n = x.size
+    kdim = max(1, min(krylovdim, n))
+    V = np.zeros((kdim + 1, n), dtype=np.complex128)
+    H = np.zeros((kdim + 1, kdim), dtype=np.complex128)
+    V[0] = x / np.linalg.norm(x)
+    start = 0
+    nmv = 0
+
+    def mv(v):
+        return matvec(v.reshape(shape, order="F")).reshape(-1, order="F")
+
+    lam, y, m = 0.0, None, 0
+    for it in range(maxiter):
+        m = kdim
+        breakdown = False
+        for j in range(start, kdim):
+            w = mv(V[j]).astype(np.complex128)
+            nmv += 1
+            for _ in range(2):
+                c = V[: j + 1].conj() @ w
+                H[: j + 1, j] += c
+                w = w - c @ V[: j + 1]
+            beta = np.linalg.norm(w)
+            H[j + 1, j] = beta
+            scale = max(1.0, np.abs(H[: j + 2, : j + 1]).max())
+            if beta <= 1e-14 * scale:
+                m = j + 1
+                breakdown = True
+                break
+            V[j + 1] = w / beta
+        evals, evecs = sla.eig(H[:m, :m])
+        i = int(np.argmin(evals.real))                            # :SR
+        lam, y = complex(evals[i]), evecs[:, i] / np.linalg.norm(evecs[:, i])
+        b = H[m, :m] @ y
+        if breakdown or abs(b) <= tol or it == maxiter - 1:
+            break
+        ritz = y @ V[:m]
+        V2 = np.zeros_like(V)
+        V2[0] = ritz / np.linalg.norm(ritz)
+        V2[1] = V[m]
+        V = V2
+        H = np.zeros_like(H)
+        H[0, 0] = lam
+        H[1, 0] = b
+        start = 1
+    ritz = y @ V[:m]
+    ritz = ritz / np.linalg.norm(ritz)
+    return lam, ritz.reshape(shape, order="F"), nmv

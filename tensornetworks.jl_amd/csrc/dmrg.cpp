@@ -29,6 +29,7 @@ struct tnh_dmrg {
   int env_center = 0;   // ProjMPS centre (0 = blocks not built)
   int psi_center = 0;
   bool cplx = false;    // psi (and every term) held complex
+  double ritz_re = 0, ritz_im = 0;  // the last Ritz value of the last pass
   // Rank-1 terms coeff |phi><phi|, ProjMPS(phi, psi; rank=2, squared=true, coeff)
   // (dmrg.jl:144-145): phi's site tensors, the overlap blocks per site, and the projection of
   // phi onto the bond being updated.
@@ -108,6 +109,21 @@ struct tnh_dmrg {
 };
 
 namespace {
+
+// the local eigensolve of dmrg.jl:51-53: Lanczos, or Arnoldi for ishermitian = false
+double local_eig(tnh_dmrg *st, const Matvec &mv, const tnh_dmrg_params *p, int *nmv) {
+  if (p->nonhermitian) {
+    const std::complex<double> l = eigsolve_nonherm_op(st->ctx, mv, st->theta, p->krylovdim,
+                                                       p->maxiter, p->lanczos_tol, nmv);
+    st->ritz_re = l.real();
+    st->ritz_im = l.imag();
+    return l.real();
+  }
+  st->ritz_re = eigsolve_ground_op(st->ctx, mv, st->theta, p->krylovdim, p->maxiter,
+                                   p->lanczos_tol, nmv);
+  st->ritz_im = 0.0;
+  return st->ritz_re;
+}
 
 void copy_in(tnh_ctx *ctx, const tnh_buf *src, bool cplx, tnh_buf **dst) {
   TNH_REQUIRE(src, TNH_ERR_ARG, "NULL tensor");
@@ -208,9 +224,8 @@ double one_site_pass(tnh_dmrg *st, int direction, const tnh_dmrg_params *p) {
     for (auto &q : st->proj)
       proj_project1(ctx, st->pblock(q, site - 1), q.phi[site - 1], st->pblock(q, site + 1), &q.P);
     int nmv = 0;
-    cost = eigsolve_ground_op(
-        ctx, [&](const tnh_buf *x, tnh_buf *y) { site_matvec(st, site, prec, x, y); }, st->theta,
-        p->krylovdim, p->maxiter, p->lanczos_tol, &nmv);
+    cost = local_eig(
+        st, [&](const tnh_buf *x, tnh_buf *y) { site_matvec(st, site, prec, x, y); }, p, &nmv);
     copy_device(ctx, A->ptr, st->theta->ptr, A->bytes);  // psi[site] = A, gmps.jl:205
     const int nxt = site + 1 - 2 * direction;
     int c = site;
@@ -383,6 +398,8 @@ int tnh_dmrg_sweep(tnh_dmrg *st, int direction, const tnh_dmrg_params *p, double
     TNH_REQUIRE(st->nterms + st->proj.size() >= 1, TNH_ERR_ARG,
                 "You must provide atleast one MPS/MPO for the Hamiltonian.");
     TNH_REQUIRE(p->nsites >= 0 && p->nsites <= 2, TNH_ERR_ARG, "nsites must be 1 or 2");
+    TNH_REQUIRE(!p->nonhermitian || st->cplx, TNH_ERR_ARG,
+                "ishermitian = false needs a complex state (create it from complex tensors)");
     double cost = 0, maxdisc = 0;
     if (p->nsites == 1) {
       cost = one_site_pass(st, direction, p);
@@ -412,9 +429,9 @@ int tnh_dmrg_sweep(tnh_dmrg *st, int direction, const tnh_dmrg_params *p, double
         proj_project(ctx, st->pblock(q, site1 - 1), q.phi[site1 - 1], q.phi[site1],
                      st->pblock(q, site1 + 2), &q.P);
       int nmv = 0;
-      cost = eigsolve_ground_op(
-          ctx, [&](const tnh_buf *x, tnh_buf *y) { bond_matvec(st, site1, prec, x, y); },
-          st->theta, p->krylovdim, p->maxiter, p->lanczos_tol, &nmv);  // :50-55
+      cost = local_eig(
+          st, [&](const tnh_buf *x, tnh_buf *y) { bond_matvec(st, site1, prec, x, y); }, p,
+          &nmv);  // :50-55
       ctx->prof_end("dmrg_lanczos", ph, 0, 0);
       ph = ctx->prof_begin();
       int k = 0;
@@ -469,6 +486,13 @@ int tnh_dmrg_energy(tnh_dmrg *st, int precision, double *energy) {
     }
     if (energy) *energy = e;
   });
+}
+
+int tnh_dmrg_last_ritz(tnh_dmrg *st, double *re, double *im) {
+  if (!st) return TNH_ERR_ARG;
+  if (re) *re = st->ritz_re;
+  if (im) *im = st->ritz_im;
+  return TNH_OK;
 }
 
 int tnh_dmrg_site(tnh_dmrg *st, int site, const tnh_buf **out) {

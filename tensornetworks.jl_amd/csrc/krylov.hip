@@ -10,6 +10,7 @@
 #include <cmath>
 #include <vector>
 
+#include "hess_eig.hpp"
 #include "tnh_internal.hpp"
 
 namespace tnh {
@@ -487,6 +488,155 @@ double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, in
   scale(ctx, theta->ptr, TNH_F64, vstride, 1.0 / rn);
   if (nmv_out) *nmv_out = nmv;
   return theta_e;
+}
+
+// Arnoldi for the eigenvalue of smallest real part of a non-Hermitian operator, as KrylovKit's
+// eigsolve(f, x0, 1, :SR; ishermitian = false) that dmrg.jl:51-53 calls with ishermitian =
+// false: the same device cycle as the Lanczos above (matvec, two Gram-Schmidt passes whose
+// coefficients are the Hessenberg columns, normalisation), a restart that keeps the one Ritz
+// vector u (A u = lambda u + b v_m, so the next projection starts with [lambda; b] in column
+// 0), and per cycle one read-back of the coefficients: the m x m Hessenberg problem is solved
+// on the host (hess_eig.hpp). A step whose new norm is at most 1e-14 times the largest |h| is
+// a breakdown (invariant subspace). theta must be complex. Returns the Ritz value.
+std::complex<double> eigsolve_nonherm_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta,
+                                         int kdim, int maxiter, double tol, int *nmv_out) {
+  using cd = std::complex<double>;
+  TNH_REQUIRE(theta && theta->ndim == 4, TNH_ERR_SHAPE, "theta must be (chi_l, d, d, chi_r)");
+  TNH_REQUIRE(theta->dtype == TNH_C128, TNH_ERR_ARG,
+              "the non-Hermitian eigensolve needs a complex (C128) theta");
+  TNH_REQUIRE(kdim >= 1 && kdim < kMaxVec, TNH_ERR_ARG, "krylovdim must be in [1, 31]");
+  TNH_REQUIRE(maxiter >= 1, TNH_ERR_ARG, "maxiter must be >= 1");
+  const int64_t n = theta->numel;
+  TNH_REQUIRE(n > 0, TNH_ERR_SHAPE, "empty theta");
+  if (int64_t(kdim) > n) kdim = int(n);
+  const int64_t vstride = 2 * n;
+  double *V = static_cast<double *>(ctx->workspace(700, sizeof(double) * vstride * (kdim + 2)));
+  double *Wv = V + vstride * (kdim + 1);
+  double *part = static_cast<double *>(ctx->workspace(701, sizeof(double) * 2 * kDotBlocks * kMaxVec));
+  double *dres = static_cast<double *>(ctx->workspace(702, sizeof(double) * 2 * kMaxVec));
+  double *hdev = static_cast<double *>(ctx->workspace(703, sizeof(double) * kStep * kMaxVec));
+  double *sdev = static_cast<double *>(ctx->workspace(704, sizeof(double) * kStateLen));
+  double *hres = ctx->host_scalars;
+  const int g = grid1(ctx, n);
+  auto dots = [&](const double *Vp, int nv, const double *w, double *out) {
+    if (n <= kOneBlockDot) {
+      hipLaunchKernelGGL(mdot1_kernel, dim3(nv), dim3(1024), 0, ctx->stream, Vp, vstride, w, n, true, out);
+    } else {
+      hipLaunchKernelGGL(mdot_kernel, dim3(kDotBlocks, nv), dim3(256), 0, ctx->stream, Vp, vstride, w, n,
+                         true, part);
+      hipLaunchKernelGGL(mdot_final_kernel, dim3(nv), dim3(64), 0, ctx->stream, part, kDotBlocks, nv, out);
+    }
+  };
+  auto view = [&](double *ptr) {
+    tnh_buf b = *theta;
+    b.ptr = ptr;
+    b.owned = false;
+    return b;
+  };
+  auto norm_of = [&](const double *w) {
+    dots(w, 1, w, dres);
+    TNH_HIP(hipGetLastError());
+    to_host(ctx, hres, dres, sizeof(double) * 2);
+    return std::sqrt(std::max(0.0, hres[0]));
+  };
+  copy_device(ctx, V, theta->ptr, sizeof(double) * vstride);
+  {
+    const double nr = norm_of(V);
+    TNH_REQUIRE(std::isfinite(nr) && nr > 0, TNH_ERR_NUMERIC,
+                "eigsolve: initial vector has zero or non-finite norm");
+    scale(ctx, V, TNH_F64, vstride, 1.0 / nr);
+  }
+  std::vector<double> hh(size_t(kStep) * kMaxVec), st(kStateLen, 0.0);
+  std::vector<cd> H(size_t(kMaxVec) * kMaxVec), beta(kMaxVec), y;
+  auto Hat = [&](int i, int j) -> cd & { return H[size_t(i) + size_t(j) * kMaxVec]; };
+  cd lam(0, 0), keep_lam(0, 0), keep_b(0, 0);
+  int start = 0, nmv = 0;
+  for (int it = 0; it < maxiter; ++it) {
+    for (int j = start; j < kdim; ++j) {
+      tnh_buf vj = view(V + j * vstride), wb = view(Wv);
+      matvec(&vj, &wb);
+      double *hj = hdev + int64_t(j) * kStep;
+      for (int pass = 0; pass < 2; ++pass) {
+        double *cj = hj + pass * 2 * kMaxVec;
+        dots(V, j + 1, Wv, cj);
+        hipLaunchKernelGGL(vsub_dev_kernel, dim3(g), dim3(256), 0, ctx->stream, Wv, V, vstride, j + 1, cj,
+                           n, true);
+      }
+      double *nj = hj + 4 * kMaxVec;
+      dots(Wv, 1, Wv, nj);
+      hipLaunchKernelGGL(normalize_copy_kernel, dim3(g), dim3(256), 0, ctx->stream, V + (j + 1) * vstride,
+                         Wv, nj, vstride);
+      TNH_HIP(hipGetLastError());
+    }
+    to_host(ctx, hh.data() + size_t(start) * kStep, hdev + int64_t(start) * kStep,
+            sizeof(double) * kStep * (kdim - start));
+    // the Hessenberg projection of this cycle
+    double scale_h = 0;
+    if (start == 1) {
+      Hat(0, 0) = keep_lam;
+      beta[0] = keep_b;
+      scale_h = std::max(std::abs(keep_lam), std::abs(keep_b));
+    }
+    int m = kdim;
+    bool brk = false;
+    for (int j = start; j < kdim; ++j) {
+      const double *hj = hh.data() + size_t(j) * kStep;
+      for (int i = 0; i <= j; ++i) {
+        const cd h(hj[2 * i] + hj[2 * kMaxVec + 2 * i], hj[2 * i + 1] + hj[2 * kMaxVec + 2 * i + 1]);
+        TNH_REQUIRE(std::isfinite(h.real()) && std::isfinite(h.imag()), TNH_ERR_NUMERIC,
+                    "eigsolve: non-finite Krylov coefficient");
+        Hat(i, j) = h;
+        scale_h = std::max(scale_h, std::abs(h));
+      }
+      const double q = hj[4 * kMaxVec];
+      TNH_REQUIRE(std::isfinite(q), TNH_ERR_NUMERIC, "eigsolve: non-finite Krylov vector");
+      beta[j] = cd(std::sqrt(std::max(q, 0.0)), 0.0);
+      if (beta[j].real() <= 1e-14 * scale_h) {  // invariant subspace: stop the cycle here
+        m = j + 1;
+        brk = true;
+        break;
+      }
+    }
+    nmv += m - start;
+    std::vector<cd> Hm(size_t(m) * m), ev;
+    for (int j = 0; j < m; ++j)
+      for (int i = 0; i < m; ++i)
+        Hm[i + size_t(j) * m] = i <= j ? Hat(i, j) : (i == j + 1 ? beta[j] : cd(0, 0));
+    TNH_REQUIRE(hess::eigenvalues(m, Hm, ev), TNH_ERR_NOCONV,
+                "eigsolve: the Hessenberg QR did not converge");
+    int w = 0;
+    for (int i = 1; i < m; ++i)
+      if (ev[i].real() < ev[w].real()) w = i;  // :SR
+    lam = ev[w];
+    hess::eigenvector(m, Hm, lam, y);
+    const cd b = brk ? cd(0, 0) : beta[m - 1] * y[m - 1];
+    std::fill(st.begin(), st.end(), 0.0);
+    st[kSM] = m;
+    for (int i = 0; i < m; ++i) {
+      st[kSY + i] = y[i].real();
+      st[kSY + kMaxVec + i] = y[i].imag();
+    }
+    to_device(ctx, sdev, st.data(), sizeof(double) * kStateLen);
+    if (brk || std::abs(b) <= tol || it == maxiter - 1) break;
+    // restart: V0 <- u = V y (unit norm), V1 <- v_m; column 0 of the next projection [lam; b]
+    hipLaunchKernelGGL(lincomb_dev_kernel, dim3(g), dim3(256), 0, ctx->stream, Wv, V, vstride, sdev, n, true);
+    dots(Wv, 1, Wv, dres);
+    TNH_HIP(hipGetLastError());
+    copy_device(ctx, V + vstride, V + int64_t(m) * vstride, sizeof(double) * vstride);
+    hipLaunchKernelGGL(normalize_copy_kernel, dim3(g), dim3(256), 0, ctx->stream, V, Wv, dres, vstride);
+    TNH_HIP(hipGetLastError());
+    keep_lam = lam;
+    keep_b = b;
+    start = 1;
+  }
+  hipLaunchKernelGGL(lincomb_dev_kernel, dim3(g), dim3(256), 0, ctx->stream, static_cast<double *>(theta->ptr), V,
+                     vstride, sdev, n, true);
+  TNH_HIP(hipGetLastError());
+  const double rn = norm_of(static_cast<double *>(theta->ptr));
+  TNH_REQUIRE(std::isfinite(rn) && rn > 0, TNH_ERR_NUMERIC, "eigsolve: zero Ritz vector");
+  scale(ctx, theta->ptr, TNH_F64, vstride, 1.0 / rn);
+  if (nmv_out) *nmv_out = nmv;
+  return lam;
 }
 
 }  // namespace tnh

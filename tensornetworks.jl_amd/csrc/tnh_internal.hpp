@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <complex>
 #include <cstdint>
 #include <cstring>
 #include <exception>
@@ -267,6 +268,9 @@ void cdot_host(tnh_ctx *ctx, const tnh_buf *x, const tnh_buf *y, double *re, dou
 using Matvec = std::function<void(const tnh_buf *x, tnh_buf *y)>;
 double eigsolve_ground_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta, int kdim,
                           int maxiter, double tol, int *nmv_out);
+// the same for a non-Hermitian operator: the Ritz value of smallest real part (complex theta)
+std::complex<double> eigsolve_nonherm_op(tnh_ctx *ctx, const Matvec &matvec, tnh_buf *theta,
+                                         int kdim, int maxiter, double tol, int *nmv_out);
 // warm (optional, in/out): a p x p basis of the Gram space from an earlier split of the same
 // bond, used as the Jacobi's starting basis when its size and dtype match, and replaced by
 // this split's eigenvectors (block-Jacobi path, p > 64).

@@ -597,6 +597,7 @@ struct DmrgParams
     mindim::Cint
     precision::Cint
     nsites::Cint
+    nonhermitian::Cint
 end
 
 """
@@ -612,7 +613,8 @@ after every pass in the reference's HDF5 layout (group "psi", gmps.jl:359-370), 
 a run resumes by reading it back (`read_checkpoint`) and calling dmrg again.
 """
 function dmrg(g::GPU, psi::GMPS, Hs::GMPS...; nsites::Int = 2, krylovdim::Int = 3,
-              kryloviter::Int = 2, minsweeps::Int = 1, maxsweeps::Int = 1000,
+              kryloviter::Int = 2, ishermitian::Bool = true, minsweeps::Int = 1,
+              maxsweeps::Int = 1000,
               tol::Float64 = 1e-10, tolgrad::Float64 = 1e-5, numconverges::Real = 4,
               verbose::Bool = true, cutoff::Float64 = 1e-12, maxdim::Int = 1000,
               mindim::Int = 1, coeffs = ones(length(Hs)), precision::Symbol = :f64,
@@ -628,7 +630,8 @@ function dmrg(g::GPU, psi::GMPS, Hs::GMPS...; nsites::Int = 2, krylovdim::Int = 
     phis = [j for j = 1:length(Hs) if TensorNetworks.rank(Hs[j]) == 1]
     length(mpos) + length(phis) == length(Hs) ||
         error("Hamiltonian must be composed of MPOs (rank 2) or MPSs (rank 1).")
-    cplx = any(!all(iszero ∘ imag, Hs[j][i]) for j in phis for i = 1:N)  # a complex phi needs a complex psi
+    # a complex phi, or the Arnoldi of ishermitian = false, needs a complex psi
+    cplx = !ishermitian || any(!all(iszero ∘ imag, Hs[j][i]) for j in phis for i = 1:N)
     dpsi = [upload(ctx, psi[i]; real_if_possible = !cplx) for i = 1:N]
     dW = [upload(ctx, Hs[j][i]) for j in mpos for i = 1:N]
     st = Ref{Ptr{Cvoid}}(C_NULL)
@@ -659,11 +662,19 @@ function dmrg(g::GPU, psi::GMPS, Hs::GMPS...; nsites::Int = 2, krylovdim::Int = 
                 prec = :f64
             end
             p = Ref(DmrgParams(krylovdim, kryloviter, 1e-14, cutoff, maxdim, mindim, PRECISION[prec],
-                               nsites))
+                               nsites, !ishermitian))
             check(ccall((:tnh_dmrg_sweep, libtnhip), Cint,
                         (Ptr{Cvoid}, Cint, Ptr{DmrgParams}, Ptr{Cdouble}, Ptr{Cint}, Ptr{Cdouble}),
                         st[], direction, p, e, D, disc), ctx.h, "tnh_dmrg_sweep")
-            cost = e[]; direction = !direction; sweeps += 1
+            cost = e[]
+            if !ishermitian  # the Ritz value of smallest real part, possibly complex
+                re = Ref{Cdouble}(0); im = Ref{Cdouble}(0)
+                check(ccall((:tnh_dmrg_last_ritz, libtnhip), Cint,
+                            (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}), st[], re, im),
+                      ctx.h, "tnh_dmrg_last_ritz")
+                cost = complex(re[], im[])
+            end
+            direction = !direction; sweeps += 1
             if sweeps >= minsweeps
                 dc = diff(cost, lastcost)
                 csweeps = (dc < tol && lastD == D[]) ? csweeps + 1 : 0

@@ -49,6 +49,7 @@ class DmrgParams(C.Structure):
         ("mindim", C.c_int),
         ("precision", C.c_int),
         ("nsites", C.c_int),
+        ("nonhermitian", C.c_int),
     ]
 
 
@@ -130,6 +131,7 @@ SIGNATURES = {
     "tnh_dmrg_destroy": (C.c_int, [C.c_void_p]),
     "tnh_dmrg_sweep": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(DmrgParams), C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "tnh_dmrg_energy": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "tnh_dmrg_last_ritz": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "tnh_dmrg_site": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(buf_p)]),
     "tnh_dmrg_center": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "tnh_dmrg_bonddims": (C.c_int, [C.c_void_p, i64p]),

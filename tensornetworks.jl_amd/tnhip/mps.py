@@ -104,7 +104,7 @@ class DMRGState:
     coeff |phi><phi|, ProjMPS(phi, psi; squared=true)), as dmrg(psi, Hs...) does
     (dmrg.jl:139-149); the rank-1 terms are summed after the MPO terms."""
 
-    def __init__(self, psi: GMPS, Hs, coeffs=None, ctx: Context | None = None):
+    def __init__(self, psi: GMPS, Hs, coeffs=None, ctx: Context | None = None, complex_state=False):
         self.ctx = ctx or Context.default()
         N = len(psi)
         Hs = list(Hs)
@@ -117,7 +117,7 @@ class DMRGState:
         if len(mpos) + len(phis) != len(Hs):
             raise ValueError("Hamiltonian must be composed of MPOs (rank 2) or MPSs (rank 1).")
         # a complex rank-1 term makes the state complex (the library keeps one dtype)
-        cplx = any(np.iscomplexobj(t) for H, _ in phis for t in H.tensors)
+        cplx = complex_state or any(np.iscomplexobj(t) for H, _ in phis for t in H.tensors)
         pts = [np.asarray(t, dtype=np.complex128) for t in psi.tensors] if cplx else psi.tensors
         dpsi = _upload_all(pts, self.ctx)
         dW = [Tensor.from_numpy(w, ctx=self.ctx) for H, _ in mpos for w in H.tensors]
@@ -136,15 +136,21 @@ class DMRGState:
             raise
 
     def sweep(self, direction: bool, krylovdim=3, maxiter=2, lanczos_tol=1e-14, cutoff=1e-12,
-              maxdim=1000, mindim=1, precision="f64", nsites=2):
+              maxdim=1000, mindim=1, precision="f64", nsites=2, nonhermitian=False):
+        """One pass; returns (last Ritz value, max bond dim, max discarded weight). With
+        nonhermitian the Ritz value (smallest real part) is returned as a complex number."""
         p = DmrgParams(int(krylovdim), int(maxiter), float(lanczos_tol), float(cutoff), int(maxdim),
-                       int(mindim), precision_code(precision), int(nsites))
+                       int(mindim), precision_code(precision), int(nsites), int(bool(nonhermitian)))
         e = C.c_double()
         D = C.c_int()
         disc = C.c_double()
-        self.ctx.check(__import__("tnhip")._lib.lib.tnh_dmrg_sweep(self.h, int(bool(direction)), C.byref(p),
-                                                                  C.byref(e), C.byref(D), C.byref(disc)),
-                       "tnh_dmrg_sweep")
+        self.ctx.check(_lib.lib.tnh_dmrg_sweep(self.h, int(bool(direction)), C.byref(p), C.byref(e),
+                                               C.byref(D), C.byref(disc)), "tnh_dmrg_sweep")
+        if nonhermitian:
+            re, im = C.c_double(), C.c_double()
+            self.ctx.check(_lib.lib.tnh_dmrg_last_ritz(self.h, C.byref(re), C.byref(im)),
+                           "tnh_dmrg_last_ritz")
+            return complex(re.value, im.value), D.value, disc.value
         return e.value, D.value, disc.value
 
     def energy(self):
@@ -203,7 +209,10 @@ def dmrg(psi: GMPS, *Hs: GMPS, nsites=2, krylovdim=3, kryloviter=2, ishermitian=
     """dmrg(psi, H::GMPS...; kwargs...), src/algorithms/mps/dmrg.jl:128-154 and :1-99.
 
     psi is updated in place and (psi, cost) returned, as in the reference. The whole bond
-    step runs on the GPU (tnh_dmrg_sweep). Additional keywords of this implementation:
+    step runs on the GPU (tnh_dmrg_sweep). ishermitian=False solves each local problem by
+    Arnoldi for the eigenvalue of smallest real part (KrylovKit's eigsolve with
+    ishermitian=false) on a complex copy of psi; the returned cost is then complex.
+    Additional keywords of this implementation:
       precision      "f64" | "f32" | "bf16x3": arithmetic of the matvecs and environment
                      updates; Krylov vectors, the Gram/SVD and reductions stay FP64.
       refine_sweeps  with a lower precision, switch to FP64 for the final passes: once
@@ -220,8 +229,6 @@ def dmrg(psi: GMPS, *Hs: GMPS, nsites=2, krylovdim=3, kryloviter=2, ishermitian=
     """
     if nsites not in (1, 2):
         raise ValueError("nsites must be 1 or 2")
-    if not ishermitian:
-        raise NotImplementedError("the GPU eigensolver is the Hermitian Lanczos")
     if psi.rank != 1:
         raise ValueError("Psi must be a GMPS of rank 1 (vector).")
     if len(Hs) == 0:
@@ -231,7 +238,7 @@ def dmrg(psi: GMPS, *Hs: GMPS, nsites=2, krylovdim=3, kryloviter=2, ishermitian=
             raise ValueError("GMPS must share the same properties.")
         if H.rank not in (1, 2):
             raise ValueError("Hamiltonian must be composed of MPOs (rank 2) or MPSs (rank 1).")
-    st = DMRGState(psi, Hs, coeffs)
+    st = DMRGState(psi, Hs, coeffs, complex_state=not ishermitian)
     if state_out is not None:
         state_out.append(st)
     cost = st.energy()                                    # :22
@@ -256,7 +263,7 @@ def dmrg(psi: GMPS, *Hs: GMPS, nsites=2, krylovdim=3, kryloviter=2, ishermitian=
         t_sweep = time.perf_counter()
         pass_prec = prec
         cost, D, disc = st.sweep(direction, krylovdim, kryloviter, 1e-14, cutoff, maxdim, mindim, prec,
-                                 nsites)
+                                 nsites, not ishermitian)
         t_sweep = time.perf_counter() - t_sweep  # the sweep returns host scalars: synchronised
         if prec == "f64":
             fp64_passes += 1
@@ -291,7 +298,7 @@ def dmrg(psi: GMPS, *Hs: GMPS, nsites=2, krylovdim=3, kryloviter=2, ishermitian=
 
             write_mps(checkpoint, GMPS(1, psi.dim, st.tensors(), st.center()))
         if verbose:
-            print(f"Sweep={sweeps}, energy={cost:.12f}, maxbonddim={D} ")
+            print(f"Sweep={sweeps}, energy={np.real(cost):.12f}, maxbonddim={D} ")
     psi.tensors = st.tensors()
     psi.center = st.center()
     if state_out is None:

@@ -268,3 +268,82 @@ def test_lanczos_cycles_are_rayleigh_ritz_on_fixed_krylov_spaces(rng, cplx):
     # the sequence decreases monotonically towards the ground state
     es = [eigsolve_ground(lambda x: A @ x, x0, k, c, 0.0)[0] for c in (1, 2, 3, 6)]
     assert all(a >= b - 1e-12 for a, b in zip(es, es[1:]))
+
+
+# ----------------------------------------------------------- Arnoldi (ishermitian = false)
+
+
+def _rayleigh_ritz_smallest_real(A, x, k):
+    """Ritz pair of smallest real part of A on K_k(A, x), from a QR basis of the monomial
+    Krylov vectors, independent of the Arnoldi recurrences."""
+    K = [x / np.linalg.norm(x)]
+    for _ in range(k - 1):
+        w = A @ K[-1]
+        K.append(w / np.linalg.norm(w))
+    Q, _ = np.linalg.qr(np.stack(K, axis=1))
+    w, V = np.linalg.eig(Q.conj().T @ A @ Q)
+    i = int(np.argmin(w.real))
+    v = Q @ V[:, i]
+    return w[i], v / np.linalg.norm(v)
+
+
+def test_arnoldi_cycles_are_rayleigh_ritz_on_fixed_krylov_spaces(rng):
+    """As for the Lanczos: cycle 1 is Rayleigh-Ritz on K_k(A, x0) for a non-normal complex A,
+    and after a restart on K_k(A, y) of the kept Ritz vector y (A y = lambda y + b v_k)."""
+    from oracle.krylov import eigsolve_arnoldi
+
+    n, k = 30, 5
+    A = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n)
+    A += np.diag(np.linspace(-2.0, 2.0, n))
+    x0 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    y = x0
+    for cycles in (1, 2, 3):
+        e, v, nmv = eigsolve_arnoldi(lambda x: A @ x, x0, krylovdim=k, maxiter=cycles, tol=0.0)
+        e_rr, v_rr = _rayleigh_ritz_smallest_real(A, y, k)
+        assert nmv == k + (cycles - 1) * (k - 1)
+        assert abs(e - e_rr) <= 1e-10 * np.abs(A).max(), (cycles, e, e_rr)
+        assert abs(abs(np.vdot(v, v_rr)) - 1) <= 1e-8
+        y = v
+
+
+def test_arnoldi_converges_to_dense_smallest_real_part(rng):
+    from oracle.krylov import eigsolve_arnoldi
+
+    n = 40
+    A = rng.standard_normal((n, n)) / np.sqrt(n) + np.diag(np.linspace(-3.0, 3.0, n))
+    A[0, 0] = -6.0  # a well separated smallest real part
+    x0 = rng.standard_normal(n)
+    e, v, _ = eigsolve_arnoldi(lambda x: A @ x, x0, krylovdim=12, maxiter=40, tol=1e-13)
+    ev = np.linalg.eigvals(A)
+    want = ev[np.argmin(ev.real)]
+    assert abs(e - want) <= 1e-10
+    assert np.linalg.norm(A @ v - e * v) <= 1e-9
+
+
+def hatano_nelson_terms(N, g, delta):
+    """Asymmetric hopping e^{+g} / e^{-g} plus zz: non-Hermitian, but with open boundaries
+    similar to the Hermitian chain of g = 0, so its spectrum is real and known from ED."""
+    return ([(["s+", "s-"], [i, i + 1], -np.exp(g)) for i in range(1, N)]
+            + [(["s-", "s+"], [i, i + 1], -np.exp(-g)) for i in range(1, N)]
+            + [(["z", "z"], [i, i + 1], delta) for i in range(1, N)])
+
+
+def test_oracle_nonhermitian_dmrg_reaches_the_similar_hermitian_spectrum():
+    """dmrg(...; ishermitian=false) on the Hatano-Nelson chain (N = 8, bonds that hold the
+    exact state): the cost is the eigenvalue of smallest real part, which equals the lowest
+    eigenvalue of the Hermitian chain it is similar to."""
+    from oracle.dmrg import dmrg
+    from oracle.mpo import mpo
+    from oracle.mps import random_mps
+
+    N, g, delta = 8, 0.25, 0.5
+    Hd = exact.dense_hamiltonian(N, hatano_nelson_terms(N, g, delta))
+    assert np.abs(Hd - Hd.conj().T).max() > 0.1  # genuinely non-Hermitian
+    ev = np.linalg.eigvals(Hd)
+    want = ev[np.argmin(ev.real)]
+    e_h = np.linalg.eigvalsh(exact.dense_hamiltonian(N, hatano_nelson_terms(N, 0.0, delta)))[0]
+    assert abs(want.real - e_h) < 1e-10 and abs(want.imag) < 1e-10
+    Ws = mpo(hatano_nelson_terms(N, g, delta), N)
+    psi, e = dmrg(random_mps(N, 2, 16, np.random.default_rng(4)), Ws, maxsweeps=10,
+                  krylovdim=6, maxdim=16, cutoff=1e-14, ishermitian=False)
+    assert abs(e - want) < 1e-9
