@@ -76,7 +76,7 @@ def measured_traffic(chi):
     return None
 
 
-def cpu_baseline(budget_s=12.0):
+def cpu_baseline(budget_s=20.0):
     """FP64 CPU port (oracle, optimal contraction order, numpy/OpenBLAS on all host
     cores) on a bounded sample of the same workload: matvecs at chi = 512 and 1024."""
     from oracle.projmps import product_optimal
@@ -96,7 +96,7 @@ def cpu_baseline(budget_s=12.0):
             product_optimal(L, W, W, R, th)
             reps += 1
             el = time.perf_counter() - t0
-            if el > budget_s / 2 or reps >= 20:
+            if el > budget_s / 2 or reps >= 40:
                 break
         best = {"value": matvec_flops(chi) * reps / el / 1e12, "unit": "TFLOP/s", "cores": cores,
                 "kind": "port",
