@@ -271,6 +271,14 @@ int tnh_inner_oplist(tnh_ctx *ctx, int N, const tnh_buf *const *psi, int nterms,
 int tnh_inner_oplist_pair(tnh_ctx *ctx, int N, const tnh_buf *const *psi,
                           const tnh_buf *const *phi, int nterms, const int *first_site,
                           const int *range, const double *ops, double *out);
+/* inner(phi, O, psi) and inner(phi, psi) (dot), src/structures/mps/mpo.jl:181-217: the N
+ * FP64/C128 site tensors of the bra phi and of the ket psi (bonds may differ) and, unless W is
+ * NULL, the N tensors (D_l, d, d, D_r) of one MPO between them. *re, *im receive
+ * <phi|W|psi> (<phi|psi> without W), unnormalised. Several MPOs are folded into one first
+ * (tnh_mpo_mpo_product, exact with cutoff 0); an MPO left of the bra enters as its adjoint,
+ * as the reference rearranges its arguments (:190-200). */
+int tnh_inner_mpo(tnh_ctx *ctx, int N, const tnh_buf *const *phi, const tnh_buf *const *W,
+                  const tnh_buf *const *psi, double *re, double *im);
 
 /* <psi|H|psi> for an MPO H = W[0..N-1] (inner(psi, H, psi), src/structures/mps/mpo.jl:181-217,
  * no normalisation) into (energy_re, energy_im), and, when dW is not NULL, its gradient

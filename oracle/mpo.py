@@ -229,3 +229,37 @@ def trotter_mpo(terms, N, t, d=2, cutoff=0.0, maxdim=0, mindim=1, st_ops=OPS):
             O3[k - 1] = Id.reshape(1, d, d, 1).copy()
         O = mpo_mpo_product(O3, O, cutoff=cutoff, maxdim=maxdim, mindim=mindim)
     return O
+
+
+def adjoint(O):
+    """adjoint(O::GMPS), src/structures/mps/mpo.jl:32-39: conj with the physical indices
+    swapped, per site."""
+    return [np.conj(moveidx(w, 2, 3)) for w in O]
+
+
+def inner(*args):
+    """inner(args::GMPS...), src/structures/mps/mpo.jl:181-217: <bra| O_1 ... O_k |ket> for
+    site-tensor lists of states (a, s, b) and MPOs (w, s_out, s_in, w'); the bra is moved to
+    the front past any operators before it, each of which becomes its adjoint (:195-204)."""
+    args = [list(a) for a in args]
+    if len(args) < 2:
+        raise ValueError("There must be atleast 2 MPS arguments (rank 1).")
+    if any(len(a) != len(args[0]) for a in args):
+        raise ValueError("GMPS must share the same length.")
+    ranks = [np.ndim(a[0]) - 2 for a in args]
+    if sum(r == 1 for r in ranks) != 2:
+        raise ValueError("The inner product must have a braket structure.")
+    if ranks[-1] != 1:
+        raise ValueError("The final GMPS must be rank 1 (MPS).")
+    idx = next(j for j, r in enumerate(ranks) if r == 1)    # :195-199
+    for j in range(idx - 1, -1, -1):                         # :200-205
+        psi = args[j + 1]
+        O = adjoint(args[j])
+        args[j] = psi
+        args[j + 1] = O
+    prod = np.ones([1] * len(args), dtype=np.complex128)     # :208
+    for site in range(len(args[0])):                         # :209-214
+        prod = contract(prod, np.conj(args[0][site]), 1, 1)
+        for j in range(1, len(args)):
+            prod = contract(prod, args[j][site], [1, prod.ndim - 1], [1, 2])
+    return prod.reshape(-1)[0]                               # :216

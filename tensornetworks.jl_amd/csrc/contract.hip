@@ -630,48 +630,54 @@ size_t heff_scratch_bound(const tnh_buf *L, const tnh_buf *W1, const tnh_buf *W2
 
 // ------------------------------------------------------------ environments
 
-void env_left(tnh_ctx *ctx, const tnh_buf *Lp, const tnh_buf *A, const tnh_buf *W,
-              tnh_buf **Lout, int prec) {
+// L'[b,w',b'] = sum L[a,w,a'] conj(B[a,s,b]) W[w,s,t,w'] K[a',t,b'] with bra B and ket K (their
+// bonds may differ); buildleft!'s contraction (projmps.jl:50-66) between two states
+void env_left_pair(tnh_ctx *ctx, const tnh_buf *Lp, const tnh_buf *B, const tnh_buf *K,
+                   const tnh_buf *W, tnh_buf **Lout, int prec) {
   if (!Lp) Lp = edge_block(ctx);
   require_dims(Lp, 3, "L");
-  require_dims(A, 3, "A");
+  require_dims(B, 3, "A");
+  require_dims(K, 3, "A");
   require_dims(W, 4, "W");
-  const int64_t ca = A->dims[0], d = A->dims[1], cb = A->dims[2];
+  const int64_t ca = B->dims[0], d = B->dims[1], cb = B->dims[2];
+  const int64_t ka = K->dims[0], kb = K->dims[2];
   const int64_t Dl = W->dims[0], Dr = W->dims[3];
-  TNH_REQUIRE(W->dims[1] == d && W->dims[2] == d, TNH_ERR_SHAPE, "W physical dims != A's");
-  TNH_REQUIRE(Lp->dims[0] == ca && Lp->dims[2] == ca && Lp->dims[1] == Dl, TNH_ERR_SHAPE,
-              "L " + shape_str(Lp) + " does not match A " + shape_str(A) + " / W " +
+  TNH_REQUIRE(W->dims[1] == d && W->dims[2] == d && K->dims[1] == d, TNH_ERR_SHAPE,
+              "W physical dims != A's");
+  TNH_REQUIRE(Lp->dims[0] == ca && Lp->dims[2] == ka && Lp->dims[1] == Dl, TNH_ERR_SHAPE,
+              "L " + shape_str(Lp) + " does not match A " + shape_str(B) + " / W " +
                   shape_str(W));
   check_prec_operand(prec, Lp, "L");
-  check_prec_operand(prec, A, "A");
+  check_prec_operand(prec, B, "A");
+  check_prec_operand(prec, K, "A");
   TNH_REQUIRE(Dl * d <= kMaxIJ && d * Dr <= kMaxIJ, TNH_ERR_UNSUPPORTED, "D*d > 64");
   bool wcc = false;
   double *Wc = make_wc_env(ctx, W, true, 820, &wcc);
-  const bool xc = is_complex(Lp->dtype) || is_complex(A->dtype);
+  const bool xc = is_complex(Lp->dtype) || is_complex(K->dtype);
   const bool yc = xc || wcc;
-  const bool oc = yc || is_complex(A->dtype);
+  const bool oc = yc || is_complex(B->dtype);
   const int tX = inter_dtype(prec, xc), tY = inter_dtype(prec, yc);
-  const int64_t nx = ca * Dl * d * cb, ny = ca * d * Dr * cb;
+  const int64_t nx = ca * Dl * d * kb, ny = ca * d * Dr * kb;
   const size_t rs = real_size(tX);
   char *X = static_cast<char *>(ctx->workspace(821, nx * rs * (xc ? 2 : 1)));
   char *Y = static_cast<char *>(ctx->workspace(822, ny * rs * (yc ? 2 : 1)));
   const int odt = oc ? TNH_C128 : TNH_F64;  // environment blocks are kept in FP64
-  int64_t od[3] = {cb, Dr, cb};
+  int64_t od[3] = {cb, Dr, kb};
   buf_ensure(ctx, Lout, odt, 3, od);
 
-  // (1) X[(a,w),(t,b')] = L[(a,w),a'] A[a',(t,b')]
+  // (1) X[(a,w),(t,b')] = L[(a,w),a'] K[a',(t,b')]
   Gemm g1;
   g1.M = int(ca * Dl);
-  g1.N = int(d * cb);
-  g1.K = int(ca);
+  g1.N = int(d * kb);
+  g1.K = int(ka);
   g1.lda = ca * Dl;
-  g1.ldb = ca;
+  g1.ldb = ka;
   g1.ldc = ca * Dl;
-  cgemm(ctx, g1, cop(Lp), cop(A), X, xc ? X + nx * rs : nullptr, real_of(tX), prec);
+  cgemm(ctx, g1, cop(Lp), cop(K), X, xc ? X + nx * rs : nullptr, real_of(tX), prec);
   // (2) Y[a,(s,w'),b'] = X[a,(w,t),b'] Wl[(w,t),(s,w')]
   Small s;
   s.na = int(ca);
-  s.nb = int(cb);
+  s.nb = int(kb);
   s.I = int(Dl * d);
   s.J = int(d * Dr);
   s.in = X;
@@ -689,10 +695,10 @@ void env_left(tnh_ctx *ctx, const tnh_buf *Lp, const tnh_buf *A, const tnh_buf *
   for (int i = 0; i < s.I; ++i) s.offI[i] = ca * i;
   for (int j = 0; j < s.J; ++j) s.offJ[j] = ca * j;
   small_contract(ctx, s);
-  // (3) L'[b,(w',b')] = A[(a,s),b]^H Y[(a,s),(w',b')]
+  // (3) L'[b,(w',b')] = B[(a,s),b]^H Y[(a,s),(w',b')]
   Gemm g3;
   g3.M = int(cb);
-  g3.N = int(Dr * cb);
+  g3.N = int(Dr * kb);
   g3.K = int(ca * d);
   g3.transA = true;
   g3.lda = ca * d;
@@ -702,8 +708,13 @@ void env_left(tnh_ctx *ctx, const tnh_buf *Lp, const tnh_buf *A, const tnh_buf *
   y.re = Y;
   y.im = yc ? Y + ny * rs : nullptr;
   y.dtype = real_of(tY);
-  cgemm(ctx, g3, cop(A, true), y, (*Lout)->re(), is_complex((*Lout)->dtype) ? (*Lout)->im() : nullptr,
+  cgemm(ctx, g3, cop(B, true), y, (*Lout)->re(), is_complex((*Lout)->dtype) ? (*Lout)->im() : nullptr,
         real_of((*Lout)->dtype), prec);
+}
+
+void env_left(tnh_ctx *ctx, const tnh_buf *Lp, const tnh_buf *A, const tnh_buf *W,
+              tnh_buf **Lout, int prec) {
+  env_left_pair(ctx, Lp, A, A, W, Lout, prec);
 }
 
 void env_right(tnh_ctx *ctx, const tnh_buf *Rp, const tnh_buf *A, const tnh_buf *W,

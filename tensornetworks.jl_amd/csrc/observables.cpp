@@ -67,6 +67,56 @@ void apply_op_site(tnh_ctx *ctx, const tnh_buf *O, const tnh_buf *A, tnh_buf **o
 
 }  // namespace
 
+// inner(phi, W, psi) / inner(phi, psi) of src/structures/mps/mpo.jl:181-217: the left block
+// carried from site 1 to N (three-index with an MPO, the two-index overlap without), then the
+// single entry of the last block.
+extern "C" int tnh_inner_mpo(tnh_ctx *ctx, int N, const tnh_buf *const *phi,
+                             const tnh_buf *const *W, const tnh_buf *const *psi, double *re,
+                             double *im) {
+  if (!ctx) return TNH_ERR_ARG;
+  tnh_buf *L0 = nullptr, *L1 = nullptr, *one = nullptr;
+  int rc = tnh_guard(ctx, [&] {
+    TNH_REQUIRE(N >= 1 && phi && psi, TNH_ERR_ARG, "inner: bad arguments");
+    for (int i = 0; i < N; ++i) {
+      TNH_REQUIRE(phi[i] && phi[i]->ndim == 3 && psi[i] && psi[i]->ndim == 3, TNH_ERR_SHAPE,
+                  "inner: site tensors must be (chi_l, d, chi_r)");
+      TNH_REQUIRE(is_double(phi[i]->dtype) && is_double(psi[i]->dtype), TNH_ERR_ARG,
+                  "inner: FP64 site tensors");
+      TNH_REQUIRE(phi[i]->dims[1] == psi[i]->dims[1], TNH_ERR_SHAPE,
+                  "GMPS must share the same physical dim.");
+      if (W) TNH_REQUIRE(W[i] && W[i]->ndim == 4, TNH_ERR_SHAPE, "inner: MPO tensors must be (D_l, d, d, D_r)");
+    }
+    TNH_REQUIRE(phi[0]->dims[0] == 1 && psi[0]->dims[0] == 1 && phi[N - 1]->dims[2] == 1 &&
+                    psi[N - 1]->dims[2] == 1,
+                TNH_ERR_SHAPE, "inner: edge bonds must be 1");
+    if (W)
+      TNH_REQUIRE(W[0]->dims[0] == 1 && W[N - 1]->dims[3] == 1, TNH_ERR_SHAPE,
+                  "inner: MPO edge bonds must be 1");
+    const tnh_buf *prev = nullptr;  // nullptr: the ones block of the chain's left end
+    if (!W) {
+      const int64_t od[2] = {1, 1};
+      one = buf_new(ctx, TNH_F64, 2, od);
+      fill(ctx, one->ptr, TNH_F64, 1, 1.0);
+      prev = one;
+    }
+    for (int i = 0; i < N; ++i) {
+      if (W)
+        env_left_pair(ctx, prev, phi[i], psi[i], W[i], &L1, TNH_PREC_F64);
+      else
+        proj_env_left(ctx, prev, phi[i], psi[i], &L1);
+      std::swap(L0, L1);
+      prev = L0;
+    }
+    TNH_REQUIRE(L0->numel == 1, TNH_ERR_SHAPE, "inner: open right bond");
+    double h[2] = {0, 0};
+    to_host(ctx, h, L0->ptr, is_complex(L0->dtype) ? 16 : 8);  // planar: re then im
+    if (re) *re = h[0];
+    if (im) *im = is_complex(L0->dtype) ? h[1] : 0.0;
+  });
+  for (auto *b : {L0, L1, one}) buf_delete(ctx, b);
+  return rc;
+}
+
 extern "C" int tnh_inner_oplist_pair(tnh_ctx *ctx, int N, const tnh_buf *const *psi,
                                      const tnh_buf *const *phi, int nterms, const int *first_site,
                                      const int *range, const double *ops, double *out) {

@@ -241,6 +241,9 @@ size_t heff_scratch_bound(const tnh_buf *L, const tnh_buf *W1, const tnh_buf *W2
 void noalloc_init(int device);
 void env_left(tnh_ctx *ctx, const tnh_buf *Lp, const tnh_buf *A, const tnh_buf *W,
               tnh_buf **Lout, int prec);
+// the same with a bra B and a ket K of possibly different bonds: L (b_B, w, b_K)
+void env_left_pair(tnh_ctx *ctx, const tnh_buf *Lp, const tnh_buf *B, const tnh_buf *K,
+                   const tnh_buf *W, tnh_buf **Lout, int prec);
 void env_right(tnh_ctx *ctx, const tnh_buf *Rp, const tnh_buf *A, const tnh_buf *W,
                tnh_buf **Rout, int prec);
 // one-site H_eff matvec (nsites = 1 DMRG): out = coeff H_eff x

@@ -22,7 +22,7 @@ using TensorNetworks
 import TensorNetworks: AbstractProjMPS, GMPS, GateList, OpList, Sitetypes, product,
                        buildleft!, buildright!, calculate, movecenter!, dmrg, tebd, svd,
                        replacesites!, applygate!, applygates!, measure!, checkdone!,
-                       trotterize, maxbonddim, op
+                       trotterize, maxbonddim, op, MPO, applyMPO, inner
 
 const libtnhip = get(ENV, "TNHIP_LIB", joinpath(@__DIR__, "..", "..", "..", "tnhip", "libtnhip.so"))
 
@@ -821,6 +821,105 @@ function coupling_energy_grad(psi::GMPS, st::Sitetypes, ol::OpList, c::AbstractV
     E, dW = mpo_energy_grad(psi, W; block = block, ctx = ctx)
     g = [sum(dW[s][r, :, :, k] .* O) for (s, r, k, O) in slots]
     return real(E), real.(g)
+end
+
+# --------------------------------------------------------------- MPOs and products
+"""
+    MPO(GPU(), st::Sitetypes, H::OpList; cutoff=1e-15, maxdim=0, mindim=1) -> GMPS
+
+MPO(st, H; kwargs), src/structures/mps/mpo.jl:323-459: the finite-state automaton built by
+the library on the host, then its SVD compression sweeps on the device (tnh_mpo_from_oplist).
+"""
+function MPO(g::GPU, st::Sitetypes, H::OpList; cutoff::Real = 1e-15, maxdim::Int = 0,
+             mindim::Int = 1)
+    N, d, nt = H.length, st.dim, length(H.sites)
+    first, rng, mats = _oplist_payload(st, H)
+    coeffs = Float64[]
+    for c in H.coeffs
+        push!(coeffs, real(c), imag(c))
+    end
+    out = fill(C_NULL, N)
+    GC.@preserve first rng mats coeffs out check(ccall((:tnh_mpo_from_oplist, libtnhip), Cint,
+            (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Cint}, Ptr{Cint}, Ptr{Float64}, Ptr{Float64}, Cdouble,
+             Cint, Cint, Ptr{Ptr{Cvoid}}),
+            g.ctx.h, N, d, nt, first, rng, Ptr{Float64}(pointer(mats)), coeffs, cutoff, maxdim,
+            mindim, out), g.ctx.h, "tnh_mpo_from_oplist")
+    ts = Array{ComplexF64}[to_host(owned(g.ctx, h)) for h in out]
+    return GMPS(2, d, ts, 1)
+end
+
+"""
+    applyMPO(GPU(), O, psi; cutoff=0, maxdim=0, mindim=1), applyMPO(GPU(), psi, O; …),
+    applyMPO(GPU(), O1, O2; …) -> GMPS
+
+applyMPO, src/structures/mps/mpo.jl:105-169: O |psi> (tnh_mpo_apply), <psi| O as
+adjoint(O) |psi> (:112), or O1 O2 (tnh_mpo_mpo_product), each with the truncating centre
+move to site 1.
+"""
+function applyMPO(g::GPU, arg1::GMPS, arg2::GMPS; cutoff::Real = 0.0, maxdim::Int = 0,
+                  mindim::Int = 1)
+    r1, r2 = TensorNetworks.rank(arg1), TensorNetworks.rank(arg2)
+    length(arg1) == length(arg2) || error("GMPS must share the same length.")
+    TensorNetworks.dim(arg1) == TensorNetworks.dim(arg2) || error("GMPS must share the same physical dims.")
+    r1 == 1 && r2 == 2 && return applyMPO(g, TensorNetworks.adjoint(arg2), arg1; cutoff = cutoff, maxdim = maxdim,
+                                          mindim = mindim)
+    r1 == 2 || error("applyMPO: an MPO (rank 2) must come first")
+    N = length(arg1)
+    a = [upload(g.ctx, arg1[i]) for i = 1:N]
+    b = [upload(g.ctx, arg2[i]) for i = 1:N]
+    ha = [t.h for t in a]; hb = [t.h for t in b]
+    out = fill(C_NULL, N)
+    fn = r2 == 1 ? :tnh_mpo_apply : :tnh_mpo_mpo_product
+    if r2 == 1
+        GC.@preserve a b ha hb out check(ccall((:tnh_mpo_apply, libtnhip), Cint,
+                (Ptr{Cvoid}, Cint, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Cdouble, Cint, Cint, Ptr{Ptr{Cvoid}}),
+                g.ctx.h, N, ha, hb, cutoff, maxdim, mindim, out), g.ctx.h, String(fn))
+    else
+        GC.@preserve a b ha hb out check(ccall((:tnh_mpo_mpo_product, libtnhip), Cint,
+                (Ptr{Cvoid}, Cint, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Cdouble, Cint, Cint, Ptr{Ptr{Cvoid}}),
+                g.ctx.h, N, ha, hb, cutoff, maxdim, mindim, out), g.ctx.h, String(fn))
+    end
+    ts = Array{ComplexF64}[to_host(owned(g.ctx, h)) for h in out]
+    return GMPS(r2, TensorNetworks.dim(arg1), ts, 1)
+end
+
+"""
+    inner(GPU(), args::GMPS...) -> ComplexF64
+
+inner(args::GMPS...), src/structures/mps/mpo.jl:181-217: <bra| O_1 ... O_k |ket>, the bra
+moved to the front past any operators before it (each becoming its adjoint, :195-204), the
+operators folded into one by exact products (cutoff 0) and the chain contracted on the
+device (tnh_inner_mpo). inner(GPU(), phi, psi) is the overlap.
+"""
+function inner(g::GPU, args::GMPS...)
+    length(args) < 2 && error("There must be atleast 2 MPS arguments (rank 1).")
+    all(TensorNetworks.dim(a) == TensorNetworks.dim(args[1]) for a in args) ||
+        error("GMPS must share the same physical dim.")
+    all(length(a) == length(args[1]) for a in args) || error("GMPS must share the same length.")
+    ranks = [TensorNetworks.rank(a) for a in args]
+    count(==(1), ranks) != 2 && error("The inner product must have a braket structure.")
+    ranks[end] != 1 && error("The final GMPS must be rank 1 (MPS).")
+    objs = collect(GMPS, args)
+    idx = findfirst(==(1), ranks)
+    for j = idx-1:-1:1
+        psi = objs[j+1]
+        objs[j+1] = TensorNetworks.adjoint(objs[j])
+        objs[j] = psi
+    end
+    bra, ops, ket = objs[1], objs[2:end-1], objs[end]
+    W = isempty(ops) ? nothing : ops[1]
+    for O in ops[2:end]
+        W = applyMPO(g, W, O)  # exact: cutoff 0
+    end
+    N = length(bra)
+    db = [upload(g.ctx, bra[i]) for i = 1:N]; dk = [upload(g.ctx, ket[i]) for i = 1:N]
+    dw = W === nothing ? DeviceTensor[] : [upload(g.ctx, W[i]) for i = 1:N]
+    hb = [t.h for t in db]; hk = [t.h for t in dk]; hw = [t.h for t in dw]
+    re = Ref{Cdouble}(0); im = Ref{Cdouble}(0)
+    GC.@preserve db dk dw hb hk hw check(ccall((:tnh_inner_mpo, libtnhip), Cint,
+            (Ptr{Cvoid}, Cint, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Cdouble}, Ptr{Cdouble}),
+            g.ctx.h, N, hb, W === nothing ? C_NULL : pointer(hw), hk, re, im), g.ctx.h, "tnh_inner_mpo")
+    return complex(re[], im[])
 end
 
 # ------------------------------- many QJMC trajectories (tnh_qjmc_run(_batched))

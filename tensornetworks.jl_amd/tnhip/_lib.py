@@ -119,6 +119,7 @@ SIGNATURES = {
     "tnh_replacesites_sites": (C.c_int, [ctx_p, C.c_int, buf_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(buf_p), C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "tnh_inner_oplist": (C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "tnh_inner_oplist_pair": (C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.POINTER(buf_p), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "tnh_inner_mpo": (C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.POINTER(buf_p), C.POINTER(buf_p), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "tnh_mpo_from_oplist": (C.c_int, [ctx_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double, C.c_int, C.c_int, C.POINTER(buf_p)]),
     "tnh_mpo_compress": (C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.c_double, C.c_int, C.c_int]),
     "tnh_mpo_apply": (C.c_int, [ctx_p, C.c_int, C.POINTER(buf_p), C.POINTER(buf_p), C.c_double, C.c_int, C.c_int, C.POINTER(buf_p)]),

@@ -157,3 +157,57 @@ def trotterMPO(st: Sitetypes, H: OpList, t, cutoff=0.0, maxdim=0, mindim=1, ctx:
             O3[k - 1] = Id.reshape(1, d, d, 1, order="F")
         O = MPOMPOProduct(O3, O, cutoff=cutoff, maxdim=maxdim, mindim=mindim, ctx=ctx)
     return GMPS(2, d, [w.numpy() for w in O], 1)
+
+
+def adjoint(O: GMPS) -> GMPS:
+    """adjoint(O::GMPS), mpo.jl:32-39: conj with the physical indices swapped."""
+    if O.rank != 2:
+        raise ValueError("The generalised MPS must be of rank 2 (an MPO).")
+    return GMPS(2, O.dim, [np.conj(np.transpose(np.asarray(w), (0, 2, 1, 3))) for w in O.tensors],
+                O.center)
+
+
+def inner(*args: GMPS, ctx: Context | None = None) -> complex:
+    """inner(args::GMPS...), mpo.jl:181-217, on the GPU: <bra| O_1 ... O_k |ket> for rank-1
+    (states) and rank-2 (MPOs) arguments, with exactly two states, the last one the ket. The
+    bra moves to the front past any operators before it, each becoming its adjoint
+    (:195-204); the operators between bra and ket are folded into one MPO by exact
+    MPOMPOProduct (cutoff 0) and the chain is contracted by tnh_inner_mpo."""
+    ctx = ctx or Context.default()
+    if len(args) < 2:
+        raise ValueError("There must be atleast 2 MPS arguments (rank 1).")
+    if any(a.dim != args[0].dim for a in args):
+        raise ValueError("GMPS must share the same physical dim.")
+    if any(len(a) != len(args[0]) for a in args):
+        raise ValueError("GMPS must share the same length.")
+    ranks = [a.rank for a in args]
+    if sum(r == 1 for r in ranks) != 2:
+        raise ValueError("The inner product must have a braket structure.")
+    if ranks[-1] != 1:
+        raise ValueError("The final GMPS must be rank 1 (MPS).")
+    args = list(args)
+    idx = ranks.index(1)
+    for j in range(idx - 1, -1, -1):
+        psi, O = args[j + 1], adjoint(args[j])
+        args[j], args[j + 1] = psi, O
+    bra, ops, ket = args[0], args[1:-1], args[-1]
+
+    def up(ts):
+        return [Tensor.from_numpy(np.asarray(t, dtype=np.result_type(t, np.float64)), ctx=ctx) for t in ts]
+
+    dbra, dket = up(bra.tensors), up(ket.tensors)
+    W = None
+    if ops:
+        W = up(ops[0].tensors)
+        for O in ops[1:]:
+            W = MPOMPOProduct(W, O, cutoff=0.0, ctx=ctx)
+    re, im = C.c_double(), C.c_double()
+    N = len(bra)
+    ctx.call("tnh_inner_mpo", N, _handles(dbra), _handles(W) if W is not None else None,
+             _handles(dket), C.byref(re), C.byref(im))
+    return complex(re.value, im.value)
+
+
+def dot(phi: GMPS, psi: GMPS, ctx: Context | None = None) -> complex:
+    """dot(phi, psi) = inner(phi, psi), mpo.jl:218."""
+    return inner(phi, psi, ctx=ctx)

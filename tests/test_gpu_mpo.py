@@ -131,3 +131,46 @@ def test_dmrg_with_compressed_mpo(ctx):
     psi = randomMPS(2, N, 8, np.random.default_rng(7))
     psi, e = dmrg(psi, M, maxdim=64, cutoff=1e-12, maxsweeps=12, krylovdim=4, verbose=False)
     assert abs(e - tfim_critical_ground_energy(N)) / N <= 1e-10
+
+
+def test_inner_chains_match_oracle_and_dense(ctx):
+    """inner(args::GMPS...) (mpo.jl:181-217) on the device (tnh_inner_mpo, with several MPOs
+    folded by exact MPOMPOProduct): <phi|psi>, <phi|O|psi>, <phi|O1 O2|psi>, and operators
+    before the bra (their adjoints), against the oracle's restatement and dense algebra.
+    Complex states of different bond dimensions, non-Hermitian complex MPOs; 1e-11 relative."""
+    from tnhip import GMPS
+    from tnhip.mpo import dot, inner
+    from oracle.exact import mpo_to_dense
+    from oracle.mpo import inner as oinner
+    from oracle.mps import MPS, random_mps, to_dense
+
+    rng = np.random.default_rng(43)
+    N = 6
+
+    def cmps(chi):
+        p = random_mps(N, 2, chi, rng)
+        return [np.asfortranarray(t + 1j * rng.standard_normal(t.shape)) for t in p.tensors]
+
+    def cmpo(D):
+        out = []
+        for i in range(N):
+            shape = (1 if i == 0 else D, 2, 2, 1 if i == N - 1 else D)
+            out.append(np.asfortranarray(rng.standard_normal(shape) + 1j * rng.standard_normal(shape)))
+        return out
+
+    phi, psi = cmps(3), cmps(5)
+    O1, O2 = cmpo(2), cmpo(3)
+    P, Q = GMPS(1, 2, phi, 0), GMPS(1, 2, psi, 0)
+    A, B = GMPS(2, 2, O1, 0), GMPS(2, 2, O2, 0)
+    vphi, vpsi = to_dense(MPS(phi)), to_dense(MPS(psi))
+    M1, M2 = mpo_to_dense(O1), mpo_to_dense(O2)
+    cases = [((P, Q), (phi, psi), np.vdot(vphi, vpsi)),
+             ((P, A, Q), (phi, O1, psi), np.vdot(vphi, M1 @ vpsi)),
+             ((P, A, B, Q), (phi, O1, O2, psi), np.vdot(vphi, M1 @ (M2 @ vpsi))),
+             ((A, P, Q), (O1, phi, psi), np.vdot(vphi, M1.conj().T @ vpsi)),
+             ((A, B, P, Q), (O1, O2, phi, psi), np.vdot(vphi, M1.conj().T @ (M2.conj().T @ vpsi)))]
+    for gargs, oargs, want in cases:
+        got = inner(*gargs)
+        assert abs(got - want) <= 1e-11 * abs(want), (len(gargs), got, want)
+        assert abs(got - oinner(*oargs)) <= 1e-11 * abs(want)
+    assert abs(dot(P, Q) - np.vdot(vphi, vpsi)) <= 1e-11 * abs(np.vdot(vphi, vpsi))

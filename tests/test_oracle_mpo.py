@@ -148,3 +148,43 @@ def test_trotter_mpo_is_the_ordered_product_of_term_exponentials(t):
     assert np.allclose(mpo_to_dense(O), ref, atol=1e-11)
     H = dense_hamiltonian(N, terms)
     assert np.linalg.norm(ref - sla.expm(t * H)) > 1e-4
+
+
+def test_inner_chains_match_dense():
+    """inner(args::GMPS...) (mpo.jl:181-217) against dense algebra: <phi|psi>,
+    <phi|O|psi>, <phi|O1 O2|psi>, and an operator before the bra, which the reference moves
+    past it as its adjoint: inner(O, phi, psi) = <phi|O^H|psi>, inner(O1, O2, phi, psi) =
+    <phi|O1^H O2^H|psi>. Complex states of different bond dimensions, non-Hermitian O."""
+    from oracle.exact import mpo_to_dense
+    from oracle.mpo import inner
+    from oracle.mps import random_mps, to_dense
+
+    rng = np.random.default_rng(41)
+    N = 5
+
+    def cmps(chi):
+        p = random_mps(N, 2, chi, rng)
+        return [t + 1j * rng.standard_normal(t.shape) for t in p.tensors]
+
+    def cmpo(D):
+        ts = []
+        for i in range(N):
+            shape = (1 if i == 0 else D, 2, 2, 1 if i == N - 1 else D)
+            ts.append(rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+        return ts
+
+    from oracle.mps import MPS
+
+    phi, psi = cmps(3), cmps(4)
+    O1, O2 = cmpo(2), cmpo(3)
+    vphi, vpsi = to_dense(MPS(phi)), to_dense(MPS(psi))
+    M1, M2 = mpo_to_dense(O1), mpo_to_dense(O2)
+    assert abs(inner(phi, psi) - np.vdot(vphi, vpsi)) <= 1e-10 * abs(np.vdot(vphi, vpsi))
+    ref = np.vdot(vphi, M1 @ vpsi)
+    assert abs(inner(phi, O1, psi) - ref) <= 1e-10 * abs(ref)
+    ref = np.vdot(vphi, M1 @ (M2 @ vpsi))
+    assert abs(inner(phi, O1, O2, psi) - ref) <= 1e-10 * abs(ref)
+    ref = np.vdot(vphi, M1.conj().T @ vpsi)
+    assert abs(inner(O1, phi, psi) - ref) <= 1e-10 * abs(ref)
+    ref = np.vdot(vphi, M1.conj().T @ (M2.conj().T @ vpsi))
+    assert abs(inner(O1, O2, phi, psi) - ref) <= 1e-10 * abs(ref)
