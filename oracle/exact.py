@@ -60,6 +60,18 @@ def tfim_bdg_ground_energy(J, h):
     return -float(s.sum())
 
 
+def tfim_bdg_first_excited_energy(J, h):
+    """First excited energy of the same open chain: one quasiparticle, E1 = E0 + 2 s_min
+    (H = sum_k 2 s_k (n_k - 1/2)); checked against ED in the tests."""
+    h = np.asarray(h, dtype=float)
+    J = np.asarray(J, dtype=float)
+    N = h.size
+    B = np.diag(h)
+    B[np.arange(N - 1), np.arange(1, N)] = J
+    s = np.linalg.svd(B, compute_uv=False)
+    return -float(s.sum()) + 2.0 * float(s.min())
+
+
 def tfim_critical_ground_energy(N):
     """Closed form E0(N) = 1 - csc(pi / (4N + 2)) for J = h = 1."""
     return 1.0 - 1.0 / np.sin(np.pi / (4 * N + 2))

@@ -157,6 +157,32 @@ def test_heff_matvec_and_split_full_size(ctx, chi):
 # --------------------------------------------------------- critical TFIM N = 100
 
 
+def test_tfim_n100_first_excited_state_vs_free_fermions(ctx):
+    """The config-3 workload with a rank-1 penalty (dmrg(psi, H, phi0; coeffs = [1, 10]),
+    dmrg.jl:144-145): the converged N = 100 ground state as phi0, then H + 10 |phi0><phi0|
+    from a fresh start, against the free-fermion first excited energy E1 = E0 + 2 s_min (the
+    closed form's one-quasiparticle state, pinned to ED at small N); the two states are
+    orthogonal (inner(phi0, psi1) on the device, mpo.jl:181-217)."""
+    from tnhip import GMPS, MPO, dmrg, spinhalf
+    from tnhip.mpo import inner
+    from oracle.exact import tfim_bdg_first_excited_energy, tfim_critical_ground_energy
+    from oracle.mps import random_mps
+
+    N = 100
+    E0 = tfim_critical_ground_energy(N)
+    E1 = tfim_bdg_first_excited_energy(np.ones(N - 1), np.ones(N))
+    M = MPO(spinhalf(), tfim_oplist(N, np.ones(N - 1), np.ones(N)))
+    kw = dict(maxdim=2048, cutoff=1e-12, maxsweeps=24, krylovdim=3, verbose=False)
+    psi0, e0 = dmrg(GMPS(1, 2, random_mps(N, 2, 16, np.random.default_rng(0)).tensors, 1), M, **kw)
+    psi1, e1 = dmrg(GMPS(1, 2, random_mps(N, 2, 16, np.random.default_rng(1)).tensors, 1), M, psi0,
+                    coeffs=[1.0, 10.0], **kw)
+    print(f"N=100 TFIM E0={e0:.14f} |dE0|/N={abs(e0 - E0) / N:.2e} E1={e1:.14f} "
+          f"|dE1|/N={abs(e1 - E1) / N:.2e} chi={psi1.maxbonddim()}", flush=True)
+    assert abs(e0 - E0) / N <= 1e-10
+    assert abs(e1 - E1) / N <= 1e-10
+    assert abs(inner(psi0, psi1)) <= 1e-5
+
+
 def test_tfim_n100_precisions_vs_closed_form_and_cpu(ctx):
     """The N = 100 critical TFIM from one random chi = 16 start (seed 0), cutoff 1e-12,
     in FP64, FP32-MFMA and split-BF16x3 (each finishing with FP64 passes), against the

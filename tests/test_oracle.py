@@ -277,6 +277,14 @@ def test_tfim_bdg_inhomogeneous(rng):
     assert abs(np.linalg.eigvalsh(H)[0] - exact.tfim_bdg_ground_energy(J, h)) < 1e-12
 
 
+@pytest.mark.parametrize("N,h", [(8, 0.9), (10, 1.0), (9, 1.3)])
+def test_tfim_first_excited_energy_vs_ed(N, h):
+    terms = ([(["z", "z"], [i, i + 1], -1.0) for i in range(1, N)]
+             + [(["x"], [i], -h) for i in range(1, N + 1)])
+    ev = np.linalg.eigvalsh(exact.dense_hamiltonian(N, terms))
+    assert abs(exact.tfim_bdg_first_excited_energy(np.ones(N - 1), h * np.ones(N)) - ev[1]) < 1e-12
+
+
 def test_tfim_n100_closed_form_value():
     assert abs(exact.tfim_critical_ground_energy(100) - (-126.961876739680733)) < 1e-9
 
