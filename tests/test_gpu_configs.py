@@ -157,6 +157,30 @@ def test_heff_matvec_and_split_full_size(ctx, chi):
 # --------------------------------------------------------- critical TFIM N = 100
 
 
+def test_tfim_n100_one_site_passes_after_two_site_growth(ctx):
+    """The config-3 workload finished by one-site passes (dmrg(...; nsites=1), dmrg.jl:35-63
+    with replacesites!'s one-site branch): the two-site run grows the bonds, then four
+    one-site passes keep every bond, do not raise the energy by more than rounding, and stay
+    within 1e-10 per site of the closed form."""
+    from tnhip import GMPS, MPO, dmrg, spinhalf
+    from oracle.exact import tfim_critical_ground_energy
+    from oracle.mps import random_mps
+
+    N = 100
+    E0 = tfim_critical_ground_energy(N)
+    M = MPO(spinhalf(), tfim_oplist(N, np.ones(N - 1), np.ones(N)))
+    psi, e2 = dmrg(GMPS(1, 2, random_mps(N, 2, 16, np.random.default_rng(0)).tensors, 1), M,
+                   maxdim=2048, cutoff=1e-12, maxsweeps=24, krylovdim=3, verbose=False)
+    bonds = [t.shape for t in psi.tensors]
+    hist = []
+    psi, e1 = dmrg(psi, M, nsites=1, maxsweeps=4, minsweeps=4, krylovdim=3, verbose=False,
+                   history=hist)
+    print(f"N=100 TFIM two-site E={e2:.14f}, one-site E={e1:.14f}", flush=True)
+    assert [t.shape for t in psi.tensors] == bonds
+    assert e1 <= e2 + 1e-11 * N
+    assert abs(e1 - E0) / N <= 1e-10
+
+
 def test_tfim_n100_first_excited_state_vs_free_fermions(ctx):
     """The config-3 workload with a rank-1 penalty (dmrg(psi, H, phi0; coeffs = [1, 10]),
     dmrg.jl:144-145): the converged N = 100 ground state as phi0, then H + 10 |phi0><phi0|
