@@ -16,6 +16,11 @@ computed independently from a QR basis of the monomial Krylov vectors, with
 k + (c - 1)(k - 1) matvecs; the converged pair is pinned by dense diagonalisation.
 Parity unpinned only for KrylovKit itself: that its version keeps exactly one vector per
 restart for these parameters cannot be checked without running it.
+
+eigsolve_arnoldi is the non-Hermitian counterpart (ishermitian = false, the eigenvalue of
+smallest real part): pinned the same way (Rayleigh-Ritz on K_k(A, y_{c-1}) cycle by cycle,
+the converged pair against dense eigenvalues, the DMRG built on it against ED of the
+Hatano-Nelson chain), with the same unpinned caveat about KrylovKit's own restart.
 """
 from __future__ import annotations
 
