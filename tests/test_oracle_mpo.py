@@ -188,3 +188,9 @@ def test_inner_chains_match_dense():
     assert abs(inner(O1, phi, psi) - ref) <= 1e-10 * abs(ref)
     ref = np.vdot(vphi, M1.conj().T @ (M2.conj().T @ vpsi))
     assert abs(inner(O1, O2, phi, psi) - ref) <= 1e-10 * abs(ref)
+    ref = np.vdot(vphi, M1.conj().T @ (M2 @ vpsi))  # one operator on each side of the bra
+    assert abs(inner(O1, phi, O2, psi) - ref) <= 1e-10 * abs(ref)
+    with pytest.raises(ValueError):
+        inner(phi, O1, psi, O2)  # the last argument must be the ket
+    with pytest.raises(ValueError):
+        inner(phi, O1)
