@@ -173,7 +173,6 @@ def inner(*args: GMPS, ctx: Context | None = None) -> complex:
     bra moves to the front past any operators before it, each becoming its adjoint
     (:195-204); the operators between bra and ket are folded into one MPO by exact
     MPOMPOProduct (cutoff 0) and the chain is contracted by tnh_inner_mpo."""
-    ctx = ctx or Context.default()
     if len(args) < 2:
         raise ValueError("There must be atleast 2 MPS arguments (rank 1).")
     if any(a.dim != args[0].dim for a in args):
@@ -185,6 +184,7 @@ def inner(*args: GMPS, ctx: Context | None = None) -> complex:
         raise ValueError("The inner product must have a braket structure.")
     if ranks[-1] != 1:
         raise ValueError("The final GMPS must be rank 1 (MPS).")
+    ctx = ctx or Context.default()
     args = list(args)
     idx = ranks.index(1)
     for j in range(idx - 1, -1, -1):
